@@ -1,0 +1,68 @@
+"""Build the native parts in-tree.
+
+    python dendro-5.01_amd/build.py            # libdgr_bssn.so (gfx950)
+    python dendro-5.01_amd/build.py --all      # + CPU oracle and test harness
+
+Everything is compiled with explicit hipcc / g++ command lines (no JIT
+cache), so the built files travel with the source tree.
+"""
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB = os.path.join(PKG, "lib")
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+# -packed-fp32-ops off: packed FP32 forms are kept out of every HIP build.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-I" + os.path.join(ROOT, "include")]
+
+
+def run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.check_call(cmd, cwd=cwd)
+
+
+def build_hip():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libdgr_bssn.so")
+    tmp = out + ".tmp"
+    run([HIPCC] + HIP_FLAGS + ["-o", tmp, os.path.join(PKG, "csrc", "dgr_bssn.hip")])
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle(with_ref=True):
+    odir = os.path.join(ROOT, "oracle")
+    run(["make", "-s", "all"], cwd=odir)
+    run(["make", "-s", "all", "ORACLE_FAST=1"], cwd=odir)
+    ref = os.environ.get("DGR_REFERENCE", "/root/reference")
+    if with_ref and os.path.isdir(ref):
+        run(["make", "-s", "ref", "REF=" + ref], cwd=odir)
+
+
+def build_test_harness():
+    src = os.path.join(ROOT, "tests", "native", "staged_host.cpp")
+    out = os.path.join(ROOT, "tests", "native", "libstaged_host.so")
+    run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-fPIC", "-shared", "-o", out, src])
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--all", action="store_true")
+    ap.add_argument("--no-hip", action="store_true")
+    a = ap.parse_args(argv)
+    if not a.no_hip:
+        build_hip()
+    if a.all:
+        build_oracle()
+        build_test_harness()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

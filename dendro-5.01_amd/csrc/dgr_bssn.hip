@@ -1,0 +1,707 @@
+/*
+ * MI355X (gfx950) BSSN right-hand side on ghost-padded blocks.
+ *
+ * Reference behaviour: bssnrhs (BSSN_GR/src/rhs.cpp:86-392) per block:
+ * deriv42 first / second / mixed derivatives (BSSN_GR/src/derivs.cpp),
+ * upwinded deriv42adv, the generated BSSN equations, Sommerfeld terms on
+ * flagged faces (rhs.cpp:838-982), then Kreiss-Oliger dissipation
+ * (ko_deriv42, rhs.cpp:331-373) on every interior point.
+ *
+ * GPU design (one workgroup per tile of at most 7x7x7 interior points; a
+ * 13^3 block is exactly one tile):
+ *   - the 24 state values at each thread's point are read once from HBM;
+ *   - the 24 padded variables are staged through LDS one at a time; the
+ *     global loads of the next variable are issued into registers before
+ *     the current one is differentiated, so HBM latency overlaps compute;
+ *   - for the 11 variables with second derivatives, the x-derivative over
+ *     the (y,z)-widened tile and the y-derivative over the z-widened tile
+ *     are written to LDS and differentiated again, which is exactly the
+ *     reference's mixed derivative by composition (bssnrhs_derivs.h);
+ *   - every stencil is evaluated with FP contraction off and the
+ *     reference's operation order, so derivative values are bit-identical
+ *     to the reference's (this is where the O(1) background cancels);
+ *   - each variable's derivatives are consumed at once by the staged
+ *     equations (bssn_staged.h) so no derivative array ever exists;
+ *   - Kreiss-Oliger and Sommerfeld terms are folded into one per-variable
+ *     register that is added after the equations, as the reference does.
+ * All arithmetic is FP64 on the vector ALUs.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/dgr_bssn.h"
+#include "bssn_staged.h"
+
+namespace {
+
+constexpr int TX = 7, TY = 7, TZ = 7;         // interior tile
+constexpr int NPT = TX * TY * TZ;              // 343 points
+constexpr int NT = 384;                        // 6 waves
+constexpr int SX = TX + 6, SY = TY + 6, SZ = TZ + 6;  // staged region
+constexpr int NSU = SX * SY * SZ;              // 2197
+constexpr int NPF = (NSU + NT - 1) / NT;       // prefetch registers per thread
+constexpr int GXY = TY + 4, GXZ = TZ + 4;      // widened x-derivative plane
+constexpr int NGX = TX * GXY * GXZ;            // 847
+constexpr int NGY = TX * TY * GXZ;             // 539
+
+// Per-axis stencil constants, computed on the host exactly as derivs.cpp
+// computes them (IEEE double, no contraction).
+struct AxisC {
+  double ih2, ih12, isq, isq12, pre, smr1, smr2, smr3;
+  int e;
+  int lf, hf;
+  int pad;
+};
+
+struct DevBlock {
+  uint64_t offset;
+  uint32_t sz[3];
+  uint32_t bflag;
+  double pmin[3];
+  double h[3];
+  AxisC ax[3];
+};
+
+struct DevTile {
+  uint32_t block;
+  uint16_t i0, j0, k0;  // first interior point of the tile (block coords)
+  uint8_t nx, ny, nz, pad;
+};
+
+struct VarPtrs {
+  const double* v[24];
+};
+struct OutPtrs {
+  double* v[24];
+};
+// The 48 array pointers live in device memory and are loaded where used,
+// which keeps them out of the scalar register file.
+struct IOPtrs {
+  VarPtrs u;
+  OutPtrs r;
+};
+
+struct DevParams {
+  double lambda[4];
+  double lambda_f[2];
+  double eta_const, eta_r0, eta_damping_exp, ko_sigma;
+};
+
+// ---------------------------------------------------------------------------
+// Stencils.  The value at axis coordinate q' of the current line is
+// A[o + q'*s]: `s` is the stride along the axis, q the coordinate of the
+// output point, e = sz-3, lf/hf the face flags of the axis.  Formulas and operation
+// order follow BSSN_GR/src/derivs.cpp (deriv42_x :29-65, deriv42_xx
+// :265-316, deriv42adv_x :513-622, ko_deriv42_x :937-1060).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double d1_42(const double* A, int o, int s, int q, int e, bool lf, bool hf,
+                                        double ih2, double ih12) {
+#pragma clang fp contract(off)
+  if (hf && q == e - 1) return (A[o + (e - 3) * s] - 4.0 * A[o + (e - 2) * s] + 3.0 * A[o + (e - 1) * s]) * ih2;
+  if (hf && q == e - 2) return (-A[o + (e - 3) * s] + A[o + (e - 1) * s]) * ih2;
+  if (lf && q == 4) return (-A[o + 3 * s] + A[o + 5 * s]) * ih2;
+  if (lf && q == 3) return (-3.0 * A[o + 3 * s] + 4.0 * A[o + 4 * s] - A[o + 5 * s]) * ih2;
+  return (A[o + (q - 2) * s] - 8.0 * A[o + (q - 1) * s] + 8.0 * A[o + (q + 1) * s] - A[o + (q + 2) * s]) * ih12;
+}
+
+__device__ __forceinline__ double d2_42(const double* A, int o, int s, int q, int e, bool lf, bool hf,
+                                        double isq, double isq12) {
+#pragma clang fp contract(off)
+  if (hf && q == e - 1)
+    return (-A[o + (e - 4) * s] + 4.0 * A[o + (e - 3) * s] - 5.0 * A[o + (e - 2) * s] + 2.0 * A[o + (e - 1) * s]) * isq;
+  if (hf && q == e - 2) return (A[o + (e - 3) * s] - 2.0 * A[o + (e - 2) * s] + A[o + (e - 1) * s]) * isq;
+  if (lf && q == 4) return (A[o + 3 * s] - 2.0 * A[o + 4 * s] + A[o + 5 * s]) * isq;
+  if (lf && q == 3) return (2.0 * A[o + 3 * s] - 5.0 * A[o + 4 * s] + 4.0 * A[o + 5 * s] - A[o + 6 * s]) * isq;
+  return (-A[o + (q - 2) * s] + 16.0 * A[o + (q - 1) * s] - 30.0 * A[o + q * s] + 16.0 * A[o + (q + 1) * s] - A[o + (q + 2) * s]) * isq12;
+}
+
+__device__ __forceinline__ double adv_42(const double* A, int o, int s, int q, int e, bool lf, bool hf, double b,
+                                         double ih2, double ih12) {
+#pragma clang fp contract(off)
+  const bool pos = b > 0.0;
+  if (hf && q == e - 1) return (A[o + (e - 3) * s] - 4.0 * A[o + (e - 2) * s] + 3.0 * A[o + (e - 1) * s]) * ih2;
+  if (hf && q == e - 2) {
+    if (pos) return (-A[o + (e - 3) * s] + A[o + (e - 1) * s]) * ih2;
+    return (A[o + (e - 4) * s] - 4.0 * A[o + (e - 3) * s] + 3.0 * A[o + (e - 2) * s]) * ih2;
+  }
+  if (hf && q == e - 3) {
+    if (pos) return (-3.0 * A[o + (e - 3) * s] + 4.0 * A[o + (e - 2) * s] - A[o + (e - 1) * s]) * ih2;
+    return (-A[o + (e - 6) * s] + 6.0 * A[o + (e - 5) * s] - 18.0 * A[o + (e - 4) * s] + 10.0 * A[o + (e - 3) * s] +
+            3.0 * A[o + (e - 2) * s]) * ih12;
+  }
+  if (lf && q == 3) return (-3.0 * A[o + 3 * s] + 4.0 * A[o + 4 * s] - A[o + 5 * s]) * ih2;
+  if (lf && q == 4) {
+    if (pos) return (-3.0 * A[o + 4 * s] + 4.0 * A[o + 5 * s] - A[o + 6 * s]) * ih2;
+    return (-A[o + 3 * s] + A[o + 5 * s]) * ih2;
+  }
+  if (lf && q == 5) {
+    if (pos) return (-3.0 * A[o + 4 * s] - 10.0 * A[o + 5 * s] + 18.0 * A[o + 6 * s] - 6.0 * A[o + 7 * s] + A[o + 8 * s]) * ih12;
+    return (A[o + 3 * s] - 4.0 * A[o + 4 * s] + 3.0 * A[o + 5 * s]) * ih2;
+  }
+  if (pos)
+    return (-3.0 * A[o + (q - 1) * s] - 10.0 * A[o + q * s] + 18.0 * A[o + (q + 1) * s] - 6.0 * A[o + (q + 2) * s] +
+            A[o + (q + 3) * s]) * ih12;
+  return (-A[o + (q - 3) * s] + 6.0 * A[o + (q - 2) * s] - 18.0 * A[o + (q - 1) * s] + 10.0 * A[o + q * s] +
+          3.0 * A[o + (q + 1) * s]) * ih12;
+}
+
+__device__ __forceinline__ double ko_42(const double* A, int o, int s, int q, int e, bool lf, bool hf, double pre,
+                                        double smr1, double smr2, double smr3) {
+#pragma clang fp contract(off)
+  if (hf && q == e - 1)
+    return (A[o + (e - 4) * s] - 3.0 * A[o + (e - 3) * s] + 3.0 * A[o + (e - 2) * s] - A[o + (e - 1) * s]) / smr3;
+  if (hf && q == e - 2)
+    return (A[o + (e - 5) * s] - 6.0 * A[o + (e - 4) * s] + 12.0 * A[o + (e - 3) * s] - 10.0 * A[o + (e - 2) * s] +
+            3.0 * A[o + (e - 1) * s]) / smr2;
+  if (hf && q == e - 3)
+    return (A[o + (e - 6) * s] - 6.0 * A[o + (e - 5) * s] + 15.0 * A[o + (e - 4) * s] - 19.0 * A[o + (e - 3) * s] +
+            12.0 * A[o + (e - 2) * s] - 3.0 * A[o + (e - 1) * s]) / smr1;
+  if (lf && q == 3) return (A[o + 6 * s] - 3.0 * A[o + 5 * s] + 3.0 * A[o + 4 * s] - A[o + 3 * s]) / smr3;
+  if (lf && q == 4)
+    return (A[o + 7 * s] - 6.0 * A[o + 6 * s] + 12.0 * A[o + 5 * s] - 10.0 * A[o + 4 * s] + 3.0 * A[o + 3 * s]) / smr2;
+  if (lf && q == 5)
+    return (A[o + 8 * s] - 6.0 * A[o + 7 * s] + 15.0 * A[o + 6 * s] - 19.0 * A[o + 5 * s] + 12.0 * A[o + 4 * s] - 3.0 * A[o + 3 * s]) /
+           smr1;
+  return pre * (-A[o + (q - 3) * s] + 6.0 * A[o + (q - 2) * s] - 15.0 * A[o + (q - 1) * s] + 20.0 * A[o + q * s] -
+                15.0 * A[o + (q + 1) * s] + 6.0 * A[o + (q + 2) * s] - A[o + (q + 3) * s]);
+}
+
+// (falloff, asymptotic value) of the Sommerfeld terms per variable
+// (BSSN_GR/src/rhs.cpp:267-320), VAR order.
+__constant__ double c_falloff[24] = {1, 1, 1, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2};
+__constant__ double c_asym[24] = {1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0};
+
+// Variables with second derivatives (bssnrhs_derivs.h).
+__host__ __device__ constexpr bool has_d2(int v) {
+  return v == dgr::V_ALPHA || v == dgr::V_CHI || (v >= dgr::V_BETA0 && v <= dgr::V_BETA2) ||
+         (v >= dgr::V_SG0 && v <= dgr::V_SG5);
+}
+// Variables whose first derivatives enter the equations.
+__host__ __device__ constexpr bool has_d1(int v) {
+  return has_d2(v) || v == dgr::V_K || (v >= dgr::V_GT0 && v <= dgr::V_GT2);
+}
+
+// Processing order: the metric first (Christoffels), then chi, alpha (the
+// Christoffels die here), Gt (closes the Ricci tensor), beta, K, then the
+// variables that only need advection and dissipation.
+constexpr int h_order[24] = {dgr::V_SG0, dgr::V_SG1, dgr::V_SG2, dgr::V_SG3, dgr::V_SG4, dgr::V_SG5,
+                             dgr::V_CHI, dgr::V_ALPHA, dgr::V_GT0, dgr::V_GT1, dgr::V_GT2,
+                             dgr::V_BETA0, dgr::V_BETA1, dgr::V_BETA2, dgr::V_K, dgr::V_B0,
+                             dgr::V_B1, dgr::V_B2, dgr::V_SA0, dgr::V_SA1, dgr::V_SA2, dgr::V_SA3,
+                             dgr::V_SA4, dgr::V_SA5};
+
+struct TileGeom {
+  uint64_t off;       // block offset
+  int sz0, sz1, sz2;
+  int i0, j0, k0, nx, ny, nz;
+  int rx, ry, rz;     // staged region extent (n+6)
+  int nreg;
+};
+
+// Issue the global loads of one variable's staged region into registers.
+__device__ __forceinline__ void prefetch(double (&r)[NPF], const double* __restrict__ g, const TileGeom& T,
+                                         int t) {
+#pragma unroll
+  for (int m = 0; m < NPF; ++m) {
+    const int e = t + m * NT;
+    if (e < T.nreg) {
+      const int a = e % T.rx;
+      const int bc = e / T.rx;
+      const int b = bc % T.ry;
+      const int c = bc / T.ry;
+      const long gi = (long)(T.i0 - 3 + a) + (long)T.sz0 * ((T.j0 - 3 + b) + (long)T.sz1 * (T.k0 - 3 + c));
+      r[m] = __builtin_nontemporal_load(g + T.off + gi);
+    }
+  }
+}
+
+__device__ __forceinline__ void stage(const double (&r)[NPF], double* su, const TileGeom& T, int t) {
+#pragma unroll
+  for (int m = 0; m < NPF; ++m) {
+    const int e = t + m * NT;
+    if (e < T.nreg) {
+      const int a = e % T.rx;
+      const int bc = e / T.rx;
+      const int b = bc % T.ry;
+      const int c = bc / T.ry;
+      su[a + SX * (b + SY * c)] = r[m];
+    }
+  }
+}
+
+// Per-thread context of one tile.
+struct Ctx {
+  bool act, on_face;
+  unsigned bflag;
+  int ti, tj, tk, gi, gj, gk;
+  double x, y, z, inv_r, eta, sigma;
+  uint64_t o;  // global index of the point
+  const AxisC* ax;  // per-axis constants of the block (device memory)
+  dgr::Params P;
+  double ex_gt[6], ex_Gt[3], ex_chi;  // dissipation (+ Sommerfeld) held until the output is final
+};
+
+// rhs = eq + (KO), or (Sommerfeld + KO) on flagged faces (rhs.cpp:265-373)
+__device__ __forceinline__ void emit(const IOPtrs* __restrict__ io, const Ctx& C, int v, double eq, double ex) {
+#pragma clang fp contract(off)
+  const double r = C.on_face ? ex : (eq + ex);
+  __builtin_nontemporal_store(r, io->r.v[v] + C.o);
+}
+
+template <int V>
+__device__ __forceinline__ void process_var(dgr::Point& s, Ctx& C, const double* su, const double* sgx,
+                                            const double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io) {
+  using namespace dgr;
+  if (!C.act) return;
+  const AxisC* ax = C.ax;
+  const int ti = C.ti, tj = C.tj, tk = C.tk, gi = C.gi, gj = C.gj, gk = C.gk;
+  // su index of this point; lines through it along x, y, z are addressed
+  // as su[o + q*stride] with q the block coordinate along the axis
+  const int c = (ti + 3) + SX * ((tj + 3) + SY * (tk + 3));
+  const int ox = c - gi, oy = c - gj * SX, oz = c - gk * (SX * SY);
+  double d1[3] = {0.0, 0.0, 0.0};
+  double d2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if constexpr (has_d2(V)) {
+    // x and y derivatives at the point are read back from the LDS planes
+    d1[0] = sgx[ti + TX * ((tj + 2) + GXY * (tk + 2))];
+    d1[1] = sgy[ti + TX * (tj + TY * (tk + 2))];
+    d1[2] = d1_42(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
+    d2[0] = d2_42(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, ax[0].isq, ax[0].isq12);
+    d2[3] = d2_42(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].isq, ax[1].isq12);
+    d2[5] = d2_42(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].isq, ax[2].isq12);
+    // mixed derivatives by composition: d_y(d_x u), d_z(d_x u), d_z(d_y u).
+    // sgx[a + TX*(b + GXY*cc)] holds d_x u at block coords (i0+a, j0-2+b, k0-2+cc);
+    // sgy[a + TX*(b + TY*cc)] holds d_y u at (i0+a, j0+b, k0-2+cc).
+    const int oxy = ti + TX * ((2 - T.j0) + GXY * (tk + 2));
+    d2[1] = d1_42(sgx, oxy, TX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
+    const int oxz = ti + TX * ((tj + 2) + GXY * (2 - T.k0));
+    d2[2] = d1_42(sgx, oxz, TX * GXY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
+    const int oyz = ti + TX * (tj + TY * (2 - T.k0));
+    d2[4] = d1_42(sgy, oyz, TX * TY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
+  } else {
+    if (has_d1(V) || C.bflag != 0u) {
+      d1[0] = d1_42(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, ax[0].ih2, ax[0].ih12);
+      d1[1] = d1_42(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
+      d1[2] = d1_42(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
+    }
+  }
+  // upwinded derivatives contracted with beta, summed in the order of the
+  // reference's printed expression: beta0*a0 + beta1*a1 + beta2*a2
+  double adv;
+  {
+    const double a0 = adv_42(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, s.beta[0], ax[0].ih2, ax[0].ih12);
+    const double a1 = adv_42(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, s.beta[1], ax[1].ih2, ax[1].ih12);
+    const double a2 = adv_42(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, s.beta[2], ax[2].ih2, ax[2].ih12);
+    adv = s.beta[0] * a0 + s.beta[1] * a1 + s.beta[2] * a2;
+  }
+  // Kreiss-Oliger sum, preceded on flagged faces by the Sommerfeld value
+  double ex;
+  {
+#pragma clang fp contract(off)
+    const double k0 = ko_42(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, ax[0].pre, ax[0].smr1, ax[0].smr2, ax[0].smr3);
+    const double k1 = ko_42(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].pre, ax[1].smr1, ax[1].smr2, ax[1].smr3);
+    const double k2 = ko_42(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].pre, ax[2].smr1, ax[2].smr2,
+                            ax[2].smr3);
+    const double ko = C.sigma * (k0 + k1 + k2);
+    if (C.on_face) {
+      const double f = su[c];
+      const double bc = -C.inv_r * (C.x * d1[0] + C.y * d1[1] + C.z * d1[2] + c_falloff[V] * (f - c_asym[V]));
+      ex = bc + ko;
+    } else {
+      ex = ko;
+    }
+  }
+  // feed the staged equations; emit every right-hand side that is final
+  if constexpr (V >= V_SG0 && V <= V_SG5) {
+    s.add_gt(V - V_SG0, d1, d2, adv);
+    C.ex_gt[V - V_SG0] = ex;
+  }
+  if constexpr (V == V_CHI) {
+    s.add_chi(d1, d2, adv);
+    C.ex_chi = ex;
+  }
+  if constexpr (V == V_ALPHA) emit(io, C, V_ALPHA, s.add_alpha(C.P, d1, d2, adv), ex);
+  if constexpr (V >= V_GT0 && V <= V_GT2) {
+    s.add_Gt(C.P, V - V_GT0, d1, adv);
+    C.ex_Gt[V - V_GT0] = ex;
+  }
+  if constexpr (V >= V_BETA0 && V <= V_BETA2) {
+    double gto[6], chio = 0.0;
+    emit(io, C, V, s.add_beta(C.P, V - V_BETA0, d1, d2, adv, gto, &chio), ex);
+    if constexpr (V == V_BETA2) {
+#pragma unroll
+      for (int p = 0; p < 6; ++p) emit(io, C, V_SG0 + p, gto[p], C.ex_gt[p]);
+      emit(io, C, V_CHI, chio, C.ex_chi);
+    }
+  }
+  if constexpr (V == V_K) {
+    emit(io, C, V_K, s.add_K(d1, adv), ex);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) emit(io, C, V_GT0 + k, s.Gt_rhs[k], C.ex_Gt[k]);
+  }
+  if constexpr (V >= V_B0 && V <= V_B2) emit(io, C, V, s.add_B(C.P, V - V_B0, adv, C.eta), ex);
+  if constexpr (V >= V_SA0 && V <= V_SA5) emit(io, C, V, s.add_At(V - V_SA0, adv), ex);
+}
+
+// Widened first-derivative planes for the mixed derivatives.
+__device__ __forceinline__ void deriv_planes(const double* su, double* sgx, double* sgy, const TileGeom& T,
+                                             const AxisC* ax, int t) {
+  // sgx: d_x u at block coords (i0+a, j0-2+b, k0-2+cc), su coords (a+3, b+1, cc+1)
+  const int nxg = T.nx * (T.ny + 4) * (T.nz + 4);
+  for (int e = t; e < nxg; e += NT) {
+    const int a = e % T.nx;
+    const int bc = e / T.nx;
+    const int b = bc % (T.ny + 4);
+    const int cc = bc / (T.ny + 4);
+    const int o = (a + 3) + SX * ((b + 1) + SY * (cc + 1)) - (T.i0 + a);
+    sgx[a + TX * (b + GXY * cc)] = d1_42(su, o, 1, T.i0 + a, ax[0].e, ax[0].lf, ax[0].hf, ax[0].ih2, ax[0].ih12);
+  }
+  // sgy: d_y u at block coords (i0+a, j0+b, k0-2+cc), su coords (a+3, b+3, cc+1)
+  const int nyg = T.nx * T.ny * (T.nz + 4);
+  for (int e = t; e < nyg; e += NT) {
+    const int a = e % T.nx;
+    const int bc = e / T.nx;
+    const int b = bc % T.ny;
+    const int cc = bc / T.ny;
+    const int o = (a + 3) + SX * ((b + 3) + SY * (cc + 1)) - (T.j0 + b) * SX;
+    sgy[a + TX * (b + TY * cc)] = d1_42(su, o, SX, T.j0 + b, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
+  }
+}
+
+template <int I>
+__device__ __forceinline__ void run_vars(dgr::Point& s, Ctx& C, double (&pf)[NPF], double* su, double* sgx,
+                                         double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io,
+                                         int t) {
+  if constexpr (I < 24) {
+    constexpr int V = h_order[I];
+    stage(pf, su, T, t);
+    __syncthreads();
+    if constexpr (has_d2(V)) deriv_planes(su, sgx, sgy, T, C.ax, t);
+    if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, t);
+    if constexpr (has_d2(V)) __syncthreads();
+    if constexpr (V == dgr::V_CHI) {
+      if (C.act) s.finish_metric();
+    }
+    process_var<V>(s, C, su, sgx, sgy, T, io);
+    __syncthreads();
+    run_vars<I + 1>(s, C, pf, su, sgx, sgy, T, io, t);
+  }
+}
+
+__global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ blocks,
+                                                 const DevTile* __restrict__ tiles, int tile_begin, int ntiles,
+                                                 const IOPtrs* __restrict__ io, DevParams P) {
+  __shared__ double su[NSU];
+  __shared__ double sgx[NGX];
+  __shared__ double sgy[NGY];
+
+  const int t = threadIdx.x;
+  Ctx C;
+  C.ti = t % TX;
+  C.tj = (t / TX) % TY;
+  C.tk = t / (TX * TY);
+  for (int i = 0; i < 4; ++i) C.P.lambda[i] = P.lambda[i];
+  C.P.lambda_f[0] = P.lambda_f[0];
+  C.P.lambda_f[1] = P.lambda_f[1];
+  C.sigma = P.ko_sigma;
+
+  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    const DevTile tl = tiles[tile_begin + w];
+    const DevBlock& B = blocks[tl.block];
+    TileGeom T;
+    T.off = B.offset;
+    T.sz0 = B.sz[0]; T.sz1 = B.sz[1]; T.sz2 = B.sz[2];
+    T.i0 = tl.i0; T.j0 = tl.j0; T.k0 = tl.k0;
+    T.nx = tl.nx; T.ny = tl.ny; T.nz = tl.nz;
+    T.rx = T.nx + 6; T.ry = T.ny + 6; T.rz = T.nz + 6;
+    T.nreg = T.rx * T.ry * T.rz;
+
+    C.act = t < NPT && C.ti < T.nx && C.tj < T.ny && C.tk < T.nz;
+    C.gi = T.i0 + C.ti; C.gj = T.j0 + C.tj; C.gk = T.k0 + C.tk;
+    C.bflag = B.bflag;
+    C.ax = blocks[tl.block].ax;
+
+    double pf[NPF];
+    prefetch(pf, io->u.v[h_order[0]], T, t);
+
+    // point coordinates, eta (rhs.cpp:214-223) and the Sommerfeld face test
+    dgr::Point s;
+    C.o = T.off + (uint64_t)C.gi + (uint64_t)T.sz0 * (C.gj + (uint64_t)T.sz1 * C.gk);
+    C.on_face = false;
+    C.eta = P.eta_const;
+    if (C.act) {
+#pragma clang fp contract(off)
+      C.x = B.pmin[0] + C.gi * B.h[0];
+      C.y = B.pmin[1] + C.gj * B.h[1];
+      C.z = B.pmin[2] + C.gk * B.h[2];
+      const double rr = sqrt(C.x * C.x + C.y * C.y + C.z * C.z);
+      if (rr >= P.eta_r0) C.eta *= pow(P.eta_r0 / rr, P.eta_damping_exp);
+      C.inv_r = 1.0 / rr;
+      const unsigned bf = C.bflag;
+      C.on_face = ((bf & 1u) && C.gi == 3) || ((bf & 2u) && C.gi == T.sz0 - 4) || ((bf & 4u) && C.gj == 3) ||
+                  ((bf & 8u) && C.gj == T.sz1 - 4) || ((bf & 16u) && C.gk == 3) ||
+                  ((bf & 32u) && C.gk == T.sz2 - 4);
+      const uint64_t o = C.o;
+      s.alpha = io->u.v[dgr::V_ALPHA][o];
+      s.chi = io->u.v[dgr::V_CHI][o];
+      s.K = io->u.v[dgr::V_K][o];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        s.beta[i] = io->u.v[dgr::V_BETA0 + i][o];
+        s.B[i] = io->u.v[dgr::V_B0 + i][o];
+      }
+#pragma unroll
+      for (int p = 0; p < 6; ++p) {
+        s.gt[p] = io->u.v[dgr::V_SG0 + p][o];
+        s.At[p] = io->u.v[dgr::V_SA0 + p][o];
+      }
+      s.begin();
+    }
+    run_vars<0>(s, C, pf, su, sgx, sgy, T, io, t);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct dgr_plan_s {
+  int device = 0;
+  uint32_t nblocks = 0;
+  uint32_t ntiles = 0;
+  uint64_t points = 0;
+  DevBlock* d_blocks = nullptr;
+  DevTile* d_tiles = nullptr;
+  std::vector<uint32_t> block_tile_begin;  // tiles of block b: [begin[b], begin[b+1])
+  DevParams params{};
+  int grid = 0;  // persistent grid size
+  IOPtrs* d_io = nullptr;  // device copy of the 48 array pointers
+  IOPtrs h_io{};           // last uploaded pointers
+  bool io_valid = false;
+};
+
+namespace {
+
+thread_local char g_errbuf[256];
+
+int hip_fail(hipError_t e, const char* what) {
+  snprintf(g_errbuf, sizeof(g_errbuf), "%s: %s", what, hipGetErrorString(e));
+  return DGR_E_HIP;
+}
+
+#define DGR_HIP(call)                                   \
+  do {                                                  \
+    hipError_t _e = (call);                             \
+    if (_e != hipSuccess) return hip_fail(_e, #call);   \
+  } while (0)
+
+// Stencil constants of one axis, with the operations of derivs.cpp
+// (idx = 1/dx; idx_by_2 = 0.5*idx; idx_by_12 = idx/12; idx_sqrd = 1/(dx*dx);
+// pre_factor_6_dx = -1/64/dx; smr3 = 59/48*64*dx, ...).
+AxisC host_axis(double h, int sz, unsigned bflag, int a) {
+  AxisC c;
+  const double ih = 1.0 / h;
+  c.ih2 = 0.5 * ih;
+  c.ih12 = ih / 12.0;
+  c.isq = 1.0 / (h * h);
+  c.isq12 = c.isq / 12.0;
+  c.pre = -1.0 / 64.0 / h;
+  c.smr3 = 59.0 / 48.0 * 64 * h;
+  c.smr2 = 43.0 / 48.0 * 64 * h;
+  c.smr1 = 49.0 / 48.0 * 64 * h;
+  c.e = sz - 3;
+  c.lf = (bflag >> (2 * a)) & 1u;
+  c.hf = (bflag >> (2 * a + 1)) & 1u;
+  c.pad = 0;
+  return c;
+}
+
+// Split n interior points into chunks of at most 7, as even as possible.
+void split(int n, std::vector<std::pair<int, int>>& out) {
+  out.clear();
+  const int parts = (n + TX - 1) / TX;
+  int start = 3;
+  for (int p = 0; p < parts; ++p) {
+    const int len = n / parts + (p < n % parts ? 1 : 0);
+    out.emplace_back(start, len);
+    start += len;
+  }
+}
+
+}  // namespace
+
+extern "C" const char* dgr_error_string(int code) {
+  switch (code) {
+    case DGR_OK: return "ok";
+    case DGR_E_INVALID: return "invalid argument";
+    case DGR_E_HIP: return g_errbuf[0] ? g_errbuf : "HIP error";
+    case DGR_E_NOMEM: return "device allocation failed";
+    case DGR_E_UNSUPPORTED: return "unsupported configuration";
+    case DGR_E_CONSTRAINT: return "constraint projection failed";
+    default: return "unknown error";
+  }
+}
+
+extern "C" int dgr_init(int device) {
+  int n = 0;
+  DGR_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return DGR_E_INVALID;
+  DGR_HIP(hipSetDevice(device));
+  return DGR_OK;
+}
+
+extern "C" int dgr_plan_create(dgr_plan_t* plan_out, const dgr_block* blocks, uint32_t nblocks,
+                               const dgr_bssn_params* p) {
+  if (!plan_out || !blocks || !p || nblocks == 0) return DGR_E_INVALID;
+  if (p->deriv_order != 42) return DGR_E_UNSUPPORTED;
+  std::vector<DevBlock> hb(nblocks);
+  std::vector<DevTile> ht;
+  dgr_plan_s* plan = new dgr_plan_s();
+  plan->block_tile_begin.resize(nblocks + 1);
+  std::vector<std::pair<int, int>> sx, sy, sz;
+  for (uint32_t b = 0; b < nblocks; ++b) {
+    const dgr_block& bl = blocks[b];
+    for (int d = 0; d < 3; ++d) {
+      if (bl.sz[d] < 13 || bl.sz[d] > 65535) { delete plan; return DGR_E_INVALID; }
+    }
+    DevBlock& db = hb[b];
+    db.offset = bl.offset;
+    for (int d = 0; d < 3; ++d) {
+      db.sz[d] = bl.sz[d];
+      db.pmin[d] = bl.pmin[d];
+      db.h[d] = (bl.pmax[d] - bl.pmin[d]) / (bl.sz[d] - 1);  // rhs.cpp:148-150
+    }
+    for (int d = 0; d < 3; ++d) db.ax[d] = host_axis(db.h[d], (int)bl.sz[d], bl.bflag, d);
+    db.bflag = bl.bflag;
+    plan->block_tile_begin[b] = (uint32_t)ht.size();
+    split(bl.sz[0] - 6, sx);
+    split(bl.sz[1] - 6, sy);
+    split(bl.sz[2] - 6, sz);
+    for (auto& cz : sz)
+      for (auto& cy : sy)
+        for (auto& cx : sx) {
+          DevTile t;
+          t.block = b;
+          t.i0 = (uint16_t)cx.first; t.j0 = (uint16_t)cy.first; t.k0 = (uint16_t)cz.first;
+          t.nx = (uint8_t)cx.second; t.ny = (uint8_t)cy.second; t.nz = (uint8_t)cz.second;
+          t.pad = 0;
+          ht.push_back(t);
+        }
+    plan->points += (uint64_t)(bl.sz[0] - 6) * (bl.sz[1] - 6) * (bl.sz[2] - 6);
+  }
+  plan->block_tile_begin[nblocks] = (uint32_t)ht.size();
+  plan->nblocks = nblocks;
+  plan->ntiles = (uint32_t)ht.size();
+  for (int i = 0; i < 4; ++i) plan->params.lambda[i] = p->lambda[i];
+  plan->params.lambda_f[0] = p->lambda_f[0];
+  plan->params.lambda_f[1] = p->lambda_f[1];
+  plan->params.eta_const = p->eta_const;
+  plan->params.eta_r0 = p->eta_r0;
+  plan->params.eta_damping_exp = p->eta_damping_exp;
+  plan->params.ko_sigma = p->ko_sigma;
+  hipError_t e = hipGetDevice(&plan->device);
+  if (e != hipSuccess) { delete plan; return hip_fail(e, "hipGetDevice"); }
+  int ncu = 0;
+  e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, plan->device);
+  if (e != hipSuccess) { delete plan; return hip_fail(e, "hipDeviceGetAttribute"); }
+  plan->grid = std::max(1, std::min<int>((int)plan->ntiles, ncu * 2));
+  if (hipMalloc(&plan->d_blocks, sizeof(DevBlock) * nblocks) != hipSuccess ||
+      hipMalloc(&plan->d_tiles, sizeof(DevTile) * ht.size()) != hipSuccess ||
+      hipMalloc(&plan->d_io, sizeof(IOPtrs)) != hipSuccess) {
+    hipFree(plan->d_blocks);
+    hipFree(plan->d_tiles);
+    delete plan;
+    return DGR_E_NOMEM;
+  }
+  e = hipMemcpy(plan->d_blocks, hb.data(), sizeof(DevBlock) * nblocks, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(plan->d_tiles, ht.data(), sizeof(DevTile) * ht.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(plan->d_blocks);
+    hipFree(plan->d_tiles);
+    hipFree(plan->d_io);
+    delete plan;
+    return hip_fail(e, "hipMemcpy(plan)");
+  }
+  *plan_out = plan;
+  return DGR_OK;
+}
+
+extern "C" int dgr_plan_destroy(dgr_plan_t plan) {
+  if (!plan) return DGR_E_INVALID;
+  hipFree(plan->d_blocks);
+  hipFree(plan->d_tiles);
+  hipFree(plan->d_io);
+  delete plan;
+  return DGR_OK;
+}
+
+extern "C" uint64_t dgr_plan_points(dgr_plan_t plan) { return plan ? plan->points : 0; }
+extern "C" uint32_t dgr_plan_tiles(dgr_plan_t plan) { return plan ? plan->ntiles : 0; }
+
+namespace {
+// Upload the 48 array pointers to the plan's device table when they differ
+// from the last call (stream-ordered, so calls on one stream see their own
+// pointers).
+int upload_ptrs(dgr_plan_t plan, double* const* rhs, const double* const* uin, hipStream_t st) {
+  if (!rhs || !uin) return DGR_E_INVALID;
+  IOPtrs io;
+  for (int v = 0; v < 24; ++v) {
+    if (!rhs[v] || !uin[v]) return DGR_E_INVALID;
+    io.u.v[v] = uin[v];
+    io.r.v[v] = rhs[v];
+  }
+  if (plan->io_valid && std::memcmp(&io, &plan->h_io, sizeof(IOPtrs)) == 0) return DGR_OK;
+  plan->h_io = io;
+  DGR_HIP(hipMemcpyAsync(plan->d_io, &plan->h_io, sizeof(IOPtrs), hipMemcpyHostToDevice, st));
+  plan->io_valid = true;
+  return DGR_OK;
+}
+}  // namespace
+
+extern "C" int dgr_bssn_rhs(dgr_plan_t plan, double* const* rhs, const double* const* uin, void* stream) {
+  if (!plan) return DGR_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = upload_ptrs(plan, rhs, uin, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_bssn_rhs, dim3(plan->grid), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, 0,
+                     (int)plan->ntiles, plan->d_io, plan->params);
+  DGR_HIP(hipGetLastError());
+  return DGR_OK;
+}
+
+extern "C" int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const double* const* uin,
+                                      void* stream) {
+  if (!plan) return DGR_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = upload_ptrs(plan, rhs, uin, st);
+  if (rc) return rc;
+  for (uint32_t b = 0; b < plan->nblocks; ++b) {
+    const int tb = (int)plan->block_tile_begin[b];
+    const int nt = (int)(plan->block_tile_begin[b + 1] - plan->block_tile_begin[b]);
+    hipLaunchKernelGGL(k_bssn_rhs, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, tb, nt, plan->d_io,
+                       plan->params);
+  }
+  DGR_HIP(hipGetLastError());
+  return DGR_OK;
+}
+
+extern "C" int dgr_bssn_rhs_blocks(double* const* rhs, const double* const* u, const dgr_block* blocks,
+                                   uint32_t nblocks, const dgr_bssn_params* p, void* stream) {
+  dgr_plan_t plan = nullptr;
+  int rc = dgr_plan_create(&plan, blocks, nblocks, p);
+  if (rc) return rc;
+  rc = dgr_bssn_rhs(plan, rhs, u, stream);
+  if (rc == DGR_OK) {
+    hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
+  }
+  dgr_plan_destroy(plan);
+  return rc;
+}

@@ -1,0 +1,124 @@
+"""Seeded synthetic initial data on padded blocks (torch, any device).
+
+Every padded point of every block is filled from an analytic field, so the
+ghost layers hold the same data a perfect unzip would give.
+Variables are in VAR order (BSSN_GR/include/grDef.h:31).
+"""
+import math
+
+import numpy as np
+import torch
+
+NUM_VARS = 24
+ALPHA, CHI, K, GT0, BETA0, B0, SG0, SA0 = 0, 1, 2, 3, 6, 9, 12, 18
+
+
+def coords(blocks, device="cpu"):
+    """x, y, z of every padded point (each (dof,) float64) and its flat index,
+    listed block by block in the order of `blocks`."""
+    xs, ys, zs, idx = [], [], [], []
+    # group consecutive runs of equal size so each run is one broadcast
+    runs = []
+    for b in blocks:
+        key = tuple(b["sz"])
+        if runs and runs[-1][0] == key:
+            runs[-1][1].append(b)
+        else:
+            runs.append((key, [b]))
+    for (s0, s1, s2), bl in runs:
+        k, j, i = torch.meshgrid(torch.arange(s2, dtype=torch.float64), torch.arange(s1, dtype=torch.float64),
+                                 torch.arange(s0, dtype=torch.float64), indexing="ij")
+        i, j, k = i.reshape(1, -1), j.reshape(1, -1), k.reshape(1, -1)
+        pmin = torch.tensor([b["pmin"] for b in bl], dtype=torch.float64)
+        pmax = torch.tensor([b["pmax"] for b in bl], dtype=torch.float64)
+        h = (pmax - pmin) / torch.tensor([s0 - 1, s1 - 1, s2 - 1], dtype=torch.float64)
+        xs.append((pmin[:, 0:1] + i * h[:, 0:1]).reshape(-1))
+        ys.append((pmin[:, 1:2] + j * h[:, 1:2]).reshape(-1))
+        zs.append((pmin[:, 2:3] + k * h[:, 2:3]).reshape(-1))
+        off = torch.tensor([b["offset"] for b in bl], dtype=torch.int64).reshape(-1, 1)
+        idx.append((off + torch.arange(s0 * s1 * s2, dtype=torch.int64).reshape(1, -1)).reshape(-1))
+    cat = lambda a: torch.cat(a).to(device)
+    return cat(xs), cat(ys), cat(zs), cat(idx)
+
+
+def _flat(dof, device):
+    U = torch.zeros((NUM_VARS, dof), dtype=torch.float64, device=device)
+    U[ALPHA] = 1.0
+    U[CHI] = 1.0
+    U[SG0 + 0] = 1.0
+    U[SG0 + 3] = 1.0
+    U[SG0 + 5] = 1.0
+    return U
+
+
+def flat_bumps(blocks, dof, seed=0, amp=1e-3, width=0.1, nbumps=8, box=None, device="cpu"):
+    """Flat space (alpha=chi=1, gt=delta, rest 0) plus, on every variable,
+    the sum of `nbumps` Gaussians amp*exp(-|x-c|^2/width^2) with seeded
+    centres uniform in the bounding box of the blocks."""
+    x, y, z, idx = coords(blocks, device)
+    if box is None:
+        lo = [min(b["pmin"][d] for b in blocks) for d in range(3)]
+        hi = [max(b["pmax"][d] for b in blocks) for d in range(3)]
+    else:
+        lo, hi = box
+    rng = np.random.default_rng(seed)
+    cen = rng.uniform(lo, hi, size=(NUM_VARS, nbumps, 3))
+    U = _flat(dof, device)
+    iw2 = 1.0 / (width * width)
+    for v in range(NUM_VARS):
+        acc = torch.zeros_like(x)
+        for c in cen[v]:
+            acc += torch.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) * iw2)
+        U[v, idx] += amp * acc
+    return U
+
+
+def gauge_wave(blocks, dof, amp=0.01, length=None, noise=1e-6, seed=0, device="cpu"):
+    """Gauge wave: alpha = 1 - A sin(2 pi x / L), physical metric
+    diag(alpha^2, 1, 1), so chi = alpha^(-2/3), gt_xx = alpha^(4/3),
+    gt_yy = gt_zz = alpha^(-2/3); K_xx = -A (2pi/L) cos(2pi x/L) with zero
+    shift; traceless At and Gt^x = -d_x gt^xx from those; plus seeded white
+    noise of size `noise` on all 24 variables."""
+    x, y, z, idx = coords(blocks, device)
+    if length is None:
+        length = max(b["pmax"][0] for b in blocks) - min(b["pmin"][0] for b in blocks)
+    k = 2.0 * math.pi / length
+    a = 1.0 - amp * torch.sin(k * x)
+    da = -amp * k * torch.cos(k * x)
+    U = torch.zeros((NUM_VARS, dof), dtype=torch.float64, device=device)
+    chi = a ** (-2.0 / 3.0)
+    Kxx = da  # -(1/2alpha) d_t gamma_xx with gamma_xx = alpha^2 moving at unit speed
+    Ktr = Kxx / (a * a)
+    U[ALPHA, idx] = a
+    U[CHI, idx] = chi
+    U[K, idx] = Ktr
+    U[GT0, idx] = (4.0 / 3.0) * a ** (-7.0 / 3.0) * da
+    U[SG0 + 0, idx] = a ** (4.0 / 3.0)
+    U[SG0 + 3, idx] = chi
+    U[SG0 + 5, idx] = chi
+    U[SA0 + 0, idx] = chi * (Kxx - a * a * Ktr / 3.0)
+    U[SA0 + 3, idx] = chi * (-Ktr / 3.0)
+    U[SA0 + 5, idx] = chi * (-Ktr / 3.0)
+    if noise:
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        U += noise * torch.randn(U.shape, generator=g, dtype=torch.float64).to(device)
+    return U
+
+
+def brill_lindquist(blocks, dof, m1=0.5, m2=0.5, sep=8.0, offset=(0.0137, 0.0211, 0.0173), device="cpu"):
+    """Brill-Lindquist punctures on the x axis at +-sep/2 (shifted by
+    `offset` so no grid point sits on a puncture): chi = psi^-4 with
+    psi = 1 + m1/(2 r1) + m2/(2 r2), alpha = chi^(1/2), gt = delta, all
+    else zero."""
+    x, y, z, idx = coords(blocks, device)
+    r1 = torch.sqrt((x - 0.5 * sep - offset[0]) ** 2 + (y - offset[1]) ** 2 + (z - offset[2]) ** 2)
+    r2 = torch.sqrt((x + 0.5 * sep - offset[0]) ** 2 + (y - offset[1]) ** 2 + (z - offset[2]) ** 2)
+    psi = 1.0 + m1 / (2.0 * r1) + m2 / (2.0 * r2)
+    chi = psi ** -4
+    U = torch.zeros((NUM_VARS, dof), dtype=torch.float64, device=device)
+    U[CHI, idx] = chi
+    U[ALPHA, idx] = torch.sqrt(chi)
+    U[SG0 + 0, idx] = 1.0
+    U[SG0 + 3, idx] = 1.0
+    U[SG0 + 5, idx] = 1.0
+    return U

@@ -1,0 +1,81 @@
+"""Block tables for the benchmark workloads.
+
+A block is a dict with the arguments bssnrhs takes per block
+(BSSN_GR/include/rhs.h:47-50): offset into the unzipped arrays, padded size
+sz[3], boundary flag bflag (OCT_DIR_* bits, include/dendro.h:84-89) and the
+padded extent pmin/pmax (corner -/+ 3 dx, BSSN_GR/src/rhs.cpp:51-62).
+Offsets are consecutive (DENDRO_BLOCK_ALIGN_FACTOR = 1, include/dendro.h:173).
+"""
+import numpy as np
+
+GHOST = 3
+
+
+def block_dict(offset, sz, bflag, corner, dx, ele_extent):
+    """Block whose interior spans corner .. corner + ele_extent*dx per axis."""
+    pmin = [corner[d] - GHOST * dx for d in range(3)]
+    pmax = [corner[d] + ele_extent[d] * dx + GHOST * dx for d in range(3)]
+    return {"offset": int(offset), "sz": [int(s) for s in sz], "bflag": int(bflag), "pmin": pmin, "pmax": pmax}
+
+
+def uniform_grid(nb, ele_order=6, dx=1.0 / 64, center=True, outer_bflags=True):
+    """nb^3 blocks of one order-`ele_order` element each (sz = ele_order+7)
+    tiling a cube; blocks on the outer layer carry the boundary flags of
+    their outer faces."""
+    sz = ele_order + 1 + 2 * GHOST
+    n = sz ** 3
+    L = nb * ele_order * dx
+    o = -0.5 * L if center else 0.0
+    blocks = []
+    off = 0
+    for bz in range(nb):
+        for by in range(nb):
+            for bx in range(nb):
+                bf = 0
+                if outer_bflags:
+                    for d, bc in enumerate((bx, by, bz)):
+                        if bc == 0:
+                            bf |= 1 << (2 * d)
+                        if bc == nb - 1:
+                            bf |= 1 << (2 * d + 1)
+                corner = [o + bx * ele_order * dx, o + by * ele_order * dx, o + bz * ele_order * dx]
+                blocks.append(block_dict(off, [sz] * 3, bf, corner, dx, [ele_order] * 3))
+                off += n
+    return blocks, off
+
+
+def mixed_blocks(nblocks, seed=0, dist=((13, 0.60), (19, 0.25), (25, 0.10), (37, 0.05)), dx=1.0 / 64,
+                 box=1.0):
+    """Blocks of uneven padded sizes drawn from `dist` (seeded), placed at
+    random corners in [-box/2, box/2]^3, all interior (bflag 0)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.array([s for s, _ in dist])
+    probs = np.array([p for _, p in dist], dtype=float)
+    probs /= probs.sum()
+    pick = rng.choice(len(sizes), size=nblocks, p=probs)
+    corners = rng.uniform(-0.5 * box, 0.5 * box, size=(nblocks, 3))
+    blocks = []
+    off = 0
+    for b in range(nblocks):
+        sz = int(sizes[pick[b]])
+        blocks.append(block_dict(off, [sz] * 3, 0, corners[b].tolist(), dx, [sz - 1 - 2 * GHOST] * 3))
+        off += sz ** 3
+    return blocks, off
+
+
+def interior_points(blocks):
+    return int(sum((b["sz"][0] - 6) * (b["sz"][1] - 6) * (b["sz"][2] - 6) for b in blocks))
+
+
+def padded_points(blocks):
+    return int(sum(b["sz"][0] * b["sz"][1] * b["sz"][2] for b in blocks))
+
+
+def interior_index(blocks):
+    """Flat indices of every interior point, block by block (int64 array)."""
+    out = []
+    for b in blocks:
+        s0, s1, s2 = b["sz"]
+        k, j, i = np.meshgrid(np.arange(3, s2 - 3), np.arange(3, s1 - 3), np.arange(3, s0 - 3), indexing="ij")
+        out.append((b["offset"] + i + s0 * (j + s1 * k)).ravel())
+    return np.concatenate(out).astype(np.int64)

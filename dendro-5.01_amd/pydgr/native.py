@@ -1,0 +1,164 @@
+"""ctypes binding of libdgr_bssn.so (the C ABI declared in include/dgr_bssn.h).
+
+Loading fails loudly when the library has not been built: there is no
+CPU fallback for any entry point.
+"""
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libdgr_bssn.so")
+NUM_VARS = 24
+
+VAR_NAMES = ["alpha", "chi", "K", "Gt0", "Gt1", "Gt2", "beta0", "beta1", "beta2", "B0", "B1", "B2",
+             "gt0", "gt1", "gt2", "gt3", "gt4", "gt5", "At0", "At1", "At2", "At3", "At4", "At5"]
+
+
+class DgrError(RuntimeError):
+    pass
+
+
+class Block(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_uint64), ("sz", ctypes.c_uint32 * 3), ("bflag", ctypes.c_uint32),
+                ("pmin", ctypes.c_double * 3), ("pmax", ctypes.c_double * 3)]
+
+
+class BSSNParams(ctypes.Structure):
+    _fields_ = [("lambda_", ctypes.c_double * 4), ("lambda_f", ctypes.c_double * 2),
+                ("eta_const", ctypes.c_double), ("eta_r0", ctypes.c_double),
+                ("eta_damping_exp", ctypes.c_double), ("ko_sigma", ctypes.c_double),
+                ("chi_floor", ctypes.c_double), ("deriv_order", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+def make_params(lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0), eta_const=2.0, eta_r0=30.0, eta_damping_exp=1.0,
+                ko_sigma=0.4, chi_floor=0.1, deriv_order=42):
+    p = BSSNParams()
+    p.lambda_[:] = [float(x) for x in lambda_]
+    p.lambda_f[:] = [float(x) for x in lambda_f]
+    p.eta_const = eta_const
+    p.eta_r0 = eta_r0
+    p.eta_damping_exp = eta_damping_exp
+    p.ko_sigma = ko_sigma
+    p.chi_floor = chi_floor
+    p.deriv_order = deriv_order
+    return p
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DgrError("libdgr_bssn.so not built (%s); run python dendro-5.01_amd/build.py" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        L.dgr_error_string.restype = ctypes.c_char_p
+        L.dgr_error_string.argtypes = [ctypes.c_int]
+        L.dgr_init.argtypes = [ctypes.c_int]
+        L.dgr_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Block), ctypes.c_uint32,
+                                      ctypes.POINTER(BSSNParams)]
+        L.dgr_plan_destroy.argtypes = [vp]
+        L.dgr_plan_points.restype = ctypes.c_uint64
+        L.dgr_plan_points.argtypes = [vp]
+        L.dgr_plan_tiles.restype = ctypes.c_uint32
+        L.dgr_plan_tiles.argtypes = [vp]
+        for f in ("dgr_bssn_rhs", "dgr_bssn_rhs_per_block"):
+            getattr(L, f).argtypes = [vp, pp, pp, vp]
+        L.dgr_bssn_rhs_blocks.argtypes = [pp, pp, ctypes.POINTER(Block), ctypes.c_uint32,
+                                          ctypes.POINTER(BSSNParams), vp]
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().dgr_error_string(rc).decode()
+        raise DgrError("%s failed: %d (%s)" % (what, rc, msg))
+
+
+def ptr_array(ptrs):
+    arr = (ctypes.c_void_p * len(ptrs))(*[ctypes.c_void_p(int(p)) for p in ptrs])
+    return ctypes.cast(arr, ctypes.POINTER(ctypes.c_void_p)), arr
+
+
+def var_ptrs(t):
+    """Device pointers of the 24 rows of a (24, dof) float64 tensor, or of a
+    list of 24 tensors."""
+    if isinstance(t, (list, tuple)):
+        return [x.data_ptr() for x in t]
+    assert t.dim() == 2 and t.shape[0] == NUM_VARS
+    return [t[v].data_ptr() for v in range(NUM_VARS)]
+
+
+def blocks_array(blocks):
+    """blocks: list of dicts or a structured description -> ctypes array."""
+    n = len(blocks)
+    arr = (Block * n)()
+    for i, b in enumerate(blocks):
+        arr[i].offset = int(b["offset"])
+        arr[i].sz[:] = [int(s) for s in b["sz"]]
+        arr[i].bflag = int(b["bflag"])
+        arr[i].pmin[:] = [float(x) for x in b["pmin"]]
+        arr[i].pmax[:] = [float(x) for x in b["pmax"]]
+    return arr
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return ctypes.c_void_p(int(getattr(stream, "cuda_stream", stream)))
+
+
+class Plan:
+    """Device-resident block table + tile work table (dgr_plan_create)."""
+
+    def __init__(self, blocks, params, device=0):
+        L = lib()
+        check(L.dgr_init(device), "dgr_init")
+        self._barr = blocks_array(blocks)
+        self.params = params
+        h = ctypes.c_void_p()
+        check(L.dgr_plan_create(ctypes.byref(h), self._barr, len(blocks), ctypes.byref(params)), "dgr_plan_create")
+        self.handle = h
+        self.nblocks = len(blocks)
+
+    @property
+    def points(self):
+        return int(lib().dgr_plan_points(self.handle))
+
+    @property
+    def tiles(self):
+        return int(lib().dgr_plan_tiles(self.handle))
+
+    def rhs(self, rhs, u, stream=None, per_block=False):
+        L = lib()
+        rp, _r = ptr_array(var_ptrs(rhs))
+        up, _u = ptr_array(var_ptrs(u))
+        f = L.dgr_bssn_rhs_per_block if per_block else L.dgr_bssn_rhs
+        check(f(self.handle, rp, up, _stream_ptr(stream)), f.__name__)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().dgr_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rhs_blocks(rhs, u, blocks, params, stream=None):
+    """One-shot bssnRHS over device arrays (dgr_bssn_rhs_blocks)."""
+    L = lib()
+    barr = blocks_array(blocks)
+    rp, _r = ptr_array(var_ptrs(rhs))
+    up, _u = ptr_array(var_ptrs(u))
+    check(L.dgr_bssn_rhs_blocks(rp, up, barr, len(blocks), ctypes.byref(params), _stream_ptr(stream)),
+          "dgr_bssn_rhs_blocks")

@@ -1,0 +1,107 @@
+/*
+ * dgr_bssn.h -- C ABI of the MI355X BSSN right-hand-side library
+ * (libdgr_bssn.so).
+ *
+ * Drop-in for Dendro-5's BSSN RHS on unzipped (ghost-padded) octree blocks.
+ * Every entry point takes plain pointers and sizes; device arrays are HIP
+ * device pointers and `stream` is a hipStream_t passed as void* (NULL = the
+ * default stream).  All calls return 0 on success or a negative DGR_E_*
+ * code; dgr_error_string() names it.  Nothing here calls exit().
+ *
+ * Data layout (same as the reference): 24 variables, each a flat double
+ * array of the unzipped DOF; block b occupies sz[0]*sz[1]*sz[2] consecutive
+ * doubles at `offset`, x fastest, IDX(i,j,k) = i + sz0*(j + sz1*k)
+ * (BSSN_GR/include/rhs.h:21).  Variables are in VAR order
+ * (BSSN_GR/include/grDef.h:31): alpha, chi, K, Gt0-2, beta0-2, B0-2,
+ * gt0-5, At0-5.  Only interior points [3, sz-3)^3 of the right-hand side
+ * are written; padding is left untouched (BSSN_GR/src/rhs.cpp:210-216).
+ */
+#ifndef DGR_BSSN_H
+#define DGR_BSSN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGR_NUM_VARS 24
+
+enum {
+  DGR_OK = 0,
+  DGR_E_INVALID = -1,   /* bad argument (null pointer, size out of range) */
+  DGR_E_HIP = -2,       /* a HIP runtime call failed */
+  DGR_E_NOMEM = -3,     /* device allocation failed */
+  DGR_E_UNSUPPORTED = -4,
+  DGR_E_CONSTRAINT = -5 /* constraint projection met det<0, |det-1|>1e-6 or |tr A|>1e-6 */
+};
+
+/* Block descriptor: the arguments bssnrhs takes per block
+ * (BSSN_GR/include/rhs.h:47-50): offset, sz, bflag, ptmin/ptmax of the
+ * padded block (ptmin = corner - 3 dx, ptmax = corner + 3 dx,
+ * BSSN_GR/src/rhs.cpp:51-62).  bflag uses the OCT_DIR_* bit numbering
+ * (include/dendro.h:84-89): bit0 -x, bit1 +x, bit2 -y, bit3 +y, bit4 -z,
+ * bit5 +z. */
+typedef struct dgr_block {
+  uint64_t offset;
+  uint32_t sz[3];
+  uint32_t bflag;
+  double pmin[3];
+  double pmax[3];
+} dgr_block;
+
+/* BSSN parameters, passed by value instead of the reference's bssn::
+ * globals (BSSN_GR/include/parameters.h:191-222) or its GPU struct
+ * cuda::BSSNComputeParams (BSSN_GR/cuda_gr/include/params_cu.h:93-108). */
+typedef struct dgr_bssn_params {
+  double lambda[4];       /* BSSN_LAMBDA */
+  double lambda_f[2];     /* BSSN_LAMBDA_F */
+  double eta_const;       /* ETA_CONST */
+  double eta_r0;          /* ETA_R0 */
+  double eta_damping_exp; /* ETA_DAMPING_EXP */
+  double ko_sigma;        /* KO_DISS_SIGMA */
+  double chi_floor;       /* CHI_FLOOR (constraint projection) */
+  uint32_t deriv_order;   /* 42 (reference deriv42) */
+  uint32_t reserved;
+} dgr_bssn_params;
+
+/* Opaque plan: the block table and the tile work table on the device. */
+typedef struct dgr_plan_s* dgr_plan_t;
+
+const char* dgr_error_string(int code);
+
+/* Select the device for this thread and load the kernels. */
+int dgr_init(int device);
+
+/* Build a plan for a block list.  Replaces the per-call setup of
+ * cuda::computeRHS (BSSN_GR/cuda_gr/src/rhs_cuda.cu:35-143): the block table
+ * is uploaded once and split into tiles of at most 7x7x7 interior points.
+ * Blocks need sz >= 13 in every direction. */
+int dgr_plan_create(dgr_plan_t* plan, const dgr_block* blocks, uint32_t nblocks,
+                    const dgr_bssn_params* params);
+int dgr_plan_destroy(dgr_plan_t plan);
+/* Interior points covered by the plan. */
+uint64_t dgr_plan_points(dgr_plan_t plan);
+uint32_t dgr_plan_tiles(dgr_plan_t plan);
+
+/* RHS of all blocks in one batched launch on `stream` (asynchronous).
+ * rhs and u are arrays of 24 device pointers.  Replaces
+ * bssnRHS(double**, const double**, const ot::Block*, unsigned)
+ * (BSSN_GR/include/rhs.h:45) and cuda::computeRHS
+ * (BSSN_GR/cuda_gr/include/rhs_cuda.cuh:36), with device-resident data. */
+int dgr_bssn_rhs(dgr_plan_t plan, double* const* rhs, const double* const* u, void* stream);
+
+/* Same result, one kernel launch per block: the per-block loop of
+ * bssnRHS (BSSN_GR/src/rhs.cpp:43-70) calling bssnrhs for each block
+ * (BSSN_GR/include/rhs.h:47-50).  Kept as the baseline the batched path is
+ * measured against. */
+int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const double* const* u, void* stream);
+
+/* One-shot form of bssnRHS: builds a temporary plan, runs, synchronises. */
+int dgr_bssn_rhs_blocks(double* const* rhs, const double* const* u, const dgr_block* blocks,
+                        uint32_t nblocks, const dgr_bssn_params* params, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

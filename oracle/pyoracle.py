@@ -1,0 +1,126 @@
+"""ctypes loader of the CPU oracle (liboracle.so) -- test infrastructure only.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, as the checker or as the timed CPU baseline.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NUM_VARS = 24
+NIN = 207
+dp = ctypes.POINTER(ctypes.c_double)
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("lambda_", ctypes.c_double * 4), ("lambda_f", ctypes.c_double * 2),
+                ("eta_const", ctypes.c_double), ("eta_r0", ctypes.c_double),
+                ("eta_damping_exp", ctypes.c_double), ("ko_sigma", ctypes.c_double)]
+
+
+def params(lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0), eta_const=2.0, eta_r0=30.0, eta_damping_exp=1.0,
+           ko_sigma=0.4):
+    p = Params()
+    p.lambda_[:] = [float(x) for x in lambda_]
+    p.lambda_f[:] = [float(x) for x in lambda_f]
+    p.eta_const, p.eta_r0, p.eta_damping_exp, p.ko_sigma = eta_const, eta_r0, eta_damping_exp, ko_sigma
+    return p
+
+
+_libs = {}
+
+
+def load(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name not in _libs:
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            subprocess.check_call(["make", "-s", "all"] + (["ORACLE_FAST=1"] if fast else []), cwd=HERE)
+        L = ctypes.CDLL(path)
+        L.orc_deriv.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_double,
+                                ctypes.POINTER(ctypes.c_uint), ctypes.c_uint, dp]
+        L.orc_bssnrhs_blocks.argtypes = [ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint),
+                                         ctypes.POINTER(ctypes.c_uint), dp, dp, ctypes.POINTER(Params), ctypes.c_int]
+        L.orc_point_eqs.argtypes = [ctypes.c_long, dp, dp, ctypes.POINTER(Params), dp]
+        L.orc_enforce_constraints.argtypes = [ctypes.POINTER(dp), ctypes.c_long, ctypes.c_double]
+        L.orc_enforce_constraints.restype = ctypes.c_int
+        L.orc_rk4_stage.argtypes = [dp, dp, dp, ctypes.c_double, ctypes.c_long]
+        L.orc_rk4_final.argtypes = [dp, dp, ctypes.POINTER(dp), ctypes.c_double, ctypes.c_long]
+        _libs[name] = L
+    return _libs[name]
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(dp)
+
+
+def _rows(U):
+    return (dp * U.shape[0])(*[_p(U[v]) for v in range(U.shape[0])]) if U.ndim == 2 else None
+
+
+def deriv(kind, axis, u, h, sz, bflag=0, beta=None, fill=np.nan):
+    """One reference operator on one padded block (see bssn_oracle.h)."""
+    L = load()
+    u = np.ascontiguousarray(u, dtype=np.float64).ravel()
+    out = np.full_like(u, fill)
+    szc = (ctypes.c_uint * 3)(*sz)
+    b = None if beta is None else _p(np.ascontiguousarray(beta, dtype=np.float64).ravel())
+    L.orc_deriv(kind, axis, _p(out), _p(u), h, szc, bflag, b)
+    return out
+
+
+def rhs_blocks(U, blocks, prm=None, nthreads=0, fast=False, fill=np.nan, out=None):
+    """bssnRHS over all blocks of a (24, dof) array; non-interior points of
+    the result keep `fill` (or whatever `out` held)."""
+    L = load(fast)
+    prm = prm or params()
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    R = np.full_like(U, fill) if out is None else out
+    nb = len(blocks)
+    offs = np.array([b["offset"] for b in blocks], dtype=np.uint64)
+    sz = np.array([b["sz"] for b in blocks], dtype=np.uint32).ravel()
+    bf = np.array([b["bflag"] for b in blocks], dtype=np.uint32)
+    pmin = np.array([b["pmin"] for b in blocks], dtype=np.float64).ravel()
+    pmax = np.array([b["pmax"] for b in blocks], dtype=np.float64).ravel()
+    L.orc_bssnrhs_blocks(_rows(R), _rows(U), nb, offs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                         sz.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
+                         bf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), _p(pmin), _p(pmax),
+                         ctypes.byref(prm), nthreads)
+    return R
+
+
+def point_eqs(inp, eta, prm=None):
+    L = load()
+    prm = prm or params()
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    n = inp.shape[1]
+    out = np.zeros((NUM_VARS, n))
+    L.orc_point_eqs(n, _p(inp), _p(np.ascontiguousarray(eta, dtype=np.float64)), ctypes.byref(prm), _p(out))
+    return out
+
+
+def enforce_constraints(U, chi_floor=0.1):
+    L = load()
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    bad = L.orc_enforce_constraints(_rows(U), U.shape[1], chi_floor)
+    return U, bad
+
+
+def rk4_stage(u_prev, k, cdt):
+    L = load()
+    out = np.empty_like(u_prev)
+    L.orc_rk4_stage(_p(out), _p(u_prev), _p(k), cdt, u_prev.size)
+    return out
+
+
+def rk4_final(u_prev, ks, dt):
+    L = load()
+    out = np.empty_like(u_prev)
+    arr = (dp * 4)(*[_p(k) for k in ks])
+    L.orc_rk4_final(_p(out), _p(u_prev), arr, dt, u_prev.size)
+    return out

@@ -1,0 +1,48 @@
+"""The C-ABI library loads and exports every entry point include/dgr_bssn.h
+declares (no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "dgr_bssn.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(dgr_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    for s in ("dgr_bssn_rhs", "dgr_bssn_rhs_per_block", "dgr_plan_create", "dgr_init"):
+        assert s in syms
+
+
+def test_library_exports_all_declared_symbols():
+    from pydgr import native
+    L = ctypes.CDLL(native.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_error_strings():
+    from pydgr import native
+    L = native.lib()
+    assert L.dgr_error_string(0) == b"ok"
+    assert L.dgr_error_string(-1) == b"invalid argument"
+
+
+def test_plan_rejects_bad_input_without_device_work():
+    from pydgr import native
+    L = native.lib()
+    h = ctypes.c_void_p()
+    p = native.make_params()
+    assert L.dgr_plan_create(ctypes.byref(h), None, 0, ctypes.byref(p)) == -1
+    b = native.blocks_array([{"offset": 0, "sz": [9, 13, 13], "bflag": 0, "pmin": [0, 0, 0], "pmax": [1, 1, 1]}])
+    assert L.dgr_plan_create(ctypes.byref(h), b, 1, ctypes.byref(p)) == -1
+    p.deriv_order = 64
+    b = native.blocks_array([{"offset": 0, "sz": [13, 13, 13], "bflag": 0, "pmin": [0, 0, 0], "pmax": [1, 1, 1]}])
+    assert L.dgr_plan_create(ctypes.byref(h), b, 1, ctypes.byref(p)) == -4
