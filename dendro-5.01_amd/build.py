@@ -32,7 +32,8 @@ def build_hip():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libdgr_bssn.so")
     tmp = out + ".tmp"
-    run([HIPCC] + HIP_FLAGS + ["-o", tmp, os.path.join(PKG, "csrc", "dgr_bssn.hip")])
+    run([HIPCC] + HIP_FLAGS + ["-o", tmp, os.path.join(PKG, "csrc", "dgr_bssn.hip"),
+                               os.path.join(PKG, "host", "bssn_rhs_host.cpp")])
     os.replace(tmp, out)
     return out
 
@@ -47,9 +48,13 @@ def build_oracle(with_ref=True):
 
 
 def build_test_harness():
-    src = os.path.join(ROOT, "tests", "native", "staged_host.cpp")
-    out = os.path.join(ROOT, "tests", "native", "libstaged_host.so")
-    run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-fPIC", "-shared", "-o", out, src])
+    nat = os.path.join(ROOT, "tests", "native")
+    run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-fPIC", "-shared", "-o",
+         os.path.join(nat, "libstaged_host.so"), os.path.join(nat, "staged_host.cpp")])
+    # host-shim driver: links the product library and the oracle
+    run([HIPCC, "-O2", "-std=c++17", "-o", os.path.join(nat, "shim_test"), os.path.join(nat, "shim_test.cpp"),
+         "-L" + LIB, "-ldgr_bssn", "-L" + os.path.join(ROOT, "oracle"), "-loracle",
+         "-Wl,-rpath," + LIB, "-Wl,-rpath," + os.path.join(ROOT, "oracle")])
 
 
 def main(argv=None):

@@ -60,20 +60,36 @@ struct Params {
   double lambda_f[2];
 };
 
-struct Point {
+// Storage of the long-lived per-point arrays: plain registers by default;
+// the GPU kernel can put them in LDS (one slot per thread) instead.
+template <int N>
+struct RegArr {
+  double v[N];
+  DGR_HD double& operator[](int i) { return v[i]; }
+  DGR_HD const double& operator[](int i) const { return v[i]; }
+};
+struct RegStore {
+  template <int N>
+  using A = RegArr<N>;
+};
+
+template <class S>
+struct PointT {
   // state at the point (Gt itself enters only through derivatives)
   double alpha, chi, K;
   double gt[6], At[6], beta[3], B[3];
   // derived
   double igt[6];
   double AUU[6];
-  double dgt[6][3];
-  double C2[3][6];
+  // d_k gt_p is kept in the storage of C2 until finish_metric() turns it
+  // into the Christoffels: dgt(p,k) = C2[p*3 + k]
+  typename S::template A<18> C2;  // C2[l*6 + p]
   double CalGt[3];
   double dchi[3];
-  double R[6];
+  typename S::template A<6> R;
   // right-hand-side accumulators
-  double gt_rhs[6], At_rhs[6], Gt_rhs[3], Bacc[3];
+  typename S::template A<6> gt_rhs, At_rhs;
+  double Gt_rhs[3], Bacc[3];
   double chi_rhs, K_rhs;
   double divb;
   double wb[3];
@@ -108,7 +124,7 @@ struct Point {
 
   // -1/2 g^{lm} d_l d_m gt_ij (first Rt term of dendro.py compute_ricci)
   DGR_HD void add_gt(int p, const double d1[3], const double d2[6], double adv) {
-    dgt[p][0] = d1[0]; dgt[p][1] = d1[1]; dgt[p][2] = d1[2];
+    C2[p * 3 + 0] = d1[0]; C2[p * 3 + 1] = d1[1]; C2[p * 3 + 2] = d1[2];
     const double s = igt[0] * d2[0] + igt[3] * d2[3] + igt[5] * d2[5] +
                      2.0 * (igt[1] * d2[1] + igt[2] * d2[2] + igt[4] * d2[4]);
     R[p] -= 0.5 * s;
@@ -116,6 +132,9 @@ struct Point {
   }
 
   DGR_HD void finish_metric() {
+    double dgt[6][3];
+    for (int p = 0; p < 6; ++p)
+      for (int k = 0; k < 3; ++k) dgt[p][k] = C2[p * 3 + k];
     // C1[k][(i,j)] = 1/2 (d_j g_ki + d_i g_kj - d_k g_ij)
     double C1[3][6];
     for (int k = 0; k < 3; ++k)
@@ -125,10 +144,10 @@ struct Point {
       }
     for (int l = 0; l < 3; ++l)
       for (int p = 0; p < 6; ++p)
-        C2[l][p] = igt[sp(l, 0)] * C1[0][p] + igt[sp(l, 1)] * C1[1][p] + igt[sp(l, 2)] * C1[2][p];
+        C2[l * 6 + p] = igt[sp(l, 0)] * C1[0][p] + igt[sp(l, 1)] * C1[1][p] + igt[sp(l, 2)] * C1[2][p];
     for (int i = 0; i < 3; ++i)
-      CalGt[i] = igt[0] * C2[i][0] + igt[3] * C2[i][3] + igt[5] * C2[i][5] +
-                 2.0 * (igt[1] * C2[i][1] + igt[2] * C2[i][2] + igt[4] * C2[i][4]);
+      CalGt[i] = igt[0] * C2[i * 6 + 0] + igt[3] * C2[i * 6 + 3] + igt[5] * C2[i * 6 + 5] +
+                 2.0 * (igt[1] * C2[i * 6 + 1] + igt[2] * C2[i * 6 + 2] + igt[4] * C2[i * 6 + 4]);
     // 1/2 Gamma^k (C1[i,j,k] + C1[j,i,k]) = 1/2 Gamma^k d_k gt_ij
     for (int p = 0; p < 6; ++p)
       R[p] += 0.5 * (CalGt[0] * dgt[p][0] + CalGt[1] * dgt[p][1] + CalGt[2] * dgt[p][2]);
@@ -144,16 +163,16 @@ struct Point {
           for (int p = 0; p < 6; ++p) {
             const int i = spi(p), j = spj(p);
             double q = 0.0;
-            if (a == j) q += C2[b][sp(c, i)];
-            if (a == i) q += C2[b][sp(c, j)];
-            if (b == j) q += C2[a][sp(i, c)];
+            if (a == j) q += C2[b * 6 + sp(c, i)];
+            if (a == i) q += C2[b * 6 + sp(c, j)];
+            if (b == j) q += C2[a * 6 + sp(i, c)];
             R[p] += q * U;
           }
         }
     // 2 alpha Gamma^i_jk A^{jk} in Gt_rhs
     for (int i = 0; i < 3; ++i) {
-      const double s = C2[i][0] * AUU[0] + C2[i][3] * AUU[3] + C2[i][5] * AUU[5] +
-                       2.0 * (C2[i][1] * AUU[1] + C2[i][2] * AUU[2] + C2[i][4] * AUU[4]);
+      const double s = C2[i * 6 + 0] * AUU[0] + C2[i * 6 + 3] * AUU[3] + C2[i * 6 + 5] * AUU[5] +
+                       2.0 * (C2[i * 6 + 1] * AUU[1] + C2[i * 6 + 2] * AUU[2] + C2[i * 6 + 4] * AUU[4]);
       Gt_rhs[i] += 2.0 * alpha * s;
     }
   }
@@ -168,7 +187,7 @@ struct Point {
     s -= CalGt[0] * d1[0] + CalGt[1] * d1[1] + CalGt[2] * d1[2];
     for (int p = 0; p < 6; ++p) {
       const int i = spi(p), j = spj(p);
-      const double c = C2[0][p] * d1[0] + C2[1][p] * d1[1] + C2[2][p] * d1[2];
+      const double c = C2[0 * 6 + p] * d1[0] + C2[1 * 6 + p] * d1[1] + C2[2 * 6 + p] * d1[2];
       R[p] += h * (d2[p] - c) - 0.25 * ichi * ichi * d1[i] * d1[j] + h * gt[p] * s;
     }
     chi_rhs = adv;
@@ -186,7 +205,7 @@ struct Point {
     double lap = 0.0;
     for (int p = 0; p < 6; ++p) {
       const int i = spi(p), j = spj(p);
-      const double c3 = C2[0][p] * d1[0] + C2[1][p] * d1[1] + C2[2][p] * d1[2] -
+      const double c3 = C2[0 * 6 + p] * d1[0] + C2[1 * 6 + p] * d1[1] + C2[2 * 6 + p] * d1[2] -
                         0.5 * ichi * (d1[i] * dchi[j] + d1[j] * dchi[i] - gt[p] * gca);
       DD[p] = d2[p] - c3;
       lap += spw(p) * igt[p] * DD[p];
@@ -269,6 +288,8 @@ struct Point {
     return adv + At_rhs[p] + (-2.0 / 3.0) * At[p] * divb + alpha * (K * At[p] - 2.0 * AA);
   }
 };
+
+using Point = PointT<RegStore>;
 
 }  // namespace dgr
 

@@ -100,47 +100,50 @@ struct DevParams {
 // order follow BSSN_GR/src/derivs.cpp (deriv42_x :29-65, deriv42_xx
 // :265-316, deriv42adv_x :513-622, ko_deriv42_x :937-1060).
 // ---------------------------------------------------------------------------
+template <bool BND>
 __device__ __forceinline__ double d1_42(const double* A, int o, int s, int q, int e, bool lf, bool hf,
                                         double ih2, double ih12) {
 #pragma clang fp contract(off)
-  if (hf && q == e - 1) return (A[o + (e - 3) * s] - 4.0 * A[o + (e - 2) * s] + 3.0 * A[o + (e - 1) * s]) * ih2;
-  if (hf && q == e - 2) return (-A[o + (e - 3) * s] + A[o + (e - 1) * s]) * ih2;
-  if (lf && q == 4) return (-A[o + 3 * s] + A[o + 5 * s]) * ih2;
-  if (lf && q == 3) return (-3.0 * A[o + 3 * s] + 4.0 * A[o + 4 * s] - A[o + 5 * s]) * ih2;
+  if (BND && hf && q == e - 1) return (A[o + (e - 3) * s] - 4.0 * A[o + (e - 2) * s] + 3.0 * A[o + (e - 1) * s]) * ih2;
+  if (BND && hf && q == e - 2) return (-A[o + (e - 3) * s] + A[o + (e - 1) * s]) * ih2;
+  if (BND && lf && q == 4) return (-A[o + 3 * s] + A[o + 5 * s]) * ih2;
+  if (BND && lf && q == 3) return (-3.0 * A[o + 3 * s] + 4.0 * A[o + 4 * s] - A[o + 5 * s]) * ih2;
   return (A[o + (q - 2) * s] - 8.0 * A[o + (q - 1) * s] + 8.0 * A[o + (q + 1) * s] - A[o + (q + 2) * s]) * ih12;
 }
 
+template <bool BND>
 __device__ __forceinline__ double d2_42(const double* A, int o, int s, int q, int e, bool lf, bool hf,
                                         double isq, double isq12) {
 #pragma clang fp contract(off)
-  if (hf && q == e - 1)
+  if (BND && hf && q == e - 1)
     return (-A[o + (e - 4) * s] + 4.0 * A[o + (e - 3) * s] - 5.0 * A[o + (e - 2) * s] + 2.0 * A[o + (e - 1) * s]) * isq;
-  if (hf && q == e - 2) return (A[o + (e - 3) * s] - 2.0 * A[o + (e - 2) * s] + A[o + (e - 1) * s]) * isq;
-  if (lf && q == 4) return (A[o + 3 * s] - 2.0 * A[o + 4 * s] + A[o + 5 * s]) * isq;
-  if (lf && q == 3) return (2.0 * A[o + 3 * s] - 5.0 * A[o + 4 * s] + 4.0 * A[o + 5 * s] - A[o + 6 * s]) * isq;
+  if (BND && hf && q == e - 2) return (A[o + (e - 3) * s] - 2.0 * A[o + (e - 2) * s] + A[o + (e - 1) * s]) * isq;
+  if (BND && lf && q == 4) return (A[o + 3 * s] - 2.0 * A[o + 4 * s] + A[o + 5 * s]) * isq;
+  if (BND && lf && q == 3) return (2.0 * A[o + 3 * s] - 5.0 * A[o + 4 * s] + 4.0 * A[o + 5 * s] - A[o + 6 * s]) * isq;
   return (-A[o + (q - 2) * s] + 16.0 * A[o + (q - 1) * s] - 30.0 * A[o + q * s] + 16.0 * A[o + (q + 1) * s] - A[o + (q + 2) * s]) * isq12;
 }
 
+template <bool BND>
 __device__ __forceinline__ double adv_42(const double* A, int o, int s, int q, int e, bool lf, bool hf, double b,
                                          double ih2, double ih12) {
 #pragma clang fp contract(off)
   const bool pos = b > 0.0;
-  if (hf && q == e - 1) return (A[o + (e - 3) * s] - 4.0 * A[o + (e - 2) * s] + 3.0 * A[o + (e - 1) * s]) * ih2;
-  if (hf && q == e - 2) {
+  if (BND && hf && q == e - 1) return (A[o + (e - 3) * s] - 4.0 * A[o + (e - 2) * s] + 3.0 * A[o + (e - 1) * s]) * ih2;
+  if (BND && hf && q == e - 2) {
     if (pos) return (-A[o + (e - 3) * s] + A[o + (e - 1) * s]) * ih2;
     return (A[o + (e - 4) * s] - 4.0 * A[o + (e - 3) * s] + 3.0 * A[o + (e - 2) * s]) * ih2;
   }
-  if (hf && q == e - 3) {
+  if (BND && hf && q == e - 3) {
     if (pos) return (-3.0 * A[o + (e - 3) * s] + 4.0 * A[o + (e - 2) * s] - A[o + (e - 1) * s]) * ih2;
     return (-A[o + (e - 6) * s] + 6.0 * A[o + (e - 5) * s] - 18.0 * A[o + (e - 4) * s] + 10.0 * A[o + (e - 3) * s] +
             3.0 * A[o + (e - 2) * s]) * ih12;
   }
-  if (lf && q == 3) return (-3.0 * A[o + 3 * s] + 4.0 * A[o + 4 * s] - A[o + 5 * s]) * ih2;
-  if (lf && q == 4) {
+  if (BND && lf && q == 3) return (-3.0 * A[o + 3 * s] + 4.0 * A[o + 4 * s] - A[o + 5 * s]) * ih2;
+  if (BND && lf && q == 4) {
     if (pos) return (-3.0 * A[o + 4 * s] + 4.0 * A[o + 5 * s] - A[o + 6 * s]) * ih2;
     return (-A[o + 3 * s] + A[o + 5 * s]) * ih2;
   }
-  if (lf && q == 5) {
+  if (BND && lf && q == 5) {
     if (pos) return (-3.0 * A[o + 4 * s] - 10.0 * A[o + 5 * s] + 18.0 * A[o + 6 * s] - 6.0 * A[o + 7 * s] + A[o + 8 * s]) * ih12;
     return (A[o + 3 * s] - 4.0 * A[o + 4 * s] + 3.0 * A[o + 5 * s]) * ih2;
   }
@@ -151,21 +154,22 @@ __device__ __forceinline__ double adv_42(const double* A, int o, int s, int q, i
           3.0 * A[o + (q + 1) * s]) * ih12;
 }
 
+template <bool BND>
 __device__ __forceinline__ double ko_42(const double* A, int o, int s, int q, int e, bool lf, bool hf, double pre,
                                         double smr1, double smr2, double smr3) {
 #pragma clang fp contract(off)
-  if (hf && q == e - 1)
+  if (BND && hf && q == e - 1)
     return (A[o + (e - 4) * s] - 3.0 * A[o + (e - 3) * s] + 3.0 * A[o + (e - 2) * s] - A[o + (e - 1) * s]) / smr3;
-  if (hf && q == e - 2)
+  if (BND && hf && q == e - 2)
     return (A[o + (e - 5) * s] - 6.0 * A[o + (e - 4) * s] + 12.0 * A[o + (e - 3) * s] - 10.0 * A[o + (e - 2) * s] +
             3.0 * A[o + (e - 1) * s]) / smr2;
-  if (hf && q == e - 3)
+  if (BND && hf && q == e - 3)
     return (A[o + (e - 6) * s] - 6.0 * A[o + (e - 5) * s] + 15.0 * A[o + (e - 4) * s] - 19.0 * A[o + (e - 3) * s] +
             12.0 * A[o + (e - 2) * s] - 3.0 * A[o + (e - 1) * s]) / smr1;
-  if (lf && q == 3) return (A[o + 6 * s] - 3.0 * A[o + 5 * s] + 3.0 * A[o + 4 * s] - A[o + 3 * s]) / smr3;
-  if (lf && q == 4)
+  if (BND && lf && q == 3) return (A[o + 6 * s] - 3.0 * A[o + 5 * s] + 3.0 * A[o + 4 * s] - A[o + 3 * s]) / smr3;
+  if (BND && lf && q == 4)
     return (A[o + 7 * s] - 6.0 * A[o + 6 * s] + 12.0 * A[o + 5 * s] - 10.0 * A[o + 4 * s] + 3.0 * A[o + 3 * s]) / smr2;
-  if (lf && q == 5)
+  if (BND && lf && q == 5)
     return (A[o + 8 * s] - 6.0 * A[o + 7 * s] + 15.0 * A[o + 6 * s] - 19.0 * A[o + 5 * s] + 12.0 * A[o + 4 * s] - 3.0 * A[o + 3 * s]) /
            smr1;
   return pre * (-A[o + (q - 3) * s] + 6.0 * A[o + (q - 2) * s] - 15.0 * A[o + (q - 1) * s] + 20.0 * A[o + q * s] -
@@ -244,7 +248,6 @@ struct Ctx {
   uint64_t o;  // global index of the point
   const AxisC* ax;  // per-axis constants of the block (device memory)
   dgr::Params P;
-  double ex_gt[6], ex_Gt[3], ex_chi;  // dissipation (+ Sommerfeld) held until the output is final
 };
 
 // rhs = eq + (KO), or (Sommerfeld + KO) on flagged faces (rhs.cpp:265-373)
@@ -253,9 +256,29 @@ __device__ __forceinline__ void emit(const IOPtrs* __restrict__ io, const Ctx& C
   const double r = C.on_face ? ex : (eq + ex);
   __builtin_nontemporal_store(r, io->r.v[v] + C.o);
 }
+// Outputs that become final several variables later park their dissipation
+// (+ Sommerfeld) term in the output array and pick it up at emission.
+__device__ __forceinline__ void park(const IOPtrs* __restrict__ io, const Ctx& C, int v, double ex) {
+  io->r.v[v][C.o] = ex;
+}
+__device__ __forceinline__ void emit_parked(const IOPtrs* __restrict__ io, const Ctx& C, int v, double eq) {
+  emit(io, C, v, eq, io->r.v[v][C.o]);
+}
 
-template <int V>
-__device__ __forceinline__ void process_var(dgr::Point& s, Ctx& C, const double* su, const double* sgx,
+template <int N>
+struct LdsArr {
+  double* p;
+  __device__ __forceinline__ double& operator[](int i) { return p[i * NT]; }
+  __device__ __forceinline__ const double& operator[](int i) const { return p[i * NT]; }
+};
+struct LdsStore {
+  template <int N>
+  using A = LdsArr<N>;
+};
+using KPoint = dgr::PointT<LdsStore>;
+
+template <int V, bool BND>
+__device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su, const double* sgx,
                                             const double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io) {
   using namespace dgr;
   if (!C.act) return;
@@ -271,45 +294,47 @@ __device__ __forceinline__ void process_var(dgr::Point& s, Ctx& C, const double*
     // x and y derivatives at the point are read back from the LDS planes
     d1[0] = sgx[ti + TX * ((tj + 2) + GXY * (tk + 2))];
     d1[1] = sgy[ti + TX * (tj + TY * (tk + 2))];
-    d1[2] = d1_42(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
-    d2[0] = d2_42(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, ax[0].isq, ax[0].isq12);
-    d2[3] = d2_42(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].isq, ax[1].isq12);
-    d2[5] = d2_42(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].isq, ax[2].isq12);
+    d1[2] = d1_42<BND>(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
+    d2[0] = d2_42<BND>(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, ax[0].isq, ax[0].isq12);
+    d2[3] = d2_42<BND>(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].isq, ax[1].isq12);
+    d2[5] = d2_42<BND>(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].isq, ax[2].isq12);
     // mixed derivatives by composition: d_y(d_x u), d_z(d_x u), d_z(d_y u).
     // sgx[a + TX*(b + GXY*cc)] holds d_x u at block coords (i0+a, j0-2+b, k0-2+cc);
     // sgy[a + TX*(b + TY*cc)] holds d_y u at (i0+a, j0+b, k0-2+cc).
     const int oxy = ti + TX * ((2 - T.j0) + GXY * (tk + 2));
-    d2[1] = d1_42(sgx, oxy, TX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
+    d2[1] = d1_42<BND>(sgx, oxy, TX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
     const int oxz = ti + TX * ((tj + 2) + GXY * (2 - T.k0));
-    d2[2] = d1_42(sgx, oxz, TX * GXY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
+    d2[2] = d1_42<BND>(sgx, oxz, TX * GXY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
     const int oyz = ti + TX * (tj + TY * (2 - T.k0));
-    d2[4] = d1_42(sgy, oyz, TX * TY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
+    d2[4] = d1_42<BND>(sgy, oyz, TX * TY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
   } else {
-    if (has_d1(V) || C.bflag != 0u) {
-      d1[0] = d1_42(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, ax[0].ih2, ax[0].ih12);
-      d1[1] = d1_42(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
-      d1[2] = d1_42(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
+    if (has_d1(V) || BND) {
+      d1[0] = d1_42<BND>(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, ax[0].ih2, ax[0].ih12);
+      d1[1] = d1_42<BND>(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
+      d1[2] = d1_42<BND>(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
     }
   }
+  __builtin_amdgcn_sched_barrier(0);
   // upwinded derivatives contracted with beta, summed in the order of the
   // reference's printed expression: beta0*a0 + beta1*a1 + beta2*a2
   double adv;
   {
-    const double a0 = adv_42(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, s.beta[0], ax[0].ih2, ax[0].ih12);
-    const double a1 = adv_42(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, s.beta[1], ax[1].ih2, ax[1].ih12);
-    const double a2 = adv_42(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, s.beta[2], ax[2].ih2, ax[2].ih12);
+    const double a0 = adv_42<BND>(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, s.beta[0], ax[0].ih2, ax[0].ih12);
+    const double a1 = adv_42<BND>(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, s.beta[1], ax[1].ih2, ax[1].ih12);
+    const double a2 = adv_42<BND>(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, s.beta[2], ax[2].ih2, ax[2].ih12);
     adv = s.beta[0] * a0 + s.beta[1] * a1 + s.beta[2] * a2;
   }
+  __builtin_amdgcn_sched_barrier(0);
   // Kreiss-Oliger sum, preceded on flagged faces by the Sommerfeld value
   double ex;
   {
 #pragma clang fp contract(off)
-    const double k0 = ko_42(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, ax[0].pre, ax[0].smr1, ax[0].smr2, ax[0].smr3);
-    const double k1 = ko_42(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].pre, ax[1].smr1, ax[1].smr2, ax[1].smr3);
-    const double k2 = ko_42(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].pre, ax[2].smr1, ax[2].smr2,
+    const double k0 = ko_42<BND>(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, ax[0].pre, ax[0].smr1, ax[0].smr2, ax[0].smr3);
+    const double k1 = ko_42<BND>(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].pre, ax[1].smr1, ax[1].smr2, ax[1].smr3);
+    const double k2 = ko_42<BND>(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].pre, ax[2].smr1, ax[2].smr2,
                             ax[2].smr3);
     const double ko = C.sigma * (k0 + k1 + k2);
-    if (C.on_face) {
+    if (BND && C.on_face) {
       const double f = su[c];
       const double bc = -C.inv_r * (C.x * d1[0] + C.y * d1[1] + C.z * d1[2] + c_falloff[V] * (f - c_asym[V]));
       ex = bc + ko;
@@ -317,39 +342,41 @@ __device__ __forceinline__ void process_var(dgr::Point& s, Ctx& C, const double*
       ex = ko;
     }
   }
+  __builtin_amdgcn_sched_barrier(0);
   // feed the staged equations; emit every right-hand side that is final
   if constexpr (V >= V_SG0 && V <= V_SG5) {
     s.add_gt(V - V_SG0, d1, d2, adv);
-    C.ex_gt[V - V_SG0] = ex;
+    park(io, C, V, ex);
   }
   if constexpr (V == V_CHI) {
     s.add_chi(d1, d2, adv);
-    C.ex_chi = ex;
+    park(io, C, V, ex);
   }
   if constexpr (V == V_ALPHA) emit(io, C, V_ALPHA, s.add_alpha(C.P, d1, d2, adv), ex);
   if constexpr (V >= V_GT0 && V <= V_GT2) {
     s.add_Gt(C.P, V - V_GT0, d1, adv);
-    C.ex_Gt[V - V_GT0] = ex;
+    park(io, C, V, ex);
   }
   if constexpr (V >= V_BETA0 && V <= V_BETA2) {
     double gto[6], chio = 0.0;
     emit(io, C, V, s.add_beta(C.P, V - V_BETA0, d1, d2, adv, gto, &chio), ex);
     if constexpr (V == V_BETA2) {
 #pragma unroll
-      for (int p = 0; p < 6; ++p) emit(io, C, V_SG0 + p, gto[p], C.ex_gt[p]);
-      emit(io, C, V_CHI, chio, C.ex_chi);
+      for (int p = 0; p < 6; ++p) emit_parked(io, C, V_SG0 + p, gto[p]);
+      emit_parked(io, C, V_CHI, chio);
     }
   }
   if constexpr (V == V_K) {
     emit(io, C, V_K, s.add_K(d1, adv), ex);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) emit(io, C, V_GT0 + k, s.Gt_rhs[k], C.ex_Gt[k]);
+    for (int k = 0; k < 3; ++k) emit_parked(io, C, V_GT0 + k, s.Gt_rhs[k]);
   }
   if constexpr (V >= V_B0 && V <= V_B2) emit(io, C, V, s.add_B(C.P, V - V_B0, adv, C.eta), ex);
   if constexpr (V >= V_SA0 && V <= V_SA5) emit(io, C, V, s.add_At(V - V_SA0, adv), ex);
 }
 
 // Widened first-derivative planes for the mixed derivatives.
+template <bool BND>
 __device__ __forceinline__ void deriv_planes(const double* su, double* sgx, double* sgy, const TileGeom& T,
                                              const AxisC* ax, int t) {
   // sgx: d_x u at block coords (i0+a, j0-2+b, k0-2+cc), su coords (a+3, b+1, cc+1)
@@ -360,7 +387,7 @@ __device__ __forceinline__ void deriv_planes(const double* su, double* sgx, doub
     const int b = bc % (T.ny + 4);
     const int cc = bc / (T.ny + 4);
     const int o = (a + 3) + SX * ((b + 1) + SY * (cc + 1)) - (T.i0 + a);
-    sgx[a + TX * (b + GXY * cc)] = d1_42(su, o, 1, T.i0 + a, ax[0].e, ax[0].lf, ax[0].hf, ax[0].ih2, ax[0].ih12);
+    sgx[a + TX * (b + GXY * cc)] = d1_42<BND>(su, o, 1, T.i0 + a, ax[0].e, ax[0].lf, ax[0].hf, ax[0].ih2, ax[0].ih12);
   }
   // sgy: d_y u at block coords (i0+a, j0+b, k0-2+cc), su coords (a+3, b+3, cc+1)
   const int nyg = T.nx * T.ny * (T.nz + 4);
@@ -370,36 +397,40 @@ __device__ __forceinline__ void deriv_planes(const double* su, double* sgx, doub
     const int b = bc % T.ny;
     const int cc = bc / T.ny;
     const int o = (a + 3) + SX * ((b + 3) + SY * (cc + 1)) - (T.j0 + b) * SX;
-    sgy[a + TX * (b + TY * cc)] = d1_42(su, o, SX, T.j0 + b, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
+    sgy[a + TX * (b + TY * cc)] = d1_42<BND>(su, o, SX, T.j0 + b, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
   }
 }
 
-template <int I>
-__device__ __forceinline__ void run_vars(dgr::Point& s, Ctx& C, double (&pf)[NPF], double* su, double* sgx,
+template <int I, bool BND>
+__device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], double* su, double* sgx,
                                          double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io,
                                          int t) {
   if constexpr (I < 24) {
     constexpr int V = h_order[I];
     stage(pf, su, T, t);
     __syncthreads();
-    if constexpr (has_d2(V)) deriv_planes(su, sgx, sgy, T, C.ax, t);
+    if constexpr (has_d2(V)) deriv_planes<BND>(su, sgx, sgy, T, C.ax, t);
     if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, t);
     if constexpr (has_d2(V)) __syncthreads();
     if constexpr (V == dgr::V_CHI) {
       if (C.act) s.finish_metric();
     }
-    process_var<V>(s, C, su, sgx, sgy, T, io);
+    process_var<V, BND>(s, C, su, sgx, sgy, T, io);
     __syncthreads();
-    run_vars<I + 1>(s, C, pf, su, sgx, sgy, T, io, t);
+    run_vars<I + 1, BND>(s, C, pf, su, sgx, sgy, T, io, t);
   }
 }
 
+template <bool BND>
 __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ blocks,
                                                  const DevTile* __restrict__ tiles, int tile_begin, int ntiles,
                                                  const IOPtrs* __restrict__ io, DevParams P) {
   __shared__ double su[NSU];
   __shared__ double sgx[NGX];
   __shared__ double sgy[NGY];
+  __shared__ double sacc[36 * NT];  // per-thread C2 (18), R, gt_rhs, At_rhs (6 each)
+  __shared__ IOPtrs s_io;           // array pointers of this launch
+  __shared__ AxisC s_ax[3];         // stencil constants of the current block
 
   const int t = threadIdx.x;
   Ctx C;
@@ -425,13 +456,21 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
     C.act = t < NPT && C.ti < T.nx && C.tj < T.ny && C.tk < T.nz;
     C.gi = T.i0 + C.ti; C.gj = T.j0 + C.tj; C.gk = T.k0 + C.tk;
     C.bflag = B.bflag;
-    C.ax = blocks[tl.block].ax;
+    if (t < 24) s_io.u.v[t] = io->u.v[t];
+    else if (t < 48) s_io.r.v[t - 24] = io->r.v[t - 24];
+    else if (t < 51) s_ax[t - 48] = B.ax[t - 48];
+    C.ax = s_ax;
 
     double pf[NPF];
     prefetch(pf, io->u.v[h_order[0]], T, t);
+    __syncthreads();
 
     // point coordinates, eta (rhs.cpp:214-223) and the Sommerfeld face test
-    dgr::Point s;
+    KPoint s;
+    s.C2.p = sacc + t;
+    s.R.p = sacc + 18 * NT + t;
+    s.gt_rhs.p = sacc + 24 * NT + t;
+    s.At_rhs.p = sacc + 30 * NT + t;
     C.o = T.off + (uint64_t)C.gi + (uint64_t)T.sz0 * (C.gj + (uint64_t)T.sz1 * C.gk);
     C.on_face = false;
     C.eta = P.eta_const;
@@ -448,22 +487,22 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
                   ((bf & 8u) && C.gj == T.sz1 - 4) || ((bf & 16u) && C.gk == 3) ||
                   ((bf & 32u) && C.gk == T.sz2 - 4);
       const uint64_t o = C.o;
-      s.alpha = io->u.v[dgr::V_ALPHA][o];
-      s.chi = io->u.v[dgr::V_CHI][o];
-      s.K = io->u.v[dgr::V_K][o];
+      s.alpha = s_io.u.v[dgr::V_ALPHA][o];
+      s.chi = s_io.u.v[dgr::V_CHI][o];
+      s.K = s_io.u.v[dgr::V_K][o];
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        s.beta[i] = io->u.v[dgr::V_BETA0 + i][o];
-        s.B[i] = io->u.v[dgr::V_B0 + i][o];
+        s.beta[i] = s_io.u.v[dgr::V_BETA0 + i][o];
+        s.B[i] = s_io.u.v[dgr::V_B0 + i][o];
       }
 #pragma unroll
       for (int p = 0; p < 6; ++p) {
-        s.gt[p] = io->u.v[dgr::V_SG0 + p][o];
-        s.At[p] = io->u.v[dgr::V_SA0 + p][o];
+        s.gt[p] = s_io.u.v[dgr::V_SG0 + p][o];
+        s.At[p] = s_io.u.v[dgr::V_SA0 + p][o];
       }
       s.begin();
     }
-    run_vars<0>(s, C, pf, su, sgx, sgy, T, io, t);
+    run_vars<0, BND>(s, C, pf, su, sgx, sgy, T, &s_io, t);
   }
 }
 
@@ -479,7 +518,9 @@ struct dgr_plan_s {
   uint64_t points = 0;
   DevBlock* d_blocks = nullptr;
   DevTile* d_tiles = nullptr;
-  std::vector<uint32_t> block_tile_begin;  // tiles of block b: [begin[b], begin[b+1])
+  uint32_t ntiles_interior = 0;           // tiles [0, n) belong to blocks with bflag 0
+  std::vector<uint32_t> block_tile_begin;  // first tile of block b
+  std::vector<uint32_t> block_tile_count;
   DevParams params{};
   int grid = 0;  // persistent grid size
   IOPtrs* d_io = nullptr;  // device copy of the 48 array pointers
@@ -501,6 +542,13 @@ int hip_fail(hipError_t e, const char* what) {
     hipError_t _e = (call);                             \
     if (_e != hipSuccess) return hip_fail(_e, #call);   \
   } while (0)
+
+void free_plan(dgr_plan_s* plan) {
+  (void)hipFree(plan->d_blocks);
+  (void)hipFree(plan->d_tiles);
+  (void)hipFree(plan->d_io);
+  delete plan;
+}
 
 // Stencil constants of one axis, with the operations of derivs.cpp
 // (idx = 1/dx; idx_by_2 = 0.5*idx; idx_by_12 = idx/12; idx_sqrd = 1/(dx*dx);
@@ -564,39 +612,46 @@ extern "C" int dgr_plan_create(dgr_plan_t* plan_out, const dgr_block* blocks, ui
   std::vector<DevBlock> hb(nblocks);
   std::vector<DevTile> ht;
   dgr_plan_s* plan = new dgr_plan_s();
-  plan->block_tile_begin.resize(nblocks + 1);
+  plan->block_tile_begin.resize(nblocks);
+  plan->block_tile_count.resize(nblocks);
   std::vector<std::pair<int, int>> sx, sy, sz;
-  for (uint32_t b = 0; b < nblocks; ++b) {
-    const dgr_block& bl = blocks[b];
-    for (int d = 0; d < 3; ++d) {
-      if (bl.sz[d] < 13 || bl.sz[d] > 65535) { delete plan; return DGR_E_INVALID; }
+  // interior blocks' tiles first, then boundary blocks' tiles: the two
+  // ranges run through the interior- and boundary-specialised kernels
+  for (int pass = 0; pass < 2; ++pass) {
+    for (uint32_t b = 0; b < nblocks; ++b) {
+      const dgr_block& bl = blocks[b];
+      if ((pass == 0) != (bl.bflag == 0)) continue;
+      for (int d = 0; d < 3; ++d) {
+        if (bl.sz[d] < 13 || bl.sz[d] > 65535) { delete plan; return DGR_E_INVALID; }
+      }
+      DevBlock& db = hb[b];
+      db.offset = bl.offset;
+      for (int d = 0; d < 3; ++d) {
+        db.sz[d] = bl.sz[d];
+        db.pmin[d] = bl.pmin[d];
+        db.h[d] = (bl.pmax[d] - bl.pmin[d]) / (bl.sz[d] - 1);  // rhs.cpp:148-150
+      }
+      for (int d = 0; d < 3; ++d) db.ax[d] = host_axis(db.h[d], (int)bl.sz[d], bl.bflag, d);
+      db.bflag = bl.bflag;
+      plan->block_tile_begin[b] = (uint32_t)ht.size();
+      split(bl.sz[0] - 6, sx);
+      split(bl.sz[1] - 6, sy);
+      split(bl.sz[2] - 6, sz);
+      for (auto& cz : sz)
+        for (auto& cy : sy)
+          for (auto& cx : sx) {
+            DevTile t;
+            t.block = b;
+            t.i0 = (uint16_t)cx.first; t.j0 = (uint16_t)cy.first; t.k0 = (uint16_t)cz.first;
+            t.nx = (uint8_t)cx.second; t.ny = (uint8_t)cy.second; t.nz = (uint8_t)cz.second;
+            t.pad = 0;
+            ht.push_back(t);
+          }
+      plan->block_tile_count[b] = (uint32_t)ht.size() - plan->block_tile_begin[b];
+      plan->points += (uint64_t)(bl.sz[0] - 6) * (bl.sz[1] - 6) * (bl.sz[2] - 6);
     }
-    DevBlock& db = hb[b];
-    db.offset = bl.offset;
-    for (int d = 0; d < 3; ++d) {
-      db.sz[d] = bl.sz[d];
-      db.pmin[d] = bl.pmin[d];
-      db.h[d] = (bl.pmax[d] - bl.pmin[d]) / (bl.sz[d] - 1);  // rhs.cpp:148-150
-    }
-    for (int d = 0; d < 3; ++d) db.ax[d] = host_axis(db.h[d], (int)bl.sz[d], bl.bflag, d);
-    db.bflag = bl.bflag;
-    plan->block_tile_begin[b] = (uint32_t)ht.size();
-    split(bl.sz[0] - 6, sx);
-    split(bl.sz[1] - 6, sy);
-    split(bl.sz[2] - 6, sz);
-    for (auto& cz : sz)
-      for (auto& cy : sy)
-        for (auto& cx : sx) {
-          DevTile t;
-          t.block = b;
-          t.i0 = (uint16_t)cx.first; t.j0 = (uint16_t)cy.first; t.k0 = (uint16_t)cz.first;
-          t.nx = (uint8_t)cx.second; t.ny = (uint8_t)cy.second; t.nz = (uint8_t)cz.second;
-          t.pad = 0;
-          ht.push_back(t);
-        }
-    plan->points += (uint64_t)(bl.sz[0] - 6) * (bl.sz[1] - 6) * (bl.sz[2] - 6);
+    if (pass == 0) plan->ntiles_interior = (uint32_t)ht.size();
   }
-  plan->block_tile_begin[nblocks] = (uint32_t)ht.size();
   plan->nblocks = nblocks;
   plan->ntiles = (uint32_t)ht.size();
   for (int i = 0; i < 4; ++i) plan->params.lambda[i] = p->lambda[i];
@@ -611,22 +666,18 @@ extern "C" int dgr_plan_create(dgr_plan_t* plan_out, const dgr_block* blocks, ui
   int ncu = 0;
   e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, plan->device);
   if (e != hipSuccess) { delete plan; return hip_fail(e, "hipDeviceGetAttribute"); }
-  plan->grid = std::max(1, std::min<int>((int)plan->ntiles, ncu * 2));
+  // the kernel holds ~140 KB of LDS, so one workgroup per CU is resident
+  plan->grid = ncu;
   if (hipMalloc(&plan->d_blocks, sizeof(DevBlock) * nblocks) != hipSuccess ||
       hipMalloc(&plan->d_tiles, sizeof(DevTile) * ht.size()) != hipSuccess ||
       hipMalloc(&plan->d_io, sizeof(IOPtrs)) != hipSuccess) {
-    hipFree(plan->d_blocks);
-    hipFree(plan->d_tiles);
-    delete plan;
+    free_plan(plan);
     return DGR_E_NOMEM;
   }
   e = hipMemcpy(plan->d_blocks, hb.data(), sizeof(DevBlock) * nblocks, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(plan->d_tiles, ht.data(), sizeof(DevTile) * ht.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    hipFree(plan->d_blocks);
-    hipFree(plan->d_tiles);
-    hipFree(plan->d_io);
-    delete plan;
+    free_plan(plan);
     return hip_fail(e, "hipMemcpy(plan)");
   }
   *plan_out = plan;
@@ -635,10 +686,7 @@ extern "C" int dgr_plan_create(dgr_plan_t* plan_out, const dgr_block* blocks, ui
 
 extern "C" int dgr_plan_destroy(dgr_plan_t plan) {
   if (!plan) return DGR_E_INVALID;
-  hipFree(plan->d_blocks);
-  hipFree(plan->d_tiles);
-  hipFree(plan->d_io);
-  delete plan;
+  free_plan(plan);
   return DGR_OK;
 }
 
@@ -670,8 +718,14 @@ extern "C" int dgr_bssn_rhs(dgr_plan_t plan, double* const* rhs, const double* c
   hipStream_t st = (hipStream_t)stream;
   int rc = upload_ptrs(plan, rhs, uin, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_bssn_rhs, dim3(plan->grid), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, 0,
-                     (int)plan->ntiles, plan->d_io, plan->params);
+  const int ni = (int)plan->ntiles_interior;
+  const int nbd = (int)plan->ntiles - ni;
+  if (ni > 0)
+    hipLaunchKernelGGL(k_bssn_rhs<false>, dim3(std::min(plan->grid, ni)), dim3(NT), 0, st, plan->d_blocks,
+                       plan->d_tiles, 0, ni, plan->d_io, plan->params);
+  if (nbd > 0)
+    hipLaunchKernelGGL(k_bssn_rhs<true>, dim3(std::min(plan->grid, nbd)), dim3(NT), 0, st, plan->d_blocks,
+                       plan->d_tiles, ni, nbd, plan->d_io, plan->params);
   DGR_HIP(hipGetLastError());
   return DGR_OK;
 }
@@ -684,9 +738,13 @@ extern "C" int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const
   if (rc) return rc;
   for (uint32_t b = 0; b < plan->nblocks; ++b) {
     const int tb = (int)plan->block_tile_begin[b];
-    const int nt = (int)(plan->block_tile_begin[b + 1] - plan->block_tile_begin[b]);
-    hipLaunchKernelGGL(k_bssn_rhs, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, tb, nt, plan->d_io,
-                       plan->params);
+    const int nt = (int)plan->block_tile_count[b];
+    if (tb < (int)plan->ntiles_interior)
+      hipLaunchKernelGGL(k_bssn_rhs<false>, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, tb, nt,
+                         plan->d_io, plan->params);
+    else
+      hipLaunchKernelGGL(k_bssn_rhs<true>, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, tb, nt,
+                         plan->d_io, plan->params);
   }
   DGR_HIP(hipGetLastError());
   return DGR_OK;

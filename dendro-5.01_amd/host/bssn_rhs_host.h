@@ -1,0 +1,57 @@
+/*
+ * Host-side mirror of the reference's BSSN RHS interface, on top of the
+ * C ABI in include/dgr_bssn.h.
+ *
+ *   bssnRHS(rhs, u, blkList, numBlocks)      BSSN_GR/include/rhs.h:45
+ *   bssnrhs(rhs, u, offset, ptmin, ptmax, sz, bflag)   BSSN_GR/include/rhs.h:47-50
+ *
+ * Same names, argument meaning and data layout: 24 unzipped arrays in VAR
+ * order, blocks at `offset`, only interior points of the RHS written.  The
+ * arrays may be host or device memory: device arrays run in place; host
+ * arrays are staged through device buffers (u and rhs copied in, rhs copied
+ * back, the PCIe round trip the reference GPU path makes on every call,
+ * BSSN_GR/cuda_gr/src/rhs_cuda.cu:164,187).
+ *
+ * Parameters come from dgr::bssn_params(), whose field names and defaults
+ * are those of the reference's bssn:: globals (BSSN_GR/src/parameters.cpp:
+ * 87-97).  The reference functions return void and have no error path;
+ * here a failure throws std::runtime_error with the library's message.
+ */
+#ifndef DGR_BSSN_RHS_HOST_H
+#define DGR_BSSN_RHS_HOST_H
+
+namespace dgr {
+
+struct BSSNGlobals {
+  unsigned int BSSN_LAMBDA[4] = {1, 1, 1, 1};
+  double BSSN_LAMBDA_F[2] = {1.0, 0.0};
+  double ETA_CONST = 2.0;
+  double ETA_R0 = 50.0;
+  double ETA_DAMPING_EXP = 1.0;
+  double KO_DISS_SIGMA = 0.01;
+  double CHI_FLOOR = 0.1;
+};
+
+// Process-wide parameter set used by bssnRHS / bssnrhs.
+BSSNGlobals& bssn_params();
+
+// What bssnRHS reads from an ot::Block (BSSN_GR/src/rhs.cpp:45-62): the
+// offset, the allocation sizes, the boundary flag and the padded extent
+// (corner -/+ 3 dx).
+struct BlockDesc {
+  unsigned int offset;
+  unsigned int sz[3];
+  unsigned int bflag;
+  double ptmin[3];
+  double ptmax[3];
+};
+
+}  // namespace dgr
+
+void bssnRHS(double** uzipVarsRHS, const double** uZipVars, const dgr::BlockDesc* blkList,
+             unsigned int numBlocks);
+
+void bssnrhs(double** uzipVarsRHS, const double** uZipVars, const unsigned int& offset, const double* ptmin,
+             const double* ptmax, const unsigned int* sz, const unsigned int& bflag);
+
+#endif

@@ -1,0 +1,119 @@
+// GPU test driver for the host-side bssnRHS / bssnrhs mirror, in the manner
+// of the reference's rhsTestBed (BSSN_GR/src/rhsTestBed.cpp:181-295):
+// fill padded blocks with smooth data, evaluate the RHS with host arrays,
+// with device arrays and block by block, and report the per-variable
+// relative L-infinity difference against the CPU oracle.
+// Exit status 0 when every difference is within 1e-11.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../dendro-5.01_amd/host/bssn_rhs_host.h"
+#include "../../oracle/bssn_oracle.h"
+
+static double field(int v, double x, double y, double z) {
+  const double flat = (v == 0 || v == 1 || v == 12 || v == 15 || v == 17) ? 1.0 : 0.0;
+  const double cx = 0.05 * std::sin(1.7 * v), cy = 0.04 * std::cos(2.3 * v), cz = 0.03 * std::sin(0.7 * v + 1);
+  const double r2 = (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz);
+  return flat + 2e-3 * std::exp(-r2 / 0.02) * (1.0 + 0.1 * v);
+}
+
+int main() {
+  const int nb = 3;  // 27 blocks of 13^3, outer faces flagged
+  const unsigned sz = 13;
+  const unsigned n = sz * sz * sz;
+  const double dx = 1.0 / 48;
+  std::vector<dgr::BlockDesc> blocks;
+  for (int bz = 0; bz < nb; ++bz)
+    for (int by = 0; by < nb; ++by)
+      for (int bx = 0; bx < nb; ++bx) {
+        dgr::BlockDesc b;
+        b.offset = (unsigned)blocks.size() * n;
+        b.sz[0] = b.sz[1] = b.sz[2] = sz;
+        const int c[3] = {bx, by, bz};
+        b.bflag = 0;
+        for (int d = 0; d < 3; ++d) {
+          const double corner = -0.5 * nb * 6 * dx + c[d] * 6 * dx;
+          b.ptmin[d] = corner - 3 * dx;
+          b.ptmax[d] = corner + 9 * dx;
+          if (c[d] == 0) b.bflag |= 1u << (2 * d);
+          if (c[d] == nb - 1) b.bflag |= 1u << (2 * d + 1);
+        }
+        blocks.push_back(b);
+      }
+  const size_t dof = blocks.size() * n;
+  std::vector<std::vector<double>> u(24, std::vector<double>(dof)), r_host(24, std::vector<double>(dof, -1.0)),
+      r_dev(24, std::vector<double>(dof, -1.0)), r_blk(24, std::vector<double>(dof, -1.0)),
+      r_orc(24, std::vector<double>(dof, -1.0));
+  for (const auto& b : blocks)
+    for (unsigned k = 0; k < sz; ++k)
+      for (unsigned j = 0; j < sz; ++j)
+        for (unsigned i = 0; i < sz; ++i) {
+          const double h = (b.ptmax[0] - b.ptmin[0]) / (sz - 1);
+          const double x = b.ptmin[0] + i * h, y = b.ptmin[1] + j * h, z = b.ptmin[2] + k * h;
+          for (int v = 0; v < 24; ++v) u[v][b.offset + i + sz * (j + sz * k)] = field(v, x, y, z);
+        }
+  dgr::bssn_params().KO_DISS_SIGMA = 0.4;
+  dgr::bssn_params().ETA_R0 = 30.0;
+
+  const double* up[24];
+  double* rp[24];
+  // 1) host arrays
+  for (int v = 0; v < 24; ++v) { up[v] = u[v].data(); rp[v] = r_host[v].data(); }
+  bssnRHS(rp, up, blocks.data(), (unsigned)blocks.size());
+  // 2) per block, host arrays (bssnrhs)
+  for (int v = 0; v < 24; ++v) rp[v] = r_blk[v].data();
+  for (const auto& b : blocks) bssnrhs(rp, up, b.offset, b.ptmin, b.ptmax, b.sz, b.bflag);
+  // 3) device arrays
+  double* du = nullptr;
+  double* dr = nullptr;
+  if (hipMalloc(&du, sizeof(double) * dof * 24) != hipSuccess || hipMalloc(&dr, sizeof(double) * dof * 24) != hipSuccess)
+    return 2;
+  const double* dup[24];
+  double* drp[24];
+  for (int v = 0; v < 24; ++v) {
+    dup[v] = du + v * dof;
+    drp[v] = dr + v * dof;
+    if (hipMemcpy((void*)dup[v], u[v].data(), sizeof(double) * dof, hipMemcpyHostToDevice) != hipSuccess) return 2;
+    if (hipMemcpy(drp[v], r_dev[v].data(), sizeof(double) * dof, hipMemcpyHostToDevice) != hipSuccess) return 2;
+  }
+  bssnRHS(drp, dup, blocks.data(), (unsigned)blocks.size());
+  for (int v = 0; v < 24; ++v)
+    if (hipMemcpy(r_dev[v].data(), drp[v], sizeof(double) * dof, hipMemcpyDeviceToHost) != hipSuccess) return 2;
+  // oracle
+  orc_params op;
+  for (int i = 0; i < 4; ++i) op.lambda[i] = 1.0;
+  op.lambda_f[0] = 1.0; op.lambda_f[1] = 0.0;
+  op.eta_const = 2.0; op.eta_r0 = 30.0; op.eta_damping_exp = 1.0; op.ko_sigma = 0.4;
+  for (int v = 0; v < 24; ++v) rp[v] = r_orc[v].data();
+  for (const auto& b : blocks) orc_bssnrhs(rp, up, b.offset, b.ptmin, b.ptmax, b.sz, b.bflag, &op);
+
+  int fail = 0;
+  const char* names[3] = {"host arrays", "per-block bssnrhs", "device arrays"};
+  std::vector<std::vector<double>>* res[3] = {&r_host, &r_blk, &r_dev};
+  for (int c = 0; c < 3; ++c) {
+    double worst = 0.0;
+    for (int v = 0; v < 24; ++v) {
+      double dmax = 0.0, smax = 0.0;
+      for (size_t p = 0; p < dof; ++p) {
+        const double o = r_orc[v][p];
+        const double g = (*res[c])[v][p];
+        if (o == -1.0) {  // padding must be untouched
+          if (g != -1.0) { printf("%s: padding written (var %d)\n", names[c], v); fail = 1; break; }
+          continue;
+        }
+        dmax = std::fmax(dmax, std::fabs(g - o));
+        smax = std::fmax(smax, std::fabs(o));
+      }
+      worst = std::fmax(worst, smax > 0 ? dmax / smax : dmax);
+    }
+    printf("%s: max rel err per variable %.3e\n", names[c], worst);
+    if (!(worst <= 1e-11)) fail = 1;
+  }
+  (void)hipFree(du);
+  (void)hipFree(dr);
+  return fail;
+}
