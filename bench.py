@@ -112,8 +112,10 @@ def main():
 
     ms, wall = timed(False, a.steps, a.warmup)
     print("bench: batched %.4f ms/step" % ms, file=sys.stderr, flush=True)
-    ms_pb, _ = timed(True, a.per_block_steps, 1)
-    print("bench: per-block %.4f ms/step" % ms_pb, file=sys.stderr, flush=True)
+    ms_pb = None
+    if a.per_block_steps > 0:
+        ms_pb, _ = timed(True, a.per_block_steps, 1)
+        print("bench: per-block %.4f ms/step" % ms_pb, file=sys.stderr, flush=True)
 
     value = pts / (ms * 1e-3)
     nbytes = algorithmic_bytes(blocks)
@@ -141,8 +143,8 @@ def main():
         "fp64": {"reference_flop_per_point": REF_FLOP_PER_POINT,
                  "achieved_tflops": REF_FLOP_PER_POINT * value / 1e12,
                  "frac_of_peak": REF_FLOP_PER_POINT * value / 1e12 / FP64_PEAK_TFLOPS},
-        "per_block": {"ms_per_step": ms_pb, "points_per_s": pts / (ms_pb * 1e-3),
-                      "batched_speedup": ms_pb / ms},
+        "per_block": None if ms_pb is None else {"ms_per_step": ms_pb, "points_per_s": pts / (ms_pb * 1e-3),
+                                                 "batched_speedup": ms_pb / ms},
         "wall_s_timed_region": wall,
     }
     if not a.no_cpu:

@@ -89,10 +89,12 @@ struct PointT {
   typename S::template A<6> R;
   // right-hand-side accumulators
   typename S::template A<6> gt_rhs, At_rhs;
-  double Gt_rhs[3], Bacc[3];
-  double chi_rhs, K_rhs;
-  double divb;
-  double wb[3];
+  // small accumulators: Gt_rhs^i, the -lambda3 beta.adv(Gt^i) part of
+  // B_rhs^i, w_j = d_j d_k beta^k, div beta, chi_rhs, K_rhs
+  enum { A_GT = 0, A_B = 3, A_WB = 6, A_DIVB = 9, A_CHI = 10, A_K = 11 };
+  typename S::template A<12> acc;
+
+  DGR_HD double Gt_out(int k) const { return acc[A_GT + k]; }
 
   DGR_HD void begin() {
     const double g00 = gt[0], g01 = gt[1], g02 = gt[2], g11 = gt[3], g12 = gt[4], g22 = gt[5];
@@ -116,10 +118,10 @@ struct PointT {
       AUU[p] = P[i][0] * igt[sp(0, j)] + P[i][1] * igt[sp(1, j)] + P[i][2] * igt[sp(2, j)];
     }
     for (int p = 0; p < 6; ++p) { R[p] = 0.0; gt_rhs[p] = 0.0; At_rhs[p] = 0.0; }
-    for (int i = 0; i < 3; ++i) { Gt_rhs[i] = 0.0; wb[i] = 0.0; Bacc[i] = 0.0; }
-    divb = 0.0;
-    K_rhs = 0.0;
-    chi_rhs = 0.0;
+    for (int i = 0; i < 3; ++i) { acc[A_GT + i] = 0.0; acc[A_WB + i] = 0.0; acc[A_B + i] = 0.0; }
+    acc[A_DIVB] = 0.0;
+    acc[A_K] = 0.0;
+    acc[A_CHI] = 0.0;
   }
 
   // -1/2 g^{lm} d_l d_m gt_ij (first Rt term of dendro.py compute_ricci)
@@ -173,7 +175,7 @@ struct PointT {
     for (int i = 0; i < 3; ++i) {
       const double s = C2[i * 6 + 0] * AUU[0] + C2[i * 6 + 3] * AUU[3] + C2[i * 6 + 5] * AUU[5] +
                        2.0 * (C2[i * 6 + 1] * AUU[1] + C2[i * 6 + 2] * AUU[2] + C2[i * 6 + 4] * AUU[4]);
-      Gt_rhs[i] += 2.0 * alpha * s;
+      acc[A_GT + i] += 2.0 * alpha * s;
     }
   }
 
@@ -190,7 +192,7 @@ struct PointT {
       const double c = C2[0 * 6 + p] * d1[0] + C2[1 * 6 + p] * d1[1] + C2[2 * 6 + p] * d1[2];
       R[p] += h * (d2[p] - c) - 0.25 * ichi * ichi * d1[i] * d1[j] + h * gt[p] * s;
     }
-    chi_rhs = adv;
+    acc[A_CHI] = adv;
   }
 
   // D_i D_j alpha with the full-metric Christoffels: its trace-free part
@@ -211,14 +213,14 @@ struct PointT {
       lap += spw(p) * igt[p] * DD[p];
     }
     for (int p = 0; p < 6; ++p) At_rhs[p] -= chi * (DD[p] - gt[p] * lap * (1.0 / 3.0));
-    K_rhs -= chi * lap;
+    acc[A_K] -= chi * lap;
     for (int i = 0; i < 3; ++i) {
       double s = 0.0, t = 0.0;
       for (int j = 0; j < 3; ++j) {
         s += AUU[sp(i, j)] * d1[j];
         t += AUU[sp(i, j)] * dchi[j];
       }
-      Gt_rhs[i] -= 2.0 * s + alpha * 3.0 * ichi * t;
+      acc[A_GT + i] -= 2.0 * s + alpha * 3.0 * ichi * t;
     }
     return P.lambda[0] * adv - 2.0 * alpha * K;
   }
@@ -230,8 +232,8 @@ struct PointT {
       const int i = spi(p), j = spj(p);
       R[p] += 0.5 * (gt[sp(k, i)] * d1[j] + gt[sp(k, j)] * d1[i]);
     }
-    Gt_rhs[k] += adv;
-    Bacc[k] = -P.lambda[3] * adv;
+    acc[A_GT + k] += adv;
+    acc[A_B + k] = -P.lambda[3] * adv;
     if (k == 2) {
       double tr = 0.0;
       for (int p = 0; p < 6; ++p) tr += spw(p) * igt[p] * R[p];
@@ -243,23 +245,24 @@ struct PointT {
   // *chi_out hold gt_rhs and chi_rhs.
   DGR_HD double add_beta(const Params& P, int k, const double d1[3], const double d2[6], double adv,
                          double* gt_out, double* chi_out) {
-    divb += d1[k];
+    acc[A_DIVB] += d1[k];
     for (int p = 0; p < 6; ++p) {
       const int i = spi(p), j = spj(p);
       gt_rhs[p] += gt[sp(i, k)] * d1[j] + gt[sp(k, j)] * d1[i];
       At_rhs[p] += At[sp(i, k)] * d1[j] + At[sp(k, j)] * d1[i];
     }
-    Gt_rhs[k] += igt[0] * d2[0] + igt[3] * d2[3] + igt[5] * d2[5] +
+    acc[A_GT + k] += igt[0] * d2[0] + igt[3] * d2[3] + igt[5] * d2[5] +
                  2.0 * (igt[1] * d2[1] + igt[2] * d2[2] + igt[4] * d2[4]);
-    for (int j = 0; j < 3; ++j) wb[j] += d2[sp(j, k)];
-    Gt_rhs[k] -= CalGt[0] * d1[0] + CalGt[1] * d1[1] + CalGt[2] * d1[2];
+    for (int j = 0; j < 3; ++j) acc[A_WB + j] += d2[sp(j, k)];
+    acc[A_GT + k] -= CalGt[0] * d1[0] + CalGt[1] * d1[1] + CalGt[2] * d1[2];
     if (k == 2) {
       const double w = -2.0 / 3.0;
-      for (int p = 0; p < 6; ++p) gt_out[p] = gt_rhs[p] + w * gt[p] * divb - 2.0 * alpha * At[p];
-      *chi_out = chi_rhs + w * chi * divb + (2.0 / 3.0) * (chi * alpha * K);
+      for (int p = 0; p < 6; ++p) gt_out[p] = gt_rhs[p] + w * gt[p] * acc[A_DIVB] - 2.0 * alpha * At[p];
+      *chi_out = acc[A_CHI] + w * chi * acc[A_DIVB] + (2.0 / 3.0) * (chi * alpha * K);
       for (int i = 0; i < 3; ++i)
-        Gt_rhs[i] += (2.0 / 3.0) * CalGt[i] * divb +
-                     (1.0 / 3.0) * (igt[sp(i, 0)] * wb[0] + igt[sp(i, 1)] * wb[1] + igt[sp(i, 2)] * wb[2]);
+        acc[A_GT + i] += (2.0 / 3.0) * CalGt[i] * acc[A_DIVB] +
+                         (1.0 / 3.0) * (igt[sp(i, 0)] * acc[A_WB + 0] + igt[sp(i, 1)] * acc[A_WB + 1] +
+                                        igt[sp(i, 2)] * acc[A_WB + 2]);
     }
     return 0.75 * (P.lambda_f[0] + P.lambda_f[1] * alpha) * B[k] + P.lambda[1] * adv;
   }
@@ -267,14 +270,14 @@ struct PointT {
   // Returns K_rhs; Gt_rhs is final afterwards.
   DGR_HD double add_K(const double d1[3], double adv) {
     for (int i = 0; i < 3; ++i)
-      Gt_rhs[i] -= alpha * (4.0 / 3.0) * (igt[sp(i, 0)] * d1[0] + igt[sp(i, 1)] * d1[1] + igt[sp(i, 2)] * d1[2]);
+      acc[A_GT + i] -= alpha * (4.0 / 3.0) * (igt[sp(i, 0)] * d1[0] + igt[sp(i, 1)] * d1[1] + igt[sp(i, 2)] * d1[2]);
     double sqrA = 0.0;
     for (int p = 0; p < 6; ++p) sqrA += spw(p) * At[p] * AUU[p];
-    return adv + K_rhs + alpha * (K * K * (1.0 / 3.0) + sqrA);
+    return adv + acc[A_K] + alpha * (K * K * (1.0 / 3.0) + sqrA);
   }
 
   DGR_HD double add_B(const Params& P, int k, double adv, double eta) const {
-    return Gt_rhs[k] - eta * B[k] + P.lambda[2] * adv + Bacc[k];
+    return acc[A_GT + k] - eta * B[k] + P.lambda[2] * adv + acc[A_B + k];
   }
 
   DGR_HD double add_At(int p, double adv) const {
@@ -285,7 +288,7 @@ struct PointT {
       const double Pkj = igt[sp(k, 0)] * At[sp(0, j)] + igt[sp(k, 1)] * At[sp(1, j)] + igt[sp(k, 2)] * At[sp(2, j)];
       AA += At[sp(i, k)] * Pkj;
     }
-    return adv + At_rhs[p] + (-2.0 / 3.0) * At[p] * divb + alpha * (K * At[p] - 2.0 * AA);
+    return adv + At_rhs[p] + (-2.0 / 3.0) * At[p] * acc[A_DIVB] + alpha * (K * At[p] - 2.0 * AA);
   }
 };
 

@@ -37,6 +37,7 @@
 
 #include "../../include/dgr_bssn.h"
 #include "bssn_staged.h"
+#include "tile_map.h"
 
 namespace {
 
@@ -209,19 +210,20 @@ struct TileGeom {
 };
 
 // Issue the global loads of one variable's staged region into registers.
+// The region is walked in the fixed SX x SY x SZ layout of the LDS buffer
+// (compile-time divisors); elements outside a partial tile's region are
+// skipped and their LDS slots are never read.
 __device__ __forceinline__ void prefetch(double (&r)[NPF], const double* __restrict__ g, const TileGeom& T,
                                          int t) {
+  const double* base = g + T.off + (T.i0 - 3) + (long)T.sz0 * ((T.j0 - 3) + (long)T.sz1 * (T.k0 - 3));
 #pragma unroll
   for (int m = 0; m < NPF; ++m) {
     const int e = t + m * NT;
-    if (e < T.nreg) {
-      const int a = e % T.rx;
-      const int bc = e / T.rx;
-      const int b = bc % T.ry;
-      const int c = bc / T.ry;
-      const long gi = (long)(T.i0 - 3 + a) + (long)T.sz0 * ((T.j0 - 3 + b) + (long)T.sz1 * (T.k0 - 3 + c));
-      r[m] = __builtin_nontemporal_load(g + T.off + gi);
-    }
+    const int a = e % SX;
+    const int b = (e / SX) % SY;
+    const int c = e / (SX * SY);
+    if (e < NSU && a < T.rx && b < T.ry && c < T.rz)
+      r[m] = __builtin_nontemporal_load(base + a + (long)T.sz0 * (b + (long)T.sz1 * c));
   }
 }
 
@@ -229,13 +231,7 @@ __device__ __forceinline__ void stage(const double (&r)[NPF], double* su, const 
 #pragma unroll
   for (int m = 0; m < NPF; ++m) {
     const int e = t + m * NT;
-    if (e < T.nreg) {
-      const int a = e % T.rx;
-      const int bc = e / T.rx;
-      const int b = bc % T.ry;
-      const int c = bc / T.ry;
-      su[a + SX * (b + SY * c)] = r[m];
-    }
+    if (e < NSU) su[e] = r[m];
   }
 }
 
@@ -265,17 +261,49 @@ __device__ __forceinline__ void emit_parked(const IOPtrs* __restrict__ io, const
   emit(io, C, v, eq, io->r.v[v][C.o]);
 }
 
+// Per-thread LDS slots: element i of a thread's array sits at p[i * NPT]
+// (one slot column per tile point).
+constexpr int NSLOT = 48;  // C2 18, R 6, gt_rhs 6, At_rhs 6, small accumulators 12
 template <int N>
 struct LdsArr {
   double* p;
-  __device__ __forceinline__ double& operator[](int i) { return p[i * NT]; }
-  __device__ __forceinline__ const double& operator[](int i) const { return p[i * NT]; }
+  __device__ __forceinline__ double& operator[](int i) { return p[i * NPT]; }
+  __device__ __forceinline__ const double& operator[](int i) const { return p[i * NPT]; }
 };
 struct LdsStore {
   template <int N>
   using A = LdsArr<N>;
 };
 using KPoint = dgr::PointT<LdsStore>;
+
+// All operators along one axis through the point: first derivative (if
+// D1), second derivative (if D2), upwinded derivative and Kreiss-Oliger
+// term.  In interior blocks the 7-point line is read from LDS once into
+// registers and every operator is evaluated from it with the reference's
+// operation order (derivs.cpp); in boundary blocks the closure-aware forms
+// above are used.
+template <bool BND, bool D1, bool D2>
+__device__ __forceinline__ void axis_ops(const double* A, int o, int s, int q, const AxisC& x, double beta,
+                                         double& d1, double& d2, double& ad, double& ko) {
+#pragma clang fp contract(off)
+  if constexpr (BND) {
+    if constexpr (D1) d1 = d1_42<true>(A, o, s, q, x.e, x.lf, x.hf, x.ih2, x.ih12);
+    if constexpr (D2) d2 = d2_42<true>(A, o, s, q, x.e, x.lf, x.hf, x.isq, x.isq12);
+    ad = adv_42<true>(A, o, s, q, x.e, x.lf, x.hf, beta, x.ih2, x.ih12);
+    ko = ko_42<true>(A, o, s, q, x.e, x.lf, x.hf, x.pre, x.smr1, x.smr2, x.smr3);
+  } else {
+    const int c = o + q * s;
+    const double m3 = A[c - 3 * s], m2 = A[c - 2 * s], m1 = A[c - s], z0 = A[c], p1 = A[c + s], p2 = A[c + 2 * s],
+                 p3 = A[c + 3 * s];
+    if constexpr (D1) d1 = (m2 - 8.0 * m1 + 8.0 * p1 - p2) * x.ih12;
+    if constexpr (D2) d2 = (-m2 + 16.0 * m1 - 30.0 * z0 + 16.0 * p1 - p2) * x.isq12;
+    if (beta > 0.0)
+      ad = (-3.0 * m1 - 10.0 * z0 + 18.0 * p1 - 6.0 * p2 + p3) * x.ih12;
+    else
+      ad = (-m3 + 6.0 * m2 - 18.0 * m1 + 10.0 * z0 + 3.0 * p1) * x.ih12;
+    ko = x.pre * (-m3 + 6.0 * m2 - 15.0 * m1 + 20.0 * z0 - 15.0 * p1 + 6.0 * p2 - p3);
+  }
+}
 
 template <int V, bool BND>
 __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su, const double* sgx,
@@ -288,16 +316,23 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
   // as su[o + q*stride] with q the block coordinate along the axis
   const int c = (ti + 3) + SX * ((tj + 3) + SY * (tk + 3));
   const int ox = c - gi, oy = c - gj * SX, oz = c - gk * (SX * SY);
+  constexpr bool D2 = has_d2(V);
+  // first derivatives along x and y come from the LDS planes for D2 vars
+  constexpr bool D1xy = !D2 && (has_d1(V) || BND);
+  constexpr bool D1z = has_d1(V) || BND;
   double d1[3] = {0.0, 0.0, 0.0};
   double d2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if constexpr (has_d2(V)) {
+  double av[3], kv[3];
+  axis_ops<BND, D1xy, D2>(su, ox, 1, gi, ax[0], s.beta[0], d1[0], d2[0], av[0], kv[0]);
+  __builtin_amdgcn_sched_barrier(0);
+  axis_ops<BND, D1xy, D2>(su, oy, SX, gj, ax[1], s.beta[1], d1[1], d2[3], av[1], kv[1]);
+  __builtin_amdgcn_sched_barrier(0);
+  axis_ops<BND, D1z, D2>(su, oz, SX * SY, gk, ax[2], s.beta[2], d1[2], d2[5], av[2], kv[2]);
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (D2) {
     // x and y derivatives at the point are read back from the LDS planes
     d1[0] = sgx[ti + TX * ((tj + 2) + GXY * (tk + 2))];
     d1[1] = sgy[ti + TX * (tj + TY * (tk + 2))];
-    d1[2] = d1_42<BND>(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
-    d2[0] = d2_42<BND>(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, ax[0].isq, ax[0].isq12);
-    d2[3] = d2_42<BND>(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].isq, ax[1].isq12);
-    d2[5] = d2_42<BND>(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].isq, ax[2].isq12);
     // mixed derivatives by composition: d_y(d_x u), d_z(d_x u), d_z(d_y u).
     // sgx[a + TX*(b + GXY*cc)] holds d_x u at block coords (i0+a, j0-2+b, k0-2+cc);
     // sgy[a + TX*(b + TY*cc)] holds d_y u at (i0+a, j0+b, k0-2+cc).
@@ -307,32 +342,16 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
     d2[2] = d1_42<BND>(sgx, oxz, TX * GXY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
     const int oyz = ti + TX * (tj + TY * (2 - T.k0));
     d2[4] = d1_42<BND>(sgy, oyz, TX * TY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
-  } else {
-    if (has_d1(V) || BND) {
-      d1[0] = d1_42<BND>(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, ax[0].ih2, ax[0].ih12);
-      d1[1] = d1_42<BND>(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
-      d1[2] = d1_42<BND>(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
-    }
+    __builtin_amdgcn_sched_barrier(0);
   }
-  __builtin_amdgcn_sched_barrier(0);
   // upwinded derivatives contracted with beta, summed in the order of the
   // reference's printed expression: beta0*a0 + beta1*a1 + beta2*a2
-  double adv;
-  {
-    const double a0 = adv_42<BND>(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, s.beta[0], ax[0].ih2, ax[0].ih12);
-    const double a1 = adv_42<BND>(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, s.beta[1], ax[1].ih2, ax[1].ih12);
-    const double a2 = adv_42<BND>(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, s.beta[2], ax[2].ih2, ax[2].ih12);
-    adv = s.beta[0] * a0 + s.beta[1] * a1 + s.beta[2] * a2;
-  }
-  __builtin_amdgcn_sched_barrier(0);
+  const double adv = s.beta[0] * av[0] + s.beta[1] * av[1] + s.beta[2] * av[2];
   // Kreiss-Oliger sum, preceded on flagged faces by the Sommerfeld value
   double ex;
   {
 #pragma clang fp contract(off)
-    const double k0 = ko_42<BND>(su, ox, 1, gi, ax[0].e, ax[0].lf, ax[0].hf, ax[0].pre, ax[0].smr1, ax[0].smr2, ax[0].smr3);
-    const double k1 = ko_42<BND>(su, oy, SX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].pre, ax[1].smr1, ax[1].smr2, ax[1].smr3);
-    const double k2 = ko_42<BND>(su, oz, SX * SY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].pre, ax[2].smr1, ax[2].smr2,
-                            ax[2].smr3);
+    const double k0 = kv[0], k1 = kv[1], k2 = kv[2];
     const double ko = C.sigma * (k0 + k1 + k2);
     if (BND && C.on_face) {
       const double f = su[c];
@@ -358,6 +377,7 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
     park(io, C, V, ex);
   }
   if constexpr (V >= V_BETA0 && V <= V_BETA2) {
+    s.B[V - V_BETA0] = io->u.v[V_B0 + (V - V_BETA0)][C.o];  // B^k only enters b_rhs^k and B_rhs^k
     double gto[6], chio = 0.0;
     emit(io, C, V, s.add_beta(C.P, V - V_BETA0, d1, d2, adv, gto, &chio), ex);
     if constexpr (V == V_BETA2) {
@@ -369,35 +389,39 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
   if constexpr (V == V_K) {
     emit(io, C, V_K, s.add_K(d1, adv), ex);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) emit_parked(io, C, V_GT0 + k, s.Gt_rhs[k]);
+    for (int k = 0; k < 3; ++k) emit_parked(io, C, V_GT0 + k, s.Gt_out(k));
   }
-  if constexpr (V >= V_B0 && V <= V_B2) emit(io, C, V, s.add_B(C.P, V - V_B0, adv, C.eta), ex);
+  if constexpr (V >= V_B0 && V <= V_B2) {
+    s.B[V - V_B0] = su[c];
+    emit(io, C, V, s.add_B(C.P, V - V_B0, adv, C.eta), ex);
+  }
   if constexpr (V >= V_SA0 && V <= V_SA5) emit(io, C, V, s.add_At(V - V_SA0, adv), ex);
 }
 
-// Widened first-derivative planes for the mixed derivatives.
+// Widened first-derivative planes for the mixed derivatives, walked in
+// their fixed LDS shapes (compile-time divisors) and masked to the tile.
 template <bool BND>
 __device__ __forceinline__ void deriv_planes(const double* su, double* sgx, double* sgy, const TileGeom& T,
                                              const AxisC* ax, int t) {
   // sgx: d_x u at block coords (i0+a, j0-2+b, k0-2+cc), su coords (a+3, b+1, cc+1)
-  const int nxg = T.nx * (T.ny + 4) * (T.nz + 4);
-  for (int e = t; e < nxg; e += NT) {
-    const int a = e % T.nx;
-    const int bc = e / T.nx;
-    const int b = bc % (T.ny + 4);
-    const int cc = bc / (T.ny + 4);
-    const int o = (a + 3) + SX * ((b + 1) + SY * (cc + 1)) - (T.i0 + a);
-    sgx[a + TX * (b + GXY * cc)] = d1_42<BND>(su, o, 1, T.i0 + a, ax[0].e, ax[0].lf, ax[0].hf, ax[0].ih2, ax[0].ih12);
+  for (int e = t; e < NGX; e += NT) {
+    const int a = e % TX;
+    const int b = (e / TX) % GXY;
+    const int cc = e / (TX * GXY);
+    if (a < T.nx && b < T.ny + 4 && cc < T.nz + 4) {
+      const int o = (a + 3) + SX * ((b + 1) + SY * (cc + 1)) - (T.i0 + a);
+      sgx[e] = d1_42<BND>(su, o, 1, T.i0 + a, ax[0].e, ax[0].lf, ax[0].hf, ax[0].ih2, ax[0].ih12);
+    }
   }
   // sgy: d_y u at block coords (i0+a, j0+b, k0-2+cc), su coords (a+3, b+3, cc+1)
-  const int nyg = T.nx * T.ny * (T.nz + 4);
-  for (int e = t; e < nyg; e += NT) {
-    const int a = e % T.nx;
-    const int bc = e / T.nx;
-    const int b = bc % T.ny;
-    const int cc = bc / T.ny;
-    const int o = (a + 3) + SX * ((b + 3) + SY * (cc + 1)) - (T.j0 + b) * SX;
-    sgy[a + TX * (b + TY * cc)] = d1_42<BND>(su, o, SX, T.j0 + b, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
+  for (int e = t; e < NGY; e += NT) {
+    const int a = e % TX;
+    const int b = (e / TX) % TY;
+    const int cc = e / (TX * TY);
+    if (a < T.nx && b < T.ny && cc < T.nz + 4) {
+      const int o = (a + 3) + SX * ((b + 3) + SY * (cc + 1)) - (T.j0 + b) * SX;
+      sgy[e] = d1_42<BND>(su, o, SX, T.j0 + b, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
+    }
   }
 }
 
@@ -428,15 +452,22 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
   __shared__ double su[NSU];
   __shared__ double sgx[NGX];
   __shared__ double sgy[NGY];
-  __shared__ double sacc[36 * NT];  // per-thread C2 (18), R, gt_rhs, At_rhs (6 each)
+  __shared__ double sacc[NSLOT * NPT];
   __shared__ IOPtrs s_io;           // array pointers of this launch
   __shared__ AxisC s_ax[3];         // stencil constants of the current block
 
   const int t = threadIdx.x;
   Ctx C;
-  C.ti = t % TX;
-  C.tj = (t / TX) % TY;
-  C.tk = t / (TX * TY);
+  // point of this thread in the tile, in the bank-conflict-free order of
+  // tile_map.h; threads beyond the tile's 343 points stay inactive
+  if (t < NPT) {
+    const int m = c_tile_map[t];
+    C.ti = m & 7;
+    C.tj = (m >> 3) & 7;
+    C.tk = m >> 6;
+  } else {
+    C.ti = C.tj = C.tk = TX;
+  }
   for (int i = 0; i < 4; ++i) C.P.lambda[i] = P.lambda[i];
   C.P.lambda_f[0] = P.lambda_f[0];
   C.P.lambda_f[1] = P.lambda_f[1];
@@ -467,10 +498,12 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
 
     // point coordinates, eta (rhs.cpp:214-223) and the Sommerfeld face test
     KPoint s;
+    // inactive threads (t >= NPT) never touch their slots
     s.C2.p = sacc + t;
-    s.R.p = sacc + 18 * NT + t;
-    s.gt_rhs.p = sacc + 24 * NT + t;
-    s.At_rhs.p = sacc + 30 * NT + t;
+    s.R.p = sacc + 18 * NPT + t;
+    s.gt_rhs.p = sacc + 24 * NPT + t;
+    s.At_rhs.p = sacc + 30 * NPT + t;
+    s.acc.p = sacc + 36 * NPT + t;
     C.o = T.off + (uint64_t)C.gi + (uint64_t)T.sz0 * (C.gj + (uint64_t)T.sz1 * C.gk);
     C.on_face = false;
     C.eta = P.eta_const;
@@ -491,10 +524,7 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
       s.chi = s_io.u.v[dgr::V_CHI][o];
       s.K = s_io.u.v[dgr::V_K][o];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        s.beta[i] = s_io.u.v[dgr::V_BETA0 + i][o];
-        s.B[i] = s_io.u.v[dgr::V_B0 + i][o];
-      }
+      for (int i = 0; i < 3; ++i) s.beta[i] = s_io.u.v[dgr::V_BETA0 + i][o];
 #pragma unroll
       for (int p = 0; p < 6; ++p) {
         s.gt[p] = s_io.u.v[dgr::V_SG0 + p][o];

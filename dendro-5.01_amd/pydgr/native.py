@@ -7,7 +7,7 @@ import ctypes
 import os
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG, "lib", "libdgr_bssn.so")
+LIB_PATH = os.environ.get("DGR_LIB_PATH") or os.path.join(_PKG, "lib", "libdgr_bssn.so")
 NUM_VARS = 24
 
 VAR_NAMES = ["alpha", "chi", "K", "Gt0", "Gt1", "Gt2", "beta0", "beta1", "beta2", "B0", "B1", "B2",

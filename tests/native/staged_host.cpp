@@ -52,7 +52,7 @@ extern "C" void staged_point_eqs(long n, const double* in, const double* eta, co
     }
     for (int d = 0; d < 3; ++d) d1[d] = I(G_K + d, pp);
     o[V_K] = s.add_K(d1, adv(V_K));
-    for (int k = 0; k < 3; ++k) o[V_GT0 + k] = s.Gt_rhs[k];
+    for (int k = 0; k < 3; ++k) o[V_GT0 + k] = s.Gt_out(k);
     for (int k = 0; k < 3; ++k) o[V_B0 + k] = s.add_B(P, k, adv(V_B0 + k), eta[pp]);
     for (int p = 0; p < 6; ++p) o[V_SA0 + p] = s.add_At(p, adv(V_SA0 + p));
     for (int v = 0; v < 24; ++v) out[(size_t)v * n + pp] = o[v];
