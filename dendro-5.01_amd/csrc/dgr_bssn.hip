@@ -33,13 +33,18 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "../../include/dgr_bssn.h"
 #include "bssn_staged.h"
+#include "dgr_internal.h"
 #include "tile_map.h"
 
 namespace {
+
+using namespace dgr_internal;
 
 constexpr int TX = 7, TY = 7, TZ = 7;         // interior tile
 constexpr int NPT = TX * TY * TZ;              // 343 points
@@ -50,49 +55,6 @@ constexpr int NPF = (NSU + NT - 1) / NT;       // prefetch registers per thread
 constexpr int GXY = TY + 4, GXZ = TZ + 4;      // widened x-derivative plane
 constexpr int NGX = TX * GXY * GXZ;            // 847
 constexpr int NGY = TX * TY * GXZ;             // 539
-
-// Per-axis stencil constants, computed on the host exactly as derivs.cpp
-// computes them (IEEE double, no contraction).
-struct AxisC {
-  double ih2, ih12, isq, isq12, pre, smr1, smr2, smr3;
-  int e;
-  int lf, hf;
-  int pad;
-};
-
-struct DevBlock {
-  uint64_t offset;
-  uint32_t sz[3];
-  uint32_t bflag;
-  double pmin[3];
-  double h[3];
-  AxisC ax[3];
-};
-
-struct DevTile {
-  uint32_t block;
-  uint16_t i0, j0, k0;  // first interior point of the tile (block coords)
-  uint8_t nx, ny, nz, pad;
-};
-
-struct VarPtrs {
-  const double* v[24];
-};
-struct OutPtrs {
-  double* v[24];
-};
-// The 48 array pointers live in device memory and are loaded where used,
-// which keeps them out of the scalar register file.
-struct IOPtrs {
-  VarPtrs u;
-  OutPtrs r;
-};
-
-struct DevParams {
-  double lambda[4];
-  double lambda_f[2];
-  double eta_const, eta_r0, eta_damping_exp, ko_sigma;
-};
 
 // ---------------------------------------------------------------------------
 // Stencils.  The value at axis coordinate q' of the current line is
@@ -541,42 +503,22 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-struct dgr_plan_s {
-  int device = 0;
-  uint32_t nblocks = 0;
-  uint32_t ntiles = 0;
-  uint64_t points = 0;
-  DevBlock* d_blocks = nullptr;
-  DevTile* d_tiles = nullptr;
-  uint32_t ntiles_interior = 0;           // tiles [0, n) belong to blocks with bflag 0
-  std::vector<uint32_t> block_tile_begin;  // first tile of block b
-  std::vector<uint32_t> block_tile_count;
-  DevParams params{};
-  int grid = 0;  // persistent grid size
-  IOPtrs* d_io = nullptr;  // device copy of the 48 array pointers
-  IOPtrs h_io{};           // last uploaded pointers
-  bool io_valid = false;
-};
-
 namespace {
-
 thread_local char g_errbuf[256];
+}  // namespace
 
-int hip_fail(hipError_t e, const char* what) {
+int dgr_internal::hip_fail(hipError_t e, const char* what) {
   snprintf(g_errbuf, sizeof(g_errbuf), "%s: %s", what, hipGetErrorString(e));
   return DGR_E_HIP;
 }
 
-#define DGR_HIP(call)                                   \
-  do {                                                  \
-    hipError_t _e = (call);                             \
-    if (_e != hipSuccess) return hip_fail(_e, #call);   \
-  } while (0)
+namespace {
 
 void free_plan(dgr_plan_s* plan) {
   (void)hipFree(plan->d_blocks);
   (void)hipFree(plan->d_tiles);
   (void)hipFree(plan->d_io);
+  (void)hipFree(plan->d_nbr);
   delete plan;
 }
 
@@ -599,6 +541,36 @@ AxisC host_axis(double h, int sz, unsigned bflag, int a) {
   c.hf = (bflag >> (2 * a + 1)) & 1u;
   c.pad = 0;
   return c;
+}
+
+// Same-level neighbour table: blocks of equal size and spacing whose
+// corners are sz-7 points apart along an axis are neighbours.  Corners are
+// compared on the integer lattice pmin/h.
+void build_neighbours(const dgr_block* blocks, const std::vector<DevBlock>& hb, uint32_t nblocks,
+                      std::vector<int32_t>& nbr) {
+  using Key = std::tuple<long long, long long, long long, uint32_t, uint32_t, uint32_t, uint64_t>;
+  auto key = [&](uint32_t b, int dx, int dy, int dz) {
+    const DevBlock& B = hb[b];
+    const int d[3] = {dx, dy, dz};
+    long long K[3];
+    for (int a = 0; a < 3; ++a) K[a] = llround(B.pmin[a] / B.h[a]) + (long long)d[a] * (long long)(B.sz[a] - 7);
+    uint64_t hb64;
+    std::memcpy(&hb64, &B.h[0], sizeof(hb64));
+    return Key(K[0], K[1], K[2], B.sz[0], B.sz[1], B.sz[2], hb64);
+  };
+  std::map<Key, int32_t> pos;
+  for (uint32_t b = 0; b < nblocks; ++b) pos.emplace(key(b, 0, 0, 0), (int32_t)b);
+  nbr.assign((size_t)nblocks * NNBR, -1);
+  for (uint32_t b = 0; b < nblocks; ++b)
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          if (!dx && !dy && !dz) continue;
+          auto it = pos.find(key(b, dx, dy, dz));
+          if (it != pos.end() && hb[it->second].h[1] == hb[b].h[1] && hb[it->second].h[2] == hb[b].h[2])
+            nbr[(size_t)b * NNBR + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] = it->second;
+        }
+  (void)blocks;
 }
 
 // Split n interior points into chunks of at most 7, as even as possible.
@@ -684,6 +656,11 @@ extern "C" int dgr_plan_create(dgr_plan_t* plan_out, const dgr_block* blocks, ui
   }
   plan->nblocks = nblocks;
   plan->ntiles = (uint32_t)ht.size();
+  for (uint32_t b = 0; b < nblocks; ++b)
+    plan->dof_end = std::max<uint64_t>(plan->dof_end, blocks[b].offset + (uint64_t)blocks[b].sz[0] * blocks[b].sz[1] *
+                                                                           blocks[b].sz[2]);
+  plan->chi_floor = p->chi_floor;
+  build_neighbours(blocks, hb, nblocks, plan->nbr);
   for (int i = 0; i < 4; ++i) plan->params.lambda[i] = p->lambda[i];
   plan->params.lambda_f[0] = p->lambda_f[0];
   plan->params.lambda_f[1] = p->lambda_f[1];
@@ -700,12 +677,15 @@ extern "C" int dgr_plan_create(dgr_plan_t* plan_out, const dgr_block* blocks, ui
   plan->grid = ncu;
   if (hipMalloc(&plan->d_blocks, sizeof(DevBlock) * nblocks) != hipSuccess ||
       hipMalloc(&plan->d_tiles, sizeof(DevTile) * ht.size()) != hipSuccess ||
-      hipMalloc(&plan->d_io, sizeof(IOPtrs)) != hipSuccess) {
+      hipMalloc(&plan->d_io, sizeof(IOPtrs)) != hipSuccess ||
+      hipMalloc(&plan->d_nbr, sizeof(int32_t) * plan->nbr.size()) != hipSuccess) {
     free_plan(plan);
     return DGR_E_NOMEM;
   }
   e = hipMemcpy(plan->d_blocks, hb.data(), sizeof(DevBlock) * nblocks, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(plan->d_tiles, ht.data(), sizeof(DevTile) * ht.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(plan->d_nbr, plan->nbr.data(), sizeof(int32_t) * plan->nbr.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     free_plan(plan);
     return hip_fail(e, "hipMemcpy(plan)");
@@ -743,21 +723,25 @@ int upload_ptrs(dgr_plan_t plan, double* const* rhs, const double* const* uin, h
 }
 }  // namespace
 
+int dgr_internal::launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st) {
+  const int ni = (int)plan->ntiles_interior;
+  const int nbd = (int)plan->ntiles - ni;
+  if (ni > 0)
+    hipLaunchKernelGGL(k_bssn_rhs<false>, dim3(std::min(plan->grid, ni)), dim3(NT), 0, st, plan->d_blocks,
+                       plan->d_tiles, 0, ni, d_io, plan->params);
+  if (nbd > 0)
+    hipLaunchKernelGGL(k_bssn_rhs<true>, dim3(std::min(plan->grid, nbd)), dim3(NT), 0, st, plan->d_blocks,
+                       plan->d_tiles, ni, nbd, d_io, plan->params);
+  DGR_HIP(hipGetLastError());
+  return DGR_OK;
+}
+
 extern "C" int dgr_bssn_rhs(dgr_plan_t plan, double* const* rhs, const double* const* uin, void* stream) {
   if (!plan) return DGR_E_INVALID;
   hipStream_t st = (hipStream_t)stream;
   int rc = upload_ptrs(plan, rhs, uin, st);
   if (rc) return rc;
-  const int ni = (int)plan->ntiles_interior;
-  const int nbd = (int)plan->ntiles - ni;
-  if (ni > 0)
-    hipLaunchKernelGGL(k_bssn_rhs<false>, dim3(std::min(plan->grid, ni)), dim3(NT), 0, st, plan->d_blocks,
-                       plan->d_tiles, 0, ni, plan->d_io, plan->params);
-  if (nbd > 0)
-    hipLaunchKernelGGL(k_bssn_rhs<true>, dim3(std::min(plan->grid, nbd)), dim3(NT), 0, st, plan->d_blocks,
-                       plan->d_tiles, ni, nbd, plan->d_io, plan->params);
-  DGR_HIP(hipGetLastError());
-  return DGR_OK;
+  return launch_rhs(plan, plan->d_io, st);
 }
 
 extern "C" int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const double* const* uin,

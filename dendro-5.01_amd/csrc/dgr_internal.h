@@ -1,0 +1,98 @@
+/*
+ * Internal declarations shared by the BSSN RHS kernels (dgr_bssn.hip) and
+ * the RK4 / ghost-refill / constraint kernels (dgr_rk4.hip).
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/dgr_bssn.h"
+
+namespace dgr_internal {
+
+// Per-axis stencil constants, computed on the host exactly as derivs.cpp
+// computes them (IEEE double, no contraction).
+struct AxisC {
+  double ih2, ih12, isq, isq12, pre, smr1, smr2, smr3;
+  int e;
+  int lf, hf;
+  int pad;
+};
+
+struct DevBlock {
+  uint64_t offset;
+  uint32_t sz[3];
+  uint32_t bflag;
+  double pmin[3];
+  double h[3];
+  AxisC ax[3];
+};
+
+struct DevTile {
+  uint32_t block;
+  uint16_t i0, j0, k0;  // first interior point of the tile (block coords)
+  uint8_t nx, ny, nz, pad;
+};
+
+struct VarPtrs {
+  const double* v[24];
+};
+struct OutPtrs {
+  double* v[24];
+};
+// The 48 array pointers live in device memory and are loaded where used,
+// which keeps them out of the scalar register file.
+struct IOPtrs {
+  VarPtrs u;
+  OutPtrs r;
+};
+
+struct DevParams {
+  double lambda[4];
+  double lambda_f[2];
+  double eta_const, eta_r0, eta_damping_exp, ko_sigma;
+};
+
+
+// Same-level neighbour of a block in direction (dx,dy,dz) in {-1,0,1}^3,
+// slot (dx+1) + 3 (dy+1) + 9 (dz+1); -1 when there is none (domain
+// boundary, or a neighbour of another size or spacing).
+constexpr int NNBR = 27;
+
+int hip_fail(hipError_t e, const char* what);
+
+#define DGR_HIP(call)                                 \
+  do {                                                \
+    hipError_t _e = (call);                           \
+    if (_e != hipSuccess) return dgr_internal::hip_fail(_e, #call); \
+  } while (0)
+
+}  // namespace dgr_internal
+
+struct dgr_plan_s {
+  int device = 0;
+  uint32_t nblocks = 0;
+  uint32_t ntiles = 0;
+  uint64_t points = 0;
+  dgr_internal::DevBlock* d_blocks = nullptr;
+  dgr_internal::DevTile* d_tiles = nullptr;
+  uint32_t ntiles_interior = 0;           // tiles [0, n) belong to blocks with bflag 0
+  std::vector<uint32_t> block_tile_begin;  // first tile of block b
+  std::vector<uint32_t> block_tile_count;
+  dgr_internal::DevParams params{};
+  int grid = 0;                          // persistent grid size
+  dgr_internal::IOPtrs* d_io = nullptr;  // device copy of the 48 array pointers
+  dgr_internal::IOPtrs h_io{};           // last uploaded pointers
+  bool io_valid = false;
+  int32_t* d_nbr = nullptr;              // NNBR neighbour slots per block
+  std::vector<int32_t> nbr;              // host copy of the neighbour table
+  uint64_t dof_end = 0;                  // one past the last padded point of any block
+  double chi_floor = 0.1;                // CHI_FLOOR of the constraint projection
+};
+
+namespace dgr_internal {
+// Launch the RHS kernels of `plan` with a device pointer table.
+int launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st);
+}  // namespace dgr_internal

@@ -1,0 +1,420 @@
+/*
+ * RK4 time step on device-resident padded blocks.
+ *
+ * Reference behaviour (BSSN_GR/src/rkBSSN.cpp:803-1011, coefficients
+ * BSSN_GR/include/rkBSSN.h:55-58): per stage s = 0..2
+ *     k_s  = RHS(u_s)                      (u_0 = u_prev)
+ *     u_{s+1} = u_prev + U[s+1] dt k_s,   then enforce_bssn_constraints
+ *     ghost exchange + unzip of u_{s+1}
+ * and finally u = u_prev + sum_s C[s] dt k_s (s = 0..3 in order), then
+ * enforce.  The reference does the combine on zipped nodal vectors; here
+ * the state stays unzipped: the combine runs on every block's interior
+ * points (shared face nodes are duplicated in both blocks and receive
+ * identical values), and the "unzip" is a same-level ghost refill that
+ * copies each ghost point from the neighbouring block's interior.
+ *
+ * Kernels:
+ *   k_stage          fused stage combine + running final sum + constraint
+ *                    projection (BSSN_GR/src/bssn_constraints.cpp:12-172)
+ *   k_enforce        the projection alone
+ *   k_ghost_refill   padding <- neighbour interiors (uniform level only;
+ *                    coarse/fine interpolation is not implemented)
+ * The four stages are captured once into a hipGraph and replayed.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/dgr_bssn.h"
+#include "dgr_internal.h"
+
+using namespace dgr_internal;
+
+namespace {
+
+constexpr int RK_NT = 256;
+enum { V_ALPHA = 0, V_CHI = 1, V_SG0 = 12, V_SA0 = 18 };
+
+struct Ptrs24 {
+  double* v[24];
+};
+
+// enforce_bssn_constraints on one point (bssn_constraints.cpp:12-172):
+// rescale gt to unit determinant, remove the trace of At, floor chi and
+// alpha.  Returns a nonzero flag where the reference would exit.
+__device__ __forceinline__ int enforce_point(double (&u)[24], double chi_floor) {
+#pragma clang fp contract(off)
+  double g00 = u[V_SG0], g01 = u[V_SG0 + 1], g02 = u[V_SG0 + 2], g11 = u[V_SG0 + 3], g12 = u[V_SG0 + 4],
+         g22 = u[V_SG0 + 5];
+  double det = g00 * (g11 * g22 - g12 * g12) - g01 * g01 * g22 + 2.0 * g01 * g02 * g12 - g02 * g02 * g11;
+  if (det < 0.0) return 1;
+  const double one_third = 1.0 / 3.0;
+  const double s = 1.0 / pow(det, one_third);
+  g00 *= s; g01 *= s; g02 *= s; g11 *= s; g12 *= s; g22 *= s;
+  det = g00 * (g11 * g22 - g12 * g12) - g01 * g01 * g22 + 2.0 * g01 * g02 * g12 - g02 * g02 * g11;
+  int bad = fabs(det - 1.0) > 1.0e-6 ? 2 : 0;
+  const double idet = 1.0 / det;
+  const double gu00 = idet * (g11 * g22 - g12 * g12);
+  const double gu01 = idet * (-g01 * g22 + g02 * g12);
+  const double gu02 = idet * (g01 * g12 - g02 * g11);
+  const double gu11 = idet * (g00 * g22 - g02 * g02);
+  const double gu12 = idet * (-g00 * g12 + g01 * g02);
+  const double gu22 = idet * (g00 * g11 - g01 * g01);
+  double a00 = u[V_SA0], a01 = u[V_SA0 + 1], a02 = u[V_SA0 + 2], a11 = u[V_SA0 + 3], a12 = u[V_SA0 + 4],
+         a22 = u[V_SA0 + 5];
+  const double t3 = one_third * (a00 * gu00 + a11 * gu11 + a22 * gu22 + 2.0 * (a01 * gu01 + a02 * gu02 + a12 * gu12));
+  a00 -= t3 * g00; a01 -= t3 * g01; a02 -= t3 * g02; a11 -= t3 * g11; a12 -= t3 * g12; a22 -= t3 * g22;
+  const double trA = a00 * gu00 + a11 * gu11 + a22 * gu22 + 2.0 * (a01 * gu01 + a02 * gu02 + a12 * gu12);
+  if (fabs(trA) > 1.0e-6) bad |= 4;
+  u[V_SA0] = a00; u[V_SA0 + 1] = a01; u[V_SA0 + 2] = a02; u[V_SA0 + 3] = a11; u[V_SA0 + 4] = a12; u[V_SA0 + 5] = a22;
+  u[V_SG0] = g00; u[V_SG0 + 1] = g01; u[V_SG0 + 2] = g02; u[V_SG0 + 3] = g11; u[V_SG0 + 4] = g12; u[V_SG0 + 5] = g22;
+  if (u[V_CHI] < chi_floor) u[V_CHI] = chi_floor;
+  u[V_ALPHA] = fmax(u[V_ALPHA], chi_floor);
+  return bad;
+}
+
+// Interior point of a tile handled by thread t (plain row-major order).
+__device__ __forceinline__ bool tile_point(const DevTile& tl, const DevBlock& B, int t, uint64_t& o) {
+  const int ti = t % 7, tj = (t / 7) % 7, tk = t / 49;
+  if (t >= 343 || ti >= tl.nx || tj >= tl.ny || tk >= tl.nz) return false;
+  const uint64_t i = tl.i0 + ti, j = tl.j0 + tj, k = tl.k0 + tk;
+  o = B.offset + i + (uint64_t)B.sz[0] * (j + (uint64_t)B.sz[1] * k);
+  return true;
+}
+
+// Coefficients in device memory, rewritten when dt changes:
+// cu[s] = U[s+1] dt (s = 0..2), cc[s] = C[s] dt (s = 0..3).
+struct RKCoef {
+  double cu[3];
+  double cc[4];
+  double chi_floor;
+};
+
+__global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ blocks, const DevTile* __restrict__ tiles,
+                                              int ntiles, const Ptrs24* __restrict__ tab, const RKCoef* __restrict__ co,
+                                              int stage, unsigned* __restrict__ flag) {
+  // tab[0] = u_prev (state), tab[1] = k_s, tab[2] = running sum, tab[3] = u_{s+1}
+  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    const DevTile tl = tiles[w];
+    const DevBlock& B = blocks[tl.block];
+    uint64_t o;
+    if (!tile_point(tl, B, threadIdx.x, o)) continue;
+    double un[24];
+    if (stage < 3) {
+      const double cu = co->cu[stage], cc = co->cc[stage];
+#pragma unroll
+      for (int v = 0; v < 24; ++v) {
+#pragma clang fp contract(off)
+        const double up = tab[0].v[v][o];
+        const double k = tab[1].v[v][o];
+        const double acc = (stage == 0) ? up : tab[2].v[v][o];
+        tab[2].v[v][o] = acc + cc * k;
+        un[v] = up + cu * k;
+      }
+      const int bad = enforce_point(un, co->chi_floor);
+      if (bad) atomicOr(flag, (unsigned)bad);
+#pragma unroll
+      for (int v = 0; v < 24; ++v) tab[3].v[v][o] = un[v];
+    } else {
+      const double cc = co->cc[3];
+#pragma unroll
+      for (int v = 0; v < 24; ++v) {
+#pragma clang fp contract(off)
+        un[v] = tab[2].v[v][o] + cc * tab[1].v[v][o];
+      }
+      const int bad = enforce_point(un, co->chi_floor);
+      if (bad) atomicOr(flag, (unsigned)bad);
+#pragma unroll
+      for (int v = 0; v < 24; ++v) tab[0].v[v][o] = un[v];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(384) k_enforce(const DevBlock* __restrict__ blocks, const DevTile* __restrict__ tiles,
+                                                int ntiles, const Ptrs24* __restrict__ tab, double chi_floor,
+                                                unsigned* __restrict__ flag) {
+  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    const DevTile tl = tiles[w];
+    const DevBlock& B = blocks[tl.block];
+    uint64_t o;
+    if (!tile_point(tl, B, threadIdx.x, o)) continue;
+    double u[24];
+#pragma unroll
+    for (int v = 0; v < 24; ++v) u[v] = tab->v[v][o];
+    const int bad = enforce_point(u, chi_floor);
+    if (bad) atomicOr(flag, (unsigned)bad);
+    if (bad & 1) continue;  // the reference exits before touching this node
+#pragma unroll
+    for (int v = 0; v < 24; ++v) tab->v[v][o] = u[v];
+  }
+}
+
+// Ghost refill: every padded point outside [3, sz-3)^3 takes the value of
+// the same physical point in the interior of the same-level neighbour.
+__global__ void __launch_bounds__(RK_NT) k_ghost_refill(const DevBlock* __restrict__ blocks,
+                                                       const int32_t* __restrict__ nbr, int nblocks,
+                                                       const Ptrs24* __restrict__ tab) {
+  for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const DevBlock& B = blocks[b];
+    const int s0 = B.sz[0], s1 = B.sz[1], s2 = B.sz[2];
+    const int n = s0 * s1 * s2;
+    for (int e = threadIdx.x; e < n; e += RK_NT) {
+      const int i = e % s0, j = (e / s0) % s1, k = e / (s0 * s1);
+      const int di = i < 3 ? -1 : (i >= s0 - 3 ? 1 : 0);
+      const int dj = j < 3 ? -1 : (j >= s1 - 3 ? 1 : 0);
+      const int dk = k < 3 ? -1 : (k >= s2 - 3 ? 1 : 0);
+      if ((di | dj | dk) == 0) continue;
+      const int nb = nbr[(size_t)b * NNBR + (di + 1) + 3 * (dj + 1) + 9 * (dk + 1)];
+      if (nb < 0) continue;
+      const DevBlock& N = blocks[nb];
+      // neighbour corners are sz-7 points apart along each axis
+      const int si = i - di * (s0 - 7), sj = j - dj * (s1 - 7), sk = k - dk * (s2 - 7);
+      const uint64_t src = N.offset + (uint64_t)si + (uint64_t)s0 * (sj + (uint64_t)s1 * sk);
+      const uint64_t dst = B.offset + (uint64_t)e;
+#pragma unroll 4
+      for (int v = 0; v < 24; ++v) tab->v[v][dst] = tab->v[v][src];
+    }
+  }
+}
+
+int grid_for(dgr_plan_t plan, int n) {
+  const int g = plan->grid * 4;
+  return n < g ? (n > 0 ? n : 1) : g;
+}
+
+}  // namespace
+
+struct dgr_rk4_s {
+  dgr_plan_t plan = nullptr;
+  uint64_t dof = 0;
+  double* buf = nullptr;        // acc, stage state, k: 3 x 24 x dof
+  Ptrs24 state{};
+  Ptrs24* d_tab = nullptr;      // [4] per stage kind, see k_stage
+  Ptrs24* d_state = nullptr;    // the state alone (refill / enforce)
+  Ptrs24* d_stage = nullptr;    // the stage state alone
+  IOPtrs* d_io = nullptr;       // RHS pointer tables: [0] (u -> k), [1] (stage -> k)
+  RKCoef* d_coef = nullptr;
+  unsigned* d_flag = nullptr;
+  double dt = -1.0;
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool use_graph = true;
+  RKCoef h_coef{};
+};
+
+namespace {
+
+int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
+  dgr_plan_t p = r->plan;
+  const int nt = (int)p->ntiles;
+  for (int s = 0; s < 4; ++s) {
+    // k_s = RHS(u_s): stage 0 reads the state, later stages the stage buffer
+    int rc = launch_rhs(p, r->d_io + (s == 0 ? 0 : 1), st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_stage, dim3(grid_for(p, nt)), dim3(384), 0, st, p->d_blocks, p->d_tiles, nt, r->d_tab,
+                       r->d_coef, s, r->d_flag);
+    hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(p, (int)p->nblocks)), dim3(RK_NT), 0, st, p->d_blocks,
+                       p->d_nbr, (int)p->nblocks, s < 3 ? r->d_stage : r->d_state);
+  }
+  DGR_HIP(hipGetLastError());
+  return DGR_OK;
+}
+
+int set_dt(dgr_rk4_s* r, double dt, hipStream_t st) {
+  if (dt == r->dt) return DGR_OK;
+  // RK4_U / RK4_C of BSSN_GR/include/rkBSSN.h:55-58; the products are
+  // formed as the reference forms them (RK4_U[s+1] * dt, RK4_C[s] * dt)
+  static const double U[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
+  static const double C[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+  for (int s = 0; s < 3; ++s) r->h_coef.cu[s] = U[s + 1] * dt;
+  for (int s = 0; s < 4; ++s) r->h_coef.cc[s] = C[s] * dt;
+  r->h_coef.chi_floor = r->plan->chi_floor;
+  DGR_HIP(hipMemcpyAsync(r->d_coef, &r->h_coef, sizeof(RKCoef), hipMemcpyHostToDevice, st));
+  DGR_HIP(hipStreamSynchronize(st));
+  r->dt = dt;
+  return DGR_OK;
+}
+
+int upload_tab(Ptrs24* d, const Ptrs24* h, int n, hipStream_t st) {
+  DGR_HIP(hipMemcpyAsync(d, h, sizeof(Ptrs24) * n, hipMemcpyHostToDevice, st));
+  return DGR_OK;
+}
+
+int check_ptrs(double* const* u) {
+  if (!u) return DGR_E_INVALID;
+  for (int v = 0; v < 24; ++v)
+    if (!u[v]) return DGR_E_INVALID;
+  return DGR_OK;
+}
+
+// Small device table of 24 pointers for a one-off kernel call; freed after
+// the stream has consumed it.
+struct TmpTab {
+  Ptrs24* d = nullptr;
+  ~TmpTab() { (void)hipFree(d); }
+};
+
+void destroy(dgr_rk4_s* r) {
+  if (r->exec) (void)hipGraphExecDestroy(r->exec);
+  if (r->graph) (void)hipGraphDestroy(r->graph);
+  if (r->st) (void)hipStreamDestroy(r->st);
+  (void)hipFree(r->buf);
+  (void)hipFree(r->d_tab);
+  (void)hipFree(r->d_io);
+  (void)hipFree(r->d_coef);
+  (void)hipFree(r->d_flag);
+  delete r;
+}
+
+}  // namespace
+
+extern "C" int dgr_ghost_refill(dgr_plan_t plan, double* const* u, void* stream) {
+  if (!plan || check_ptrs(u)) return DGR_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  Ptrs24 h;
+  for (int v = 0; v < 24; ++v) h.v[v] = u[v];
+  TmpTab t;
+  DGR_HIP(hipMalloc(&t.d, sizeof(Ptrs24)));
+  DGR_HIP(hipMemcpyAsync(t.d, &h, sizeof(Ptrs24), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(plan, (int)plan->nblocks)), dim3(RK_NT), 0, st, plan->d_blocks,
+                     plan->d_nbr, (int)plan->nblocks, t.d);
+  DGR_HIP(hipGetLastError());
+  DGR_HIP(hipStreamSynchronize(st));
+  return DGR_OK;
+}
+
+extern "C" int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* stream) {
+  if (!plan || check_ptrs(u)) return DGR_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  Ptrs24 h;
+  for (int v = 0; v < 24; ++v) h.v[v] = u[v];
+  TmpTab t;
+  unsigned* d_flag = nullptr;
+  DGR_HIP(hipMalloc(&t.d, sizeof(Ptrs24) + 256));
+  d_flag = (unsigned*)((char*)t.d + sizeof(Ptrs24));
+  DGR_HIP(hipMemcpyAsync(t.d, &h, sizeof(Ptrs24), hipMemcpyHostToDevice, st));
+  DGR_HIP(hipMemsetAsync(d_flag, 0, sizeof(unsigned), st));
+  const int nt = (int)plan->ntiles;
+  hipLaunchKernelGGL(k_enforce, dim3(grid_for(plan, nt)), dim3(384), 0, st, plan->d_blocks, plan->d_tiles, nt, t.d,
+                     plan->chi_floor, d_flag);
+  DGR_HIP(hipGetLastError());
+  unsigned flag = 0;
+  DGR_HIP(hipMemcpyAsync(&flag, d_flag, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  DGR_HIP(hipStreamSynchronize(st));
+  return flag ? DGR_E_CONSTRAINT : DGR_OK;
+}
+
+extern "C" int dgr_rk4_create(dgr_rk4_t* out, dgr_plan_t plan, double* const* u, int use_graph) {
+  if (!out || !plan || check_ptrs(u)) return DGR_E_INVALID;
+  dgr_rk4_s* r = new dgr_rk4_s();
+  r->plan = plan;
+  r->dof = plan->dof_end;
+  r->use_graph = use_graph != 0;
+  for (int v = 0; v < 24; ++v) r->state.v[v] = u[v];
+  const size_t nb = sizeof(double) * r->dof;
+  if (hipMalloc(&r->buf, nb * 24 * 3) != hipSuccess || hipMalloc(&r->d_tab, sizeof(Ptrs24) * 6) != hipSuccess ||
+      hipMalloc(&r->d_io, sizeof(IOPtrs) * 2) != hipSuccess || hipMalloc(&r->d_coef, sizeof(RKCoef)) != hipSuccess ||
+      hipMalloc(&r->d_flag, 256) != hipSuccess) {
+    destroy(r);
+    return DGR_E_NOMEM;
+  }
+  Ptrs24 acc, stg, k;
+  for (int v = 0; v < 24; ++v) {
+    acc.v[v] = r->buf + (size_t)v * r->dof;
+    stg.v[v] = r->buf + (size_t)(24 + v) * r->dof;
+    k.v[v] = r->buf + (size_t)(48 + v) * r->dof;
+  }
+  // tables: [0..3] k_stage (state, k, acc, stage), [4] state, [5] stage
+  Ptrs24 h[6] = {r->state, k, acc, stg, r->state, stg};
+  IOPtrs io[2];
+  for (int v = 0; v < 24; ++v) {
+    io[0].u.v[v] = r->state.v[v];
+    io[0].r.v[v] = k.v[v];
+    io[1].u.v[v] = stg.v[v];
+    io[1].r.v[v] = k.v[v];
+  }
+  r->d_state = r->d_tab + 4;
+  r->d_stage = r->d_tab + 5;
+  hipError_t e = hipStreamCreateWithFlags(&r->st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMemcpy(r->d_tab, h, sizeof(h), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(r->d_io, io, sizeof(io), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(r->d_flag, 0, 256);
+  // the stage buffer's padding must hold valid data before the first
+  // refill: start it as a copy of the state
+  for (int v = 0; v < 24 && e == hipSuccess; ++v)
+    e = hipMemcpy(stg.v[v], r->state.v[v], nb, hipMemcpyDeviceToDevice);
+  if (e != hipSuccess) {
+    destroy(r);
+    return hip_fail(e, "dgr_rk4_create");
+  }
+  *out = r;
+  return DGR_OK;
+}
+
+extern "C" int dgr_rk4_step(dgr_rk4_t r, double dt, void* stream) { return dgr_rk4_steps(r, dt, 1, stream); }
+
+extern "C" int dgr_rk4_steps(dgr_rk4_t r, double dt, int nsteps, void* stream) {
+  if (!r || nsteps < 0 || !(dt > 0.0)) return DGR_E_INVALID;
+  hipStream_t caller = (hipStream_t)stream;
+  // order after the caller's work, run on the evolver's stream (graphs
+  // cannot be captured on the legacy default stream), order the caller after
+  hipEvent_t ev;
+  DGR_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  hipError_t e = hipEventRecord(ev, caller);
+  if (e == hipSuccess) e = hipStreamWaitEvent(r->st, ev, 0);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(ev);
+    return hip_fail(e, "dgr_rk4_steps");
+  }
+  int rc = set_dt(r, dt, r->st);
+  if (rc == DGR_OK && r->use_graph && !r->exec) {
+    // capture one step once; later steps replay it
+    e = hipStreamBeginCapture(r->st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) rc = hip_fail(e, "hipStreamBeginCapture");
+    if (rc == DGR_OK) {
+      rc = enqueue_step(r, r->st);
+      e = hipStreamEndCapture(r->st, &r->graph);
+      if (rc == DGR_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+    }
+    if (rc == DGR_OK) {
+      e = hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+    }
+  }
+  for (int n = 0; rc == DGR_OK && n < nsteps; ++n) {
+    if (r->use_graph) {
+      e = hipGraphLaunch(r->exec, r->st);
+      if (e != hipSuccess) rc = hip_fail(e, "hipGraphLaunch");
+    } else {
+      rc = enqueue_step(r, r->st);
+    }
+  }
+  if (rc == DGR_OK) {
+    e = hipEventRecord(ev, r->st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(caller, ev, 0);
+    if (e != hipSuccess) rc = hip_fail(e, "dgr_rk4_steps");
+  }
+  (void)hipEventDestroy(ev);
+  return rc;
+}
+
+extern "C" int dgr_rk4_status(dgr_rk4_t r) {
+  if (!r) return DGR_E_INVALID;
+  unsigned flag = 0;
+  DGR_HIP(hipStreamSynchronize(r->st));
+  DGR_HIP(hipMemcpy(&flag, r->d_flag, sizeof(unsigned), hipMemcpyDeviceToHost));
+  DGR_HIP(hipMemset(r->d_flag, 0, sizeof(unsigned)));
+  return flag ? DGR_E_CONSTRAINT : DGR_OK;
+}
+
+extern "C" int dgr_rk4_destroy(dgr_rk4_t r) {
+  if (!r) return DGR_E_INVALID;
+  (void)hipStreamSynchronize(r->st);
+  destroy(r);
+  return DGR_OK;
+}

@@ -71,6 +71,13 @@ def lib():
             getattr(L, f).argtypes = [vp, pp, pp, vp]
         L.dgr_bssn_rhs_blocks.argtypes = [pp, pp, ctypes.POINTER(Block), ctypes.c_uint32,
                                           ctypes.POINTER(BSSNParams), vp]
+        L.dgr_ghost_refill.argtypes = [vp, pp, vp]
+        L.dgr_enforce_constraints.argtypes = [vp, pp, vp]
+        L.dgr_rk4_create.argtypes = [ctypes.POINTER(vp), vp, pp, ctypes.c_int]
+        L.dgr_rk4_step.argtypes = [vp, ctypes.c_double, vp]
+        L.dgr_rk4_steps.argtypes = [vp, ctypes.c_double, ctypes.c_int, vp]
+        L.dgr_rk4_status.argtypes = [vp]
+        L.dgr_rk4_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -142,6 +149,19 @@ class Plan:
         f = L.dgr_bssn_rhs_per_block if per_block else L.dgr_bssn_rhs
         check(f(self.handle, rp, up, _stream_ptr(stream)), f.__name__)
 
+    def ghost_refill(self, u, stream=None):
+        up, _u = ptr_array(var_ptrs(u))
+        check(lib().dgr_ghost_refill(self.handle, up, _stream_ptr(stream)), "dgr_ghost_refill")
+
+    def enforce_constraints(self, u, stream=None):
+        """Returns True when every point passed the reference's checks."""
+        up, _u = ptr_array(var_ptrs(u))
+        rc = lib().dgr_enforce_constraints(self.handle, up, _stream_ptr(stream))
+        if rc == -5:
+            return False
+        check(rc, "dgr_enforce_constraints")
+        return True
+
     def close(self):
         if getattr(self, "handle", None):
             lib().dgr_plan_destroy(self.handle)
@@ -162,3 +182,36 @@ def rhs_blocks(rhs, u, blocks, params, stream=None):
     up, _u = ptr_array(var_ptrs(u))
     check(L.dgr_bssn_rhs_blocks(rp, up, barr, len(blocks), ctypes.byref(params), _stream_ptr(stream)),
           "dgr_bssn_rhs_blocks")
+
+
+class RK4:
+    """Device-resident RK4 evolver of the (24, dof) state `u` (dgr_rk4_*)."""
+
+    def __init__(self, plan, u, use_graph=True):
+        self.plan = plan
+        self.u = u
+        up, _u = ptr_array(var_ptrs(u))
+        h = ctypes.c_void_p()
+        check(lib().dgr_rk4_create(ctypes.byref(h), plan.handle, up, 1 if use_graph else 0), "dgr_rk4_create")
+        self.handle = h
+
+    def step(self, dt, nsteps=1, stream=None):
+        check(lib().dgr_rk4_steps(self.handle, float(dt), int(nsteps), _stream_ptr(stream)), "dgr_rk4_steps")
+
+    def status_ok(self):
+        rc = lib().dgr_rk4_status(self.handle)
+        if rc == -5:
+            return False
+        check(rc, "dgr_rk4_status")
+        return True
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().dgr_rk4_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -101,6 +101,40 @@ int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const double* co
 int dgr_bssn_rhs_blocks(double* const* rhs, const double* const* u, const dgr_block* blocks,
                         uint32_t nblocks, const dgr_bssn_params* params, void* stream);
 
+/* Same-level "unzip": every ghost point of every block (outside
+ * [3, sz-3)^3) is overwritten with the same physical point of the
+ * neighbouring block's interior, for neighbours of equal size and spacing
+ * (corners sz-7 points apart).  Stands in for the ghost exchange + unzip
+ * between RK stages (BSSN_GR/src/rkBSSN.cpp:909-914, Mesh::unzip
+ * include/mesh.tcc:7436) on uniform layouts; ghosts with no same-level
+ * neighbour (domain boundary, level jumps) are left untouched.
+ * Synchronous. */
+int dgr_ghost_refill(dgr_plan_t plan, double* const* u, void* stream);
+
+/* enforce_bssn_constraints (BSSN_GR/src/bssn_constraints.cpp:12-172) on
+ * every interior point: det(gt) = 1, tr(At) = 0, chi and alpha floored at
+ * params.chi_floor.  Returns DGR_E_CONSTRAINT where the reference exits
+ * (det < 0, |det-1| > 1e-6 after rescaling, |tr A| > 1e-6); points with
+ * det < 0 are left unchanged.  Synchronous. */
+int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* stream);
+
+/* Device-resident RK4 evolution of the 24 state arrays u (caller-owned
+ * device memory, ghost layers filled).  One step replaces the RK4 branch of
+ * RK_BSSN::performSingleIteration (BSSN_GR/src/rkBSSN.cpp:803-1011):
+ * four RHS evaluations, the stage combines u_prev + RK4_U[s+1] dt k_s with
+ * enforce_bssn_constraints, the final u_prev + sum_s RK4_C[s] dt k_s
+ * (rkBSSN.h:55-58) and a same-level ghost refill after every stage.  The
+ * step is captured into a hipGraph on first use (use_graph != 0) and
+ * replayed.  Work buffers: 3 x 24 x dof doubles. */
+typedef struct dgr_rk4_s* dgr_rk4_t;
+int dgr_rk4_create(dgr_rk4_t* rk, dgr_plan_t plan, double* const* u, int use_graph);
+int dgr_rk4_step(dgr_rk4_t rk, double dt, void* stream);
+int dgr_rk4_steps(dgr_rk4_t rk, double dt, int nsteps, void* stream);
+/* Waits for the evolver and returns DGR_E_CONSTRAINT if any projection
+ * since the last call met a condition the reference exits on. */
+int dgr_rk4_status(dgr_rk4_t rk);
+int dgr_rk4_destroy(dgr_rk4_t rk);
+
 #ifdef __cplusplus
 }
 #endif

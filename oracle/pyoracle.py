@@ -124,3 +124,64 @@ def rk4_final(u_prev, ks, dt):
     arr = (dp * 4)(*[_p(k) for k in ks])
     L.orc_rk4_final(_p(out), _p(u_prev), arr, dt, u_prev.size)
     return out
+
+
+def ghost_refill_uniform(U, nb, sz=13):
+    """Same-level ghost refill on the nb^3 uniform block grid of
+    pydgr.layout.uniform_grid (blocks in x-fastest order, corners sz-7 points
+    apart): every ghost box of every block takes the neighbour's interior
+    values; boxes without a neighbour (domain boundary) are left as they are."""
+    E = sz - 7
+    V = U.reshape(U.shape[0], nb, nb, nb, sz, sz, sz)  # [var, bz, by, bx, k, j, i]
+
+    def rng(d):
+        return {-1: (0, 3), 0: (3, sz - 3), 1: (sz - 3, sz)}[d]
+
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                if dx == dy == dz == 0:
+                    continue
+                b = [slice(max(0, -d), nb - max(0, d)) for d in (dz, dy, dx)]
+                s = [slice(max(0, d), nb - max(0, -d)) for d in (dz, dy, dx)]
+                (k0, k1), (j0, j1), (i0, i1) = rng(dz), rng(dy), rng(dx)
+                V[:, b[0], b[1], b[2], k0:k1, j0:j1, i0:i1] = \
+                    V[:, s[0], s[1], s[2], k0 - dz * E:k1 - dz * E, j0 - dy * E:j1 - dy * E, i0 - dx * E:i1 - dx * E]
+    return U
+
+
+def rk4_step_uniform(U, blocks, nb, dt, prm=None, chi_floor=0.1, nthreads=0):
+    """One RK4 step of rkBSSN.cpp:803-1011 on unzipped uniform-grid blocks:
+    stage combines u_prev + (U[s+1]*dt)*k_s with the constraint projection,
+    the final u_prev + sum_s (C[s]*dt)*k_s in order s = 0..3, and a
+    same-level ghost refill after every stage.  Returns (u_new, bad)."""
+    from pydgr import layout  # noqa: WPS433 (test infrastructure)
+    Uc = [0.0, 0.5, 0.5, 1.0]
+    Cc = [1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0]
+    ii = layout.interior_index(blocks)
+    uprev = np.ascontiguousarray(U, dtype=np.float64)
+    stage = uprev.copy()
+    cur = uprev
+    acc = None
+    bad = 0
+    for s in range(4):
+        k = rhs_blocks(cur, blocks, prm, nthreads=nthreads, fill=0.0)
+        kk = k[:, ii]
+        up = uprev[:, ii]
+        if s == 0:
+            acc = up + (Cc[0] * dt) * kk
+        else:
+            acc = acc + (Cc[s] * dt) * kk
+        if s < 3:
+            un = up + (Uc[s + 1] * dt) * kk
+            un, b = enforce_constraints(np.ascontiguousarray(un), chi_floor)
+            bad |= b
+            stage[:, ii] = un
+            ghost_refill_uniform(stage, nb)
+            cur = stage
+    unew = uprev.copy()
+    fin, b = enforce_constraints(np.ascontiguousarray(acc), chi_floor)
+    bad |= b
+    unew[:, ii] = fin
+    ghost_refill_uniform(unew, nb)
+    return unew, bad
