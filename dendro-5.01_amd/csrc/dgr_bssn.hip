@@ -548,7 +548,7 @@ AxisC host_axis(double h, int sz, unsigned bflag, int a) {
 // compared on the integer lattice pmin/h.
 void build_neighbours(const dgr_block* blocks, const std::vector<DevBlock>& hb, uint32_t nblocks,
                       std::vector<int32_t>& nbr) {
-  using Key = std::tuple<long long, long long, long long, uint32_t, uint32_t, uint32_t, uint64_t>;
+  using Key = std::tuple<long long, long long, long long, uint32_t, uint32_t, uint32_t, uint64_t, uint32_t>;
   auto key = [&](uint32_t b, int dx, int dy, int dz) {
     const DevBlock& B = hb[b];
     const int d[3] = {dx, dy, dz};
@@ -556,7 +556,7 @@ void build_neighbours(const dgr_block* blocks, const std::vector<DevBlock>& hb, 
     for (int a = 0; a < 3; ++a) K[a] = llround(B.pmin[a] / B.h[a]) + (long long)d[a] * (long long)(B.sz[a] - 7);
     uint64_t hb64;
     std::memcpy(&hb64, &B.h[0], sizeof(hb64));
-    return Key(K[0], K[1], K[2], B.sz[0], B.sz[1], B.sz[2], hb64);
+    return Key(K[0], K[1], K[2], B.sz[0], B.sz[1], B.sz[2], hb64, B.group);
   };
   std::map<Key, int32_t> pos;
   for (uint32_t b = 0; b < nblocks; ++b) pos.emplace(key(b, 0, 0, 0), (int32_t)b);
@@ -635,6 +635,9 @@ extern "C" int dgr_plan_create(dgr_plan_t* plan_out, const dgr_block* blocks, ui
       }
       for (int d = 0; d < 3; ++d) db.ax[d] = host_axis(db.h[d], (int)bl.sz[d], bl.bflag, d);
       db.bflag = bl.bflag;
+      db.group = bl.group;
+      db.pad = 0;
+      plan->ngroups = std::max<uint32_t>(plan->ngroups, bl.group + 1);
       plan->block_tile_begin[b] = (uint32_t)ht.size();
       split(bl.sz[0] - 6, sx);
       split(bl.sz[1] - 6, sy);

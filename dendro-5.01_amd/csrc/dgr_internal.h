@@ -28,6 +28,8 @@ struct DevBlock {
   double pmin[3];
   double h[3];
   AxisC ax[3];
+  uint32_t group;
+  uint32_t pad;
 };
 
 struct DevTile {
@@ -90,6 +92,7 @@ struct dgr_plan_s {
   std::vector<int32_t> nbr;              // host copy of the neighbour table
   uint64_t dof_end = 0;                  // one past the last padded point of any block
   double chi_floor = 0.1;                // CHI_FLOOR of the constraint projection
+  uint32_t ngroups = 1;                  // 1 + largest block group
 };
 
 namespace dgr_internal {

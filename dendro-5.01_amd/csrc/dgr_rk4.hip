@@ -86,26 +86,26 @@ __device__ __forceinline__ bool tile_point(const DevTile& tl, const DevBlock& B,
   return true;
 }
 
-// Coefficients in device memory, rewritten when dt changes:
-// cu[s] = U[s+1] dt (s = 0..2), cc[s] = C[s] dt (s = 0..3).
-struct RKCoef {
-  double cu[3];
-  double cc[4];
-  double chi_floor;
-};
+// RK4_U[s+1] and RK4_C[s] of BSSN_GR/include/rkBSSN.h:55-58
+__constant__ double c_rkU[3] = {1.0 / 2.0, 1.0 / 2.0, 1.0};
+__constant__ double c_rkC[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
 
+// Stage s of the RK4 step.  gdt[g] is the time step of block group g; the
+// coefficients are formed as the reference forms them (RK4_U[s+1] * dt,
+// RK4_C[s] * dt) and then multiply k.
 __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ blocks, const DevTile* __restrict__ tiles,
-                                              int ntiles, const Ptrs24* __restrict__ tab, const RKCoef* __restrict__ co,
-                                              int stage, unsigned* __restrict__ flag) {
+                                              int ntiles, const Ptrs24* __restrict__ tab, const double* __restrict__ gdt,
+                                              double chi_floor, int stage, unsigned* __restrict__ flag) {
   // tab[0] = u_prev (state), tab[1] = k_s, tab[2] = running sum, tab[3] = u_{s+1}
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const DevTile tl = tiles[w];
     const DevBlock& B = blocks[tl.block];
     uint64_t o;
     if (!tile_point(tl, B, threadIdx.x, o)) continue;
+    const double dt = gdt[B.group];
     double un[24];
     if (stage < 3) {
-      const double cu = co->cu[stage], cc = co->cc[stage];
+      const double cu = c_rkU[stage] * dt, cc = c_rkC[stage] * dt;
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
 #pragma clang fp contract(off)
@@ -115,18 +115,18 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
         tab[2].v[v][o] = acc + cc * k;
         un[v] = up + cu * k;
       }
-      const int bad = enforce_point(un, co->chi_floor);
+      const int bad = enforce_point(un, chi_floor);
       if (bad) atomicOr(flag, (unsigned)bad);
 #pragma unroll
       for (int v = 0; v < 24; ++v) tab[3].v[v][o] = un[v];
     } else {
-      const double cc = co->cc[3];
+      const double cc = c_rkC[3] * dt;
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
 #pragma clang fp contract(off)
         un[v] = tab[2].v[v][o] + cc * tab[1].v[v][o];
       }
-      const int bad = enforce_point(un, co->chi_floor);
+      const int bad = enforce_point(un, chi_floor);
       if (bad) atomicOr(flag, (unsigned)bad);
 #pragma unroll
       for (int v = 0; v < 24; ++v) tab[0].v[v][o] = un[v];
@@ -197,14 +197,15 @@ struct dgr_rk4_s {
   Ptrs24* d_state = nullptr;    // the state alone (refill / enforce)
   Ptrs24* d_stage = nullptr;    // the stage state alone
   IOPtrs* d_io = nullptr;       // RHS pointer tables: [0] (u -> k), [1] (stage -> k)
-  RKCoef* d_coef = nullptr;
+  double* d_gdt = nullptr;      // time step per block group
+  std::vector<double> h_gdt;
+  bool gdt_set = false;
   unsigned* d_flag = nullptr;
-  double dt = -1.0;
+  bool own_buf = true;
   hipStream_t st = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = true;
-  RKCoef h_coef{};
 };
 
 namespace {
@@ -217,7 +218,7 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
     int rc = launch_rhs(p, r->d_io + (s == 0 ? 0 : 1), st);
     if (rc) return rc;
     hipLaunchKernelGGL(k_stage, dim3(grid_for(p, nt)), dim3(384), 0, st, p->d_blocks, p->d_tiles, nt, r->d_tab,
-                       r->d_coef, s, r->d_flag);
+                       r->d_gdt, p->chi_floor, s, r->d_flag);
     hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(p, (int)p->nblocks)), dim3(RK_NT), 0, st, p->d_blocks,
                        p->d_nbr, (int)p->nblocks, s < 3 ? r->d_stage : r->d_state);
   }
@@ -225,18 +226,21 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
   return DGR_OK;
 }
 
-int set_dt(dgr_rk4_s* r, double dt, hipStream_t st) {
-  if (dt == r->dt) return DGR_OK;
-  // RK4_U / RK4_C of BSSN_GR/include/rkBSSN.h:55-58; the products are
-  // formed as the reference forms them (RK4_U[s+1] * dt, RK4_C[s] * dt)
-  static const double U[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
-  static const double C[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
-  for (int s = 0; s < 3; ++s) r->h_coef.cu[s] = U[s + 1] * dt;
-  for (int s = 0; s < 4; ++s) r->h_coef.cc[s] = C[s] * dt;
-  r->h_coef.chi_floor = r->plan->chi_floor;
-  DGR_HIP(hipMemcpyAsync(r->d_coef, &r->h_coef, sizeof(RKCoef), hipMemcpyHostToDevice, st));
-  DGR_HIP(hipStreamSynchronize(st));
-  r->dt = dt;
+int set_group_dt(dgr_rk4_s* r, const double* dt, uint32_t n) {
+  if (n < r->plan->ngroups) return DGR_E_INVALID;
+  bool same = r->gdt_set;
+  for (uint32_t g = 0; g < r->plan->ngroups; ++g) {
+    if (!(dt[g] > 0.0)) return DGR_E_INVALID;
+    same = same && r->h_gdt[g] == dt[g];
+  }
+  if (same) return DGR_OK;
+  for (uint32_t g = 0; g < r->plan->ngroups; ++g) r->h_gdt[g] = dt[g];
+  // stream-ordered before the next step; the host copy stays unchanged
+  // until the next call, which first waits for the evolver's stream
+  DGR_HIP(hipStreamSynchronize(r->st));
+  DGR_HIP(hipMemcpyAsync(r->d_gdt, r->h_gdt.data(), sizeof(double) * r->plan->ngroups, hipMemcpyHostToDevice,
+                         r->st));
+  r->gdt_set = true;
   return DGR_OK;
 }
 
@@ -263,10 +267,10 @@ void destroy(dgr_rk4_s* r) {
   if (r->exec) (void)hipGraphExecDestroy(r->exec);
   if (r->graph) (void)hipGraphDestroy(r->graph);
   if (r->st) (void)hipStreamDestroy(r->st);
-  (void)hipFree(r->buf);
+  if (r->own_buf) (void)hipFree(r->buf);
   (void)hipFree(r->d_tab);
   (void)hipFree(r->d_io);
-  (void)hipFree(r->d_coef);
+  (void)hipFree(r->d_gdt);
   (void)hipFree(r->d_flag);
   delete r;
 }
@@ -309,17 +313,23 @@ extern "C" int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* 
   return flag ? DGR_E_CONSTRAINT : DGR_OK;
 }
 
-extern "C" int dgr_rk4_create(dgr_rk4_t* out, dgr_plan_t plan, double* const* u, int use_graph) {
+extern "C" uint64_t dgr_rk4_work_doubles(dgr_plan_t plan) { return plan ? 3ull * 24ull * plan->dof_end : 0; }
+
+extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double* const* u, double* work,
+                                        int use_graph) {
   if (!out || !plan || check_ptrs(u)) return DGR_E_INVALID;
   dgr_rk4_s* r = new dgr_rk4_s();
   r->plan = plan;
   r->dof = plan->dof_end;
   r->use_graph = use_graph != 0;
+  r->h_gdt.assign(plan->ngroups, 0.0);
   for (int v = 0; v < 24; ++v) r->state.v[v] = u[v];
   const size_t nb = sizeof(double) * r->dof;
-  if (hipMalloc(&r->buf, nb * 24 * 3) != hipSuccess || hipMalloc(&r->d_tab, sizeof(Ptrs24) * 6) != hipSuccess ||
-      hipMalloc(&r->d_io, sizeof(IOPtrs) * 2) != hipSuccess || hipMalloc(&r->d_coef, sizeof(RKCoef)) != hipSuccess ||
-      hipMalloc(&r->d_flag, 256) != hipSuccess) {
+  r->own_buf = work == nullptr;
+  r->buf = work;
+  if ((r->own_buf && hipMalloc(&r->buf, nb * 24 * 3) != hipSuccess) ||
+      hipMalloc(&r->d_tab, sizeof(Ptrs24) * 6) != hipSuccess || hipMalloc(&r->d_io, sizeof(IOPtrs) * 2) != hipSuccess ||
+      hipMalloc(&r->d_gdt, sizeof(double) * plan->ngroups) != hipSuccess || hipMalloc(&r->d_flag, 256) != hipSuccess) {
     destroy(r);
     return DGR_E_NOMEM;
   }
@@ -344,8 +354,8 @@ extern "C" int dgr_rk4_create(dgr_rk4_t* out, dgr_plan_t plan, double* const* u,
   if (e == hipSuccess) e = hipMemcpy(r->d_tab, h, sizeof(h), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(r->d_io, io, sizeof(io), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(r->d_flag, 0, 256);
-  // the stage buffer's padding must hold valid data before the first
-  // refill: start it as a copy of the state
+  // ghosts the refill never reaches (no same-level neighbour) keep the
+  // state's values in the stage buffer
   for (int v = 0; v < 24 && e == hipSuccess; ++v)
     e = hipMemcpy(stg.v[v], r->state.v[v], nb, hipMemcpyDeviceToDevice);
   if (e != hipSuccess) {
@@ -356,10 +366,27 @@ extern "C" int dgr_rk4_create(dgr_rk4_t* out, dgr_plan_t plan, double* const* u,
   return DGR_OK;
 }
 
+extern "C" int dgr_rk4_create(dgr_rk4_t* out, dgr_plan_t plan, double* const* u, int use_graph) {
+  return dgr_rk4_create_with_work(out, plan, u, nullptr, use_graph);
+}
+
+extern "C" int dgr_rk4_set_group_dt(dgr_rk4_t r, const double* dt, uint32_t ngroups) {
+  if (!r || !dt) return DGR_E_INVALID;
+  return set_group_dt(r, dt, ngroups);
+}
+
 extern "C" int dgr_rk4_step(dgr_rk4_t r, double dt, void* stream) { return dgr_rk4_steps(r, dt, 1, stream); }
 
 extern "C" int dgr_rk4_steps(dgr_rk4_t r, double dt, int nsteps, void* stream) {
-  if (!r || nsteps < 0 || !(dt > 0.0)) return DGR_E_INVALID;
+  if (!r || !(dt > 0.0)) return DGR_E_INVALID;
+  std::vector<double> all(r->plan->ngroups, dt);
+  int rc = set_group_dt(r, all.data(), (uint32_t)all.size());
+  if (rc) return rc;
+  return dgr_rk4_run(r, nsteps, stream);
+}
+
+extern "C" int dgr_rk4_run(dgr_rk4_t r, int nsteps, void* stream) {
+  if (!r || nsteps < 0 || !r->gdt_set) return DGR_E_INVALID;
   hipStream_t caller = (hipStream_t)stream;
   // order after the caller's work, run on the evolver's stream (graphs
   // cannot be captured on the legacy default stream), order the caller after
@@ -369,10 +396,10 @@ extern "C" int dgr_rk4_steps(dgr_rk4_t r, double dt, int nsteps, void* stream) {
   if (e == hipSuccess) e = hipStreamWaitEvent(r->st, ev, 0);
   if (e != hipSuccess) {
     (void)hipEventDestroy(ev);
-    return hip_fail(e, "dgr_rk4_steps");
+    return hip_fail(e, "dgr_rk4_run");
   }
-  int rc = set_dt(r, dt, r->st);
-  if (rc == DGR_OK && r->use_graph && !r->exec) {
+  int rc = DGR_OK;
+  if (r->use_graph && !r->exec) {
     // capture one step once; later steps replay it
     e = hipStreamBeginCapture(r->st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) rc = hip_fail(e, "hipStreamBeginCapture");
@@ -397,7 +424,7 @@ extern "C" int dgr_rk4_steps(dgr_rk4_t r, double dt, int nsteps, void* stream) {
   if (rc == DGR_OK) {
     e = hipEventRecord(ev, r->st);
     if (e == hipSuccess) e = hipStreamWaitEvent(caller, ev, 0);
-    if (e != hipSuccess) rc = hip_fail(e, "dgr_rk4_steps");
+    if (e != hipSuccess) rc = hip_fail(e, "dgr_rk4_run");
   }
   (void)hipEventDestroy(ev);
   return rc;

@@ -20,7 +20,21 @@ class DgrError(RuntimeError):
 
 class Block(ctypes.Structure):
     _fields_ = [("offset", ctypes.c_uint64), ("sz", ctypes.c_uint32 * 3), ("bflag", ctypes.c_uint32),
-                ("pmin", ctypes.c_double * 3), ("pmax", ctypes.c_double * 3)]
+                ("pmin", ctypes.c_double * 3), ("pmax", ctypes.c_double * 3), ("group", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+BLOCK_DTYPE = None
+
+
+def _block_dtype():
+    import numpy as np
+    global BLOCK_DTYPE
+    if BLOCK_DTYPE is None:
+        BLOCK_DTYPE = np.dtype([("offset", "<u8"), ("sz", "<u4", 3), ("bflag", "<u4"), ("pmin", "<f8", 3),
+                                ("pmax", "<f8", 3), ("group", "<u4"), ("reserved", "<u4")], align=True)
+        assert BLOCK_DTYPE.itemsize == ctypes.sizeof(Block)
+    return BLOCK_DTYPE
 
 
 class BSSNParams(ctypes.Structure):
@@ -74,6 +88,11 @@ def lib():
         L.dgr_ghost_refill.argtypes = [vp, pp, vp]
         L.dgr_enforce_constraints.argtypes = [vp, pp, vp]
         L.dgr_rk4_create.argtypes = [ctypes.POINTER(vp), vp, pp, ctypes.c_int]
+        L.dgr_rk4_work_doubles.restype = ctypes.c_uint64
+        L.dgr_rk4_work_doubles.argtypes = [vp]
+        L.dgr_rk4_create_with_work.argtypes = [ctypes.POINTER(vp), vp, pp, vp, ctypes.c_int]
+        L.dgr_rk4_set_group_dt.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32]
+        L.dgr_rk4_run.argtypes = [vp, ctypes.c_int, vp]
         L.dgr_rk4_step.argtypes = [vp, ctypes.c_double, vp]
         L.dgr_rk4_steps.argtypes = [vp, ctypes.c_double, ctypes.c_int, vp]
         L.dgr_rk4_status.argtypes = [vp]
@@ -103,15 +122,22 @@ def var_ptrs(t):
 
 
 def blocks_array(blocks):
-    """blocks: list of dicts or a structured description -> ctypes array."""
-    n = len(blocks)
-    arr = (Block * n)()
-    for i, b in enumerate(blocks):
-        arr[i].offset = int(b["offset"])
-        arr[i].sz[:] = [int(s) for s in b["sz"]]
-        arr[i].bflag = int(b["bflag"])
-        arr[i].pmin[:] = [float(x) for x in b["pmin"]]
-        arr[i].pmax[:] = [float(x) for x in b["pmax"]]
+    """Block dicts (offset, sz, bflag, pmin, pmax, optional group) or a
+    numpy array of BLOCK_DTYPE -> ctypes array of dgr_block."""
+    import numpy as np
+    dt = _block_dtype()
+    if isinstance(blocks, np.ndarray) and blocks.dtype == dt:
+        a = np.ascontiguousarray(blocks)
+    else:
+        n = len(blocks)
+        a = np.zeros(n, dtype=dt)
+        a["offset"] = [int(b["offset"]) for b in blocks]
+        a["sz"] = [b["sz"] for b in blocks]
+        a["bflag"] = [int(b["bflag"]) for b in blocks]
+        a["pmin"] = [b["pmin"] for b in blocks]
+        a["pmax"] = [b["pmax"] for b in blocks]
+        a["group"] = [int(b.get("group", 0)) for b in blocks]
+    arr = (Block * len(a)).from_buffer_copy(a.tobytes())
     return arr
 
 
@@ -185,18 +211,35 @@ def rhs_blocks(rhs, u, blocks, params, stream=None):
 
 
 class RK4:
-    """Device-resident RK4 evolver of the (24, dof) state `u` (dgr_rk4_*)."""
+    """Device-resident RK4 evolver of the (24, dof) state `u` (dgr_rk4_*).
+    `work`: optional caller-owned float64 device tensor of at least
+    Plan-dependent dgr_rk4_work_doubles elements."""
 
-    def __init__(self, plan, u, use_graph=True):
+    def __init__(self, plan, u, use_graph=True, work=None):
         self.plan = plan
         self.u = u
         up, _u = ptr_array(var_ptrs(u))
         h = ctypes.c_void_p()
-        check(lib().dgr_rk4_create(ctypes.byref(h), plan.handle, up, 1 if use_graph else 0), "dgr_rk4_create")
+        L = lib()
+        if work is None:
+            check(L.dgr_rk4_create(ctypes.byref(h), plan.handle, up, 1 if use_graph else 0), "dgr_rk4_create")
+        else:
+            need = int(L.dgr_rk4_work_doubles(plan.handle))
+            if work.numel() < need:
+                raise DgrError("work buffer too small: %d < %d doubles" % (work.numel(), need))
+            check(L.dgr_rk4_create_with_work(ctypes.byref(h), plan.handle, up, ctypes.c_void_p(work.data_ptr()),
+                                             1 if use_graph else 0), "dgr_rk4_create_with_work")
         self.handle = h
 
     def step(self, dt, nsteps=1, stream=None):
         check(lib().dgr_rk4_steps(self.handle, float(dt), int(nsteps), _stream_ptr(stream)), "dgr_rk4_steps")
+
+    def set_group_dt(self, dts):
+        arr = (ctypes.c_double * len(dts))(*[float(x) for x in dts])
+        check(lib().dgr_rk4_set_group_dt(self.handle, arr, len(dts)), "dgr_rk4_set_group_dt")
+
+    def run(self, nsteps=1, stream=None):
+        check(lib().dgr_rk4_run(self.handle, int(nsteps), _stream_ptr(stream)), "dgr_rk4_run")
 
     def status_ok(self):
         rc = lib().dgr_rk4_status(self.handle)

@@ -48,6 +48,10 @@ typedef struct dgr_block {
   uint32_t bflag;
   double pmin[3];
   double pmax[3];
+  uint32_t group;    /* independent block sets (e.g. one per served job): ghost
+                        refill only pairs blocks of the same group, and RK4
+                        time steps are set per group; 0 for a single mesh */
+  uint32_t reserved;
 } dgr_block;
 
 /* BSSN parameters, passed by value instead of the reference's bssn::
@@ -128,6 +132,14 @@ int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* stream);
  * replayed.  Work buffers: 3 x 24 x dof doubles. */
 typedef struct dgr_rk4_s* dgr_rk4_t;
 int dgr_rk4_create(dgr_rk4_t* rk, dgr_plan_t plan, double* const* u, int use_graph);
+/* Same, with caller-owned work memory of at least dgr_rk4_work_doubles(plan)
+ * doubles (no allocation inside). */
+uint64_t dgr_rk4_work_doubles(dgr_plan_t plan);
+int dgr_rk4_create_with_work(dgr_rk4_t* rk, dgr_plan_t plan, double* const* u, double* work, int use_graph);
+/* Per-group time steps dt[g], g < ngroups (every group in the plan must be
+ * covered); dgr_rk4_step(s) set the same dt for all groups. */
+int dgr_rk4_set_group_dt(dgr_rk4_t rk, const double* dt, uint32_t ngroups);
+int dgr_rk4_run(dgr_rk4_t rk, int nsteps, void* stream);
 int dgr_rk4_step(dgr_rk4_t rk, double dt, void* stream);
 int dgr_rk4_steps(dgr_rk4_t rk, double dt, int nsteps, void* stream);
 /* Waits for the evolver and returns DGR_E_CONSTRAINT if any projection
