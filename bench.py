@@ -1,22 +1,33 @@
-"""Benchmark: FP64 BSSN RHS interior points/s on one MI355X.
+"""Benchmark: FP64 BSSN RHS points/s on one MI355X.
 
-Workload (one "step" = one BSSN right-hand-side evaluation over every block,
+Headline workload (one "step" = one full RK4 step, replayed as one hipGraph,
 inputs resident in HBM): 4096 blocks of 13^3 points (one order-6 element
 each, 16x16x16 blocks tiling a cube, outer layer with boundary flags),
-gauge-wave data (A = 0.01) plus 1e-6 seeded white noise on all 24
-variables, KO sigma 0.4, eta 2, 1+log lapse, Gamma-driver shift.
+gauge-wave data (A = 0.01) plus 1e-6 seeded white noise on all 24 variables,
+CFL 0.25, KO sigma 0.4, eta 2, 1+log lapse, Gamma-driver shift.  A step is
+four RHS evaluations, four stage combines with the constraint projection and
+four same-level ghost refills; `value` counts RHS interior-point evaluations
+per second (4 x interior points per step).
 
-Reported: batched single-launch path (`value`), the plain per-block launch
-path for comparison, the roofline of the RHS kernel (algorithmic bytes:
-8*24*(sz^3 + (sz-6)^3) per block, i.e. the padded input read once and the
-interior RHS written once), and the CPU oracle timed on a bounded sample on
-the host cores.
+Also reported on the same line:
+  rhs            the batched single-launch RHS kernel alone (HIP events on
+                 its stream), the basis of `roofline` (algorithmic bytes
+                 8*24*(sz^3 + (sz-6)^3) per block: padded input read once,
+                 interior RHS written once);
+  per_block      the plain one-launch-per-block RHS path;
+  uneven         3000 blocks of padded sizes 13/19/25/37 (60/25/10/5 %,
+                 seeded), flat space plus Gaussian bumps: RHS and RK4 step;
+  serving        a bounded Poisson stream (20 jobs/s) of 50-step evolutions
+                 of 64..4096 blocks under a 200 GB HBM budget: p50/p99 job
+                 latency and aggregate RHS points/s;
+  cpu_baseline   the CPU oracle on a bounded sample of the headline blocks.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -26,11 +37,23 @@ sys.path.insert(0, os.path.join(ROOT, "dendro-5.01_amd"))
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
 REF_FLOP_PER_POINT = 6490   # reference deriv42 bssnrhs, bflag 0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bssn_rhs_uniform4096_traffic.txt")
 
 
 def algorithmic_bytes(blocks):
     return sum(8 * 24 * (b["sz"][0] * b["sz"][1] * b["sz"][2] + (b["sz"][0] - 6) * (b["sz"][1] - 6) * (b["sz"][2] - 6))
                for b in blocks)
+
+
+def measured_traffic():
+    """HBM bytes per RHS launch on the headline workload, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (null if absent)."""
+    try:
+        txt = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, re.M)
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(blocks, U_cpu, seconds=8.0, nblk=512):
@@ -67,14 +90,105 @@ def cpu_baseline(blocks, U_cpu, seconds=8.0, nblk=512):
                       "-O3 -march=native -ffp-contract=fast" % (nblk, pts, reps, el)}
 
 
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def time_rhs(plan, R, U, stream, steps, warm, per_block=False):
+    """Mean duration (ms) of one RHS launch (or per-block launch sequence),
+    HIP events recorded on the launch stream."""
+    import torch
+    for _ in range(warm):
+        plan.rhs(R, U, stream, per_block=per_block)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        plan.rhs(R, U, stream, per_block=per_block)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def time_rk4(rk, stream, steps, warm):
+    """(ms per RK4 step from HIP events, wall seconds of the timed region)."""
+    import torch
+    rk.run(warm, stream)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    rk.run(steps, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if not rk.status_ok():
+        raise RuntimeError("constraint projection failed during the timed RK4 steps")
+    return e0.elapsed_time(e1) / steps, wall
+
+
+def uneven(stream, steps):
+    import torch
+    from pydgr import RK4, Plan, data, layout, make_params
+    blocks, dof = layout.mixed_blocks(3000, seed=0)
+    pts = layout.interior_points(blocks)
+    dev = torch.device("cuda:0")
+    U = data.flat_bumps(blocks, dof, seed=0, amp=1e-3, width=0.1, device=dev)
+    R = torch.zeros_like(U)
+    plan = Plan(blocks, make_params(ko_sigma=0.4, eta_const=2.0))
+    ms = time_rhs(plan, R, U, stream, steps, 2)
+    ms_pb = time_rhs(plan, R, U, stream, 1, 0, per_block=True)
+    nbytes = algorithmic_bytes(blocks)
+    del R
+    rk = RK4(plan, U, use_graph=True)
+    rk.step(0.25 / 64, 0)  # sets dt for every block
+    ms_rk, _ = time_rk4(rk, stream, max(2, steps // 4), 1)
+    rk.close()
+    plan.close()
+    sizes = {}
+    for b in blocks:
+        sizes[b["sz"][0]] = sizes.get(b["sz"][0], 0) + 1
+    out = {"workload": "3000 blocks, padded sizes %s (seed 0), flat space + 8 Gaussian bumps" % sizes,
+           "interior_points": pts,
+           "rhs_ms": ms, "rhs_points_per_s": pts / (ms * 1e-3),
+           "rhs_hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "rhs_fp64_frac": REF_FLOP_PER_POINT * pts / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+           "per_block_ms": ms_pb, "batched_speedup": ms_pb / ms,
+           "rk4_ms_per_step": ms_rk, "rk4_rhs_points_per_s": 4 * pts / (ms_rk * 1e-3)}
+    del U
+    torch.cuda.empty_cache()
+    return out
+
+
+def serving(njobs, budget_gb, max_points):
+    import torch
+    from pydgr.serving import Server, poisson_jobs
+    jobs = poisson_jobs(njobs, rate=20.0, seed=0, min_blocks=64, max_blocks=4096, steps=50)
+    srv = Server(int(budget_gb * 1e9), device="cuda:0", keep_results=False, max_resident_points=max_points)
+    stats = srv.run(jobs, realtime=True)
+    del srv
+    torch.cuda.empty_cache()
+    stats["workload"] = ("%d jobs, Poisson arrivals 20/s (seed 0), 64..4096 blocks of 13^3, 50 RK4 steps each, "
+                         "perturbed flat space or Brill-Lindquist (mass ratio 1..3), HBM budget %g GB, at most %s "
+                         "resident interior points" % (njobs, budget_gb, max_points))
+    return stats
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nb", type=int, default=16, help="blocks per axis (16 -> 4096 blocks)")
-    ap.add_argument("--per-block-steps", type=int, default=3)
+    ap.add_argument("--per-block-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--serving-jobs", type=int, default=24, help="0 skips the serving run")
+    ap.add_argument("--serving-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serving-max-points", type=int, default=3_000_000,
+                    help="cap on resident interior points (0: memory budget only)")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -83,43 +197,40 @@ def main():
         sys.exit(2)
 
     import torch
-    from pydgr import Plan, data, layout, make_params
+    from pydgr import RK4, Plan, data, layout, make_params
 
     dev = torch.device("cuda:0")
-    blocks, dof = layout.uniform_grid(a.nb, ele_order=6, dx=1.0 / 64, outer_bflags=True)
+    dx = 1.0 / 64
+    blocks, dof = layout.uniform_grid(a.nb, ele_order=6, dx=dx, outer_bflags=True)
     pts = layout.interior_points(blocks)
     U = data.gauge_wave(blocks, dof, amp=0.01, noise=1e-6, seed=0, device=dev)
     R = torch.zeros_like(U)
     plan = Plan(blocks, make_params(ko_sigma=0.4, eta_const=2.0))
     stream = torch.cuda.current_stream()
-    print("bench: %d blocks 13^3, %d interior points, %.2f GB state" % (len(blocks), pts, U.numel() * 8 / 1e9),
-          file=sys.stderr, flush=True)
+    log("%d blocks 13^3, %d interior points, %.2f GB state" % (len(blocks), pts, U.numel() * 8 / 1e9))
 
-    def timed(per_block, steps, warm):
-        for _ in range(warm):
-            plan.rhs(R, U, stream, per_block=per_block)
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        e0.record(stream)
-        for _ in range(steps):
-            plan.rhs(R, U, stream, per_block=per_block)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        return e0.elapsed_time(e1) / steps, wall
-
-    ms, wall = timed(False, a.steps, a.warmup)
-    print("bench: batched %.4f ms/step" % ms, file=sys.stderr, flush=True)
+    rhs_ms = time_rhs(plan, R, U, stream, a.steps, a.warmup)
+    log("batched RHS %.4f ms" % rhs_ms)
     ms_pb = None
     if a.per_block_steps > 0:
-        ms_pb, _ = timed(True, a.per_block_steps, 1)
-        print("bench: per-block %.4f ms/step" % ms_pb, file=sys.stderr, flush=True)
+        ms_pb = time_rhs(plan, R, U, stream, a.per_block_steps, 1, per_block=True)
+        log("per-block RHS %.4f ms" % ms_pb)
+    U_cpu = U.cpu() if not a.no_cpu else None
+    del R
+    rk = RK4(plan, U, use_graph=True)
+    rk.step(0.25 * dx, 0)  # sets dt for every block
+    ms, wall = time_rk4(rk, stream, a.steps, a.warmup)
+    log("RK4 step (hipGraph) %.4f ms" % ms)
+    rk.close()
+    plan.close()
+    del U
+    torch.cuda.empty_cache()
 
-    value = pts / (ms * 1e-3)
+    value = 4 * pts / (ms * 1e-3)
+    rhs_pps = pts / (rhs_ms * 1e-3)
     nbytes = algorithmic_bytes(blocks)
-    achieved = nbytes / (ms * 1e-3) / 1e9
+    achieved = nbytes / (rhs_ms * 1e-3) / 1e9
+    traffic = measured_traffic()
     line = {
         "metric": "FP64 BSSN RHS points/s: batched vs per-block; p50/p99 evolve-job latency",
         "value": value,
@@ -133,22 +244,30 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded gauge wave + 1e-6 white noise on padded blocks)",
-        "config": {"workload": "uniform grid: %d blocks of 13^3 (one order-6 element each), gauge wave, one BSSN "
-                               "RHS evaluation per step" % len(blocks),
+        "config": {"workload": "uniform grid: %d blocks of 13^3 (one order-6 element each), gauge wave, one full RK4 "
+                               "step per step (hipGraph): 4 RHS + 4 stage combines + 4 ghost refills" % len(blocks),
                    "blocks": len(blocks), "block_size": 13, "interior_points": pts, "ko_sigma": 0.4,
-                   "deriv": "deriv42"},
+                   "cfl": 0.25, "dx": dx, "deriv": "deriv42"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_bssn_rhs", "bytes_per_launch": nbytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_bssn_rhs", "kernel_ms": rhs_ms, "bytes_per_launch": nbytes},
+        "rhs": {"ms": rhs_ms, "points_per_s": rhs_pps},
         "fp64": {"reference_flop_per_point": REF_FLOP_PER_POINT,
-                 "achieved_tflops": REF_FLOP_PER_POINT * value / 1e12,
-                 "frac_of_peak": REF_FLOP_PER_POINT * value / 1e12 / FP64_PEAK_TFLOPS},
-        "per_block": None if ms_pb is None else {"ms_per_step": ms_pb, "points_per_s": pts / (ms_pb * 1e-3),
-                                                 "batched_speedup": ms_pb / ms},
+                 "achieved_tflops": REF_FLOP_PER_POINT * rhs_pps / 1e12,
+                 "frac_of_peak": REF_FLOP_PER_POINT * rhs_pps / 1e12 / FP64_PEAK_TFLOPS},
+        "per_block": None if ms_pb is None else {"ms": ms_pb, "points_per_s": pts / (ms_pb * 1e-3),
+                                                 "batched_speedup": ms_pb / rhs_ms,
+                                                 "rk4_graph_speedup_per_rhs": ms_pb / (ms / 4)},
         "wall_s_timed_region": wall,
     }
+    if not a.no_uneven:
+        line["uneven"] = uneven(stream, a.steps)
+        log("uneven: RHS %.4f ms, RK4 %.4f ms" % (line["uneven"]["rhs_ms"], line["uneven"]["rk4_ms_per_step"]))
+    if a.serving_jobs > 0:
+        line["serving"] = serving(a.serving_jobs, a.serving_budget_gb, a.serving_max_points or None)
+        log("serving: p50 %.3f s p99 %.3f s" % (line["serving"]["p50_latency_s"], line["serving"]["p99_latency_s"]))
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(blocks, U.cpu(), seconds=a.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(blocks, U_cpu, seconds=a.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

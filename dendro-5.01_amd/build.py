@@ -52,6 +52,8 @@ def build_test_harness():
     nat = os.path.join(ROOT, "tests", "native")
     run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-fPIC", "-shared", "-o",
          os.path.join(nat, "libstaged_host.so"), os.path.join(nat, "staged_host.cpp")])
+    run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-fPIC", "-shared", "-o",
+         os.path.join(nat, "libforms_host.so"), os.path.join(nat, "forms_host.cpp")])
     # host-shim driver: links the product library and the oracle
     run([HIPCC, "-O2", "-std=c++17", "-o", os.path.join(nat, "shim_test"), os.path.join(nat, "shim_test.cpp"),
          "-L" + LIB, "-ldgr_bssn", "-L" + os.path.join(ROOT, "oracle"), "-loracle",

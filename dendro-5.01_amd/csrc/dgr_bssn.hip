@@ -39,6 +39,7 @@
 
 #include "../../include/dgr_bssn.h"
 #include "bssn_staged.h"
+#include "stencil_forms.h"
 #include "dgr_internal.h"
 #include "tile_map.h"
 
@@ -59,84 +60,50 @@ constexpr int NGY = TX * TY * GXZ;             // 539
 // ---------------------------------------------------------------------------
 // Stencils.  The value at axis coordinate q' of the current line is
 // A[o + q'*s]: `s` is the stride along the axis, q the coordinate of the
-// output point, e = sz-3, lf/hf the face flags of the axis.  Formulas and operation
-// order follow BSSN_GR/src/derivs.cpp (deriv42_x :29-65, deriv42_xx
-// :265-316, deriv42adv_x :513-622, ko_deriv42_x :937-1060).
+// output point.  Interior formulas and operation order follow
+// BSSN_GR/src/derivs.cpp (deriv42_x :29-65, deriv42_xx :265-316,
+// deriv42adv_x :513-622, ko_deriv42_x :937-1060).  In boundary tiles the
+// interior formula is evaluated everywhere and the one-sided closures of
+// points within three of a flagged face replace it from the coefficient
+// rows of stencil_forms.h (one code path instead of a branch per case; the
+// rows reproduce the closures bit for bit).
 // ---------------------------------------------------------------------------
-template <bool BND>
-__device__ __forceinline__ double d1_42(const double* A, int o, int s, int q, int e, bool lf, bool hf,
-                                        double ih2, double ih12) {
-#pragma clang fp contract(off)
-  if (BND && hf && q == e - 1) return (A[o + (e - 3) * s] - 4.0 * A[o + (e - 2) * s] + 3.0 * A[o + (e - 1) * s]) * ih2;
-  if (BND && hf && q == e - 2) return (-A[o + (e - 3) * s] + A[o + (e - 1) * s]) * ih2;
-  if (BND && lf && q == 4) return (-A[o + 3 * s] + A[o + 5 * s]) * ih2;
-  if (BND && lf && q == 3) return (-3.0 * A[o + 3 * s] + 4.0 * A[o + 4 * s] - A[o + 5 * s]) * ih2;
-  return (A[o + (q - 2) * s] - 8.0 * A[o + (q - 1) * s] + 8.0 * A[o + (q + 1) * s] - A[o + (q + 2) * s]) * ih12;
+__constant__ dgr_forms::Form c_forms[dgr_forms::F_NFORMS] = DGR_FORMS_TABLE;
+
+// The variable arrays are global memory; saying so turns generic (flat)
+// accesses into global ones, which do not hold up waits on LDS reads.
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ const gdouble* gmem(const double* p) { return (const gdouble*)p; }
+__device__ __forceinline__ gdouble* gmem(double* p) { return (gdouble*)p; }
+__shared__ dgr_forms::Form s_forms[dgr_forms::F_NFORMS];  // boundary kernel only
+
+// Closure row f over the 7-point window centred at A[c], scaled.  Only
+// points with a nonzero coefficient are read (a closure never reaches past
+// its face, where a derivative plane may end).
+__device__ __forceinline__ double eval_form(int f, const double* A, int c, int s, const AxisC& x) {
+  const dgr_forms::Form& F = s_forms[f];
+  double w[7];
+#pragma unroll
+  for (int m = 0; m < 7; ++m) w[m] = F.c[m] != 0.0 ? A[c + (m - 3) * s] : 0.0;
+  const double sum = F.desc ? dgr_forms::form_sum_down(F.c, w) : dgr_forms::form_sum_up(F.c, w);
+  return dgr_forms::form_scale(sum, F.sel, &x.ih2);
+}
+
+// Is q within reach of a flagged face's closures (three points)?
+__device__ __forceinline__ bool near_face(int q, const AxisC& x) {
+  return (x.lf && q <= 5) || (x.hf && q >= x.e - 3);
 }
 
 template <bool BND>
-__device__ __forceinline__ double d2_42(const double* A, int o, int s, int q, int e, bool lf, bool hf,
-                                        double isq, double isq12) {
+__device__ __forceinline__ double d1_42(const double* A, int o, int s, int q, const AxisC& x) {
 #pragma clang fp contract(off)
-  if (BND && hf && q == e - 1)
-    return (-A[o + (e - 4) * s] + 4.0 * A[o + (e - 3) * s] - 5.0 * A[o + (e - 2) * s] + 2.0 * A[o + (e - 1) * s]) * isq;
-  if (BND && hf && q == e - 2) return (A[o + (e - 3) * s] - 2.0 * A[o + (e - 2) * s] + A[o + (e - 1) * s]) * isq;
-  if (BND && lf && q == 4) return (A[o + 3 * s] - 2.0 * A[o + 4 * s] + A[o + 5 * s]) * isq;
-  if (BND && lf && q == 3) return (2.0 * A[o + 3 * s] - 5.0 * A[o + 4 * s] + 4.0 * A[o + 5 * s] - A[o + 6 * s]) * isq;
-  return (-A[o + (q - 2) * s] + 16.0 * A[o + (q - 1) * s] - 30.0 * A[o + q * s] + 16.0 * A[o + (q + 1) * s] - A[o + (q + 2) * s]) * isq12;
-}
-
-template <bool BND>
-__device__ __forceinline__ double adv_42(const double* A, int o, int s, int q, int e, bool lf, bool hf, double b,
-                                         double ih2, double ih12) {
-#pragma clang fp contract(off)
-  const bool pos = b > 0.0;
-  if (BND && hf && q == e - 1) return (A[o + (e - 3) * s] - 4.0 * A[o + (e - 2) * s] + 3.0 * A[o + (e - 1) * s]) * ih2;
-  if (BND && hf && q == e - 2) {
-    if (pos) return (-A[o + (e - 3) * s] + A[o + (e - 1) * s]) * ih2;
-    return (A[o + (e - 4) * s] - 4.0 * A[o + (e - 3) * s] + 3.0 * A[o + (e - 2) * s]) * ih2;
+  const int c = o + q * s;
+  double r = (A[c - 2 * s] - 8.0 * A[c - s] + 8.0 * A[c + s] - A[c + 2 * s]) * x.ih12;
+  if (BND && near_face(q, x)) {
+    const int f = dgr_forms::form_d1(q, x.e, x.lf, x.hf);
+    if (f != dgr_forms::F_D1) r = eval_form(f, A, c, s, x);
   }
-  if (BND && hf && q == e - 3) {
-    if (pos) return (-3.0 * A[o + (e - 3) * s] + 4.0 * A[o + (e - 2) * s] - A[o + (e - 1) * s]) * ih2;
-    return (-A[o + (e - 6) * s] + 6.0 * A[o + (e - 5) * s] - 18.0 * A[o + (e - 4) * s] + 10.0 * A[o + (e - 3) * s] +
-            3.0 * A[o + (e - 2) * s]) * ih12;
-  }
-  if (BND && lf && q == 3) return (-3.0 * A[o + 3 * s] + 4.0 * A[o + 4 * s] - A[o + 5 * s]) * ih2;
-  if (BND && lf && q == 4) {
-    if (pos) return (-3.0 * A[o + 4 * s] + 4.0 * A[o + 5 * s] - A[o + 6 * s]) * ih2;
-    return (-A[o + 3 * s] + A[o + 5 * s]) * ih2;
-  }
-  if (BND && lf && q == 5) {
-    if (pos) return (-3.0 * A[o + 4 * s] - 10.0 * A[o + 5 * s] + 18.0 * A[o + 6 * s] - 6.0 * A[o + 7 * s] + A[o + 8 * s]) * ih12;
-    return (A[o + 3 * s] - 4.0 * A[o + 4 * s] + 3.0 * A[o + 5 * s]) * ih2;
-  }
-  if (pos)
-    return (-3.0 * A[o + (q - 1) * s] - 10.0 * A[o + q * s] + 18.0 * A[o + (q + 1) * s] - 6.0 * A[o + (q + 2) * s] +
-            A[o + (q + 3) * s]) * ih12;
-  return (-A[o + (q - 3) * s] + 6.0 * A[o + (q - 2) * s] - 18.0 * A[o + (q - 1) * s] + 10.0 * A[o + q * s] +
-          3.0 * A[o + (q + 1) * s]) * ih12;
-}
-
-template <bool BND>
-__device__ __forceinline__ double ko_42(const double* A, int o, int s, int q, int e, bool lf, bool hf, double pre,
-                                        double smr1, double smr2, double smr3) {
-#pragma clang fp contract(off)
-  if (BND && hf && q == e - 1)
-    return (A[o + (e - 4) * s] - 3.0 * A[o + (e - 3) * s] + 3.0 * A[o + (e - 2) * s] - A[o + (e - 1) * s]) / smr3;
-  if (BND && hf && q == e - 2)
-    return (A[o + (e - 5) * s] - 6.0 * A[o + (e - 4) * s] + 12.0 * A[o + (e - 3) * s] - 10.0 * A[o + (e - 2) * s] +
-            3.0 * A[o + (e - 1) * s]) / smr2;
-  if (BND && hf && q == e - 3)
-    return (A[o + (e - 6) * s] - 6.0 * A[o + (e - 5) * s] + 15.0 * A[o + (e - 4) * s] - 19.0 * A[o + (e - 3) * s] +
-            12.0 * A[o + (e - 2) * s] - 3.0 * A[o + (e - 1) * s]) / smr1;
-  if (BND && lf && q == 3) return (A[o + 6 * s] - 3.0 * A[o + 5 * s] + 3.0 * A[o + 4 * s] - A[o + 3 * s]) / smr3;
-  if (BND && lf && q == 4)
-    return (A[o + 7 * s] - 6.0 * A[o + 6 * s] + 12.0 * A[o + 5 * s] - 10.0 * A[o + 4 * s] + 3.0 * A[o + 3 * s]) / smr2;
-  if (BND && lf && q == 5)
-    return (A[o + 8 * s] - 6.0 * A[o + 7 * s] + 15.0 * A[o + 6 * s] - 19.0 * A[o + 5 * s] + 12.0 * A[o + 4 * s] - 3.0 * A[o + 3 * s]) /
-           smr1;
-  return pre * (-A[o + (q - 3) * s] + 6.0 * A[o + (q - 2) * s] - 15.0 * A[o + (q - 1) * s] + 20.0 * A[o + q * s] -
-                15.0 * A[o + (q + 1) * s] + 6.0 * A[o + (q + 2) * s] - A[o + (q + 3) * s]);
+  return r;
 }
 
 // (falloff, asymptotic value) of the Sommerfeld terms per variable
@@ -185,7 +152,7 @@ __device__ __forceinline__ void prefetch(double (&r)[NPF], const double* __restr
     const int b = (e / SX) % SY;
     const int c = e / (SX * SY);
     if (e < NSU && a < T.rx && b < T.ry && c < T.rz)
-      r[m] = __builtin_nontemporal_load(base + a + (long)T.sz0 * (b + (long)T.sz1 * c));
+      r[m] = __builtin_nontemporal_load(gmem(base) + a + (long)T.sz0 * (b + (long)T.sz1 * c));
   }
 }
 
@@ -212,15 +179,15 @@ struct Ctx {
 __device__ __forceinline__ void emit(const IOPtrs* __restrict__ io, const Ctx& C, int v, double eq, double ex) {
 #pragma clang fp contract(off)
   const double r = C.on_face ? ex : (eq + ex);
-  __builtin_nontemporal_store(r, io->r.v[v] + C.o);
+  __builtin_nontemporal_store(r, gmem(io->r.v[v]) + C.o);
 }
 // Outputs that become final several variables later park their dissipation
 // (+ Sommerfeld) term in the output array and pick it up at emission.
 __device__ __forceinline__ void park(const IOPtrs* __restrict__ io, const Ctx& C, int v, double ex) {
-  io->r.v[v][C.o] = ex;
+  gmem(io->r.v[v])[C.o] = ex;
 }
 __device__ __forceinline__ void emit_parked(const IOPtrs* __restrict__ io, const Ctx& C, int v, double eq) {
-  emit(io, C, v, eq, io->r.v[v][C.o]);
+  emit(io, C, v, eq, gmem(io->r.v[v])[C.o]);
 }
 
 // Per-thread LDS slots: element i of a thread's array sits at p[i * NPT]
@@ -248,22 +215,39 @@ template <bool BND, bool D1, bool D2>
 __device__ __forceinline__ void axis_ops(const double* A, int o, int s, int q, const AxisC& x, double beta,
                                          double& d1, double& d2, double& ad, double& ko) {
 #pragma clang fp contract(off)
+  const int c = o + q * s;
+  const double m3 = A[c - 3 * s], m2 = A[c - 2 * s], m1 = A[c - s], z0 = A[c], p1 = A[c + s], p2 = A[c + 2 * s],
+               p3 = A[c + 3 * s];
+  if constexpr (D1) d1 = (m2 - 8.0 * m1 + 8.0 * p1 - p2) * x.ih12;
+  if constexpr (D2) d2 = (-m2 + 16.0 * m1 - 30.0 * z0 + 16.0 * p1 - p2) * x.isq12;
+  if (beta > 0.0)
+    ad = (-3.0 * m1 - 10.0 * z0 + 18.0 * p1 - 6.0 * p2 + p3) * x.ih12;
+  else
+    ad = (-m3 + 6.0 * m2 - 18.0 * m1 + 10.0 * z0 + 3.0 * p1) * x.ih12;
+  ko = x.pre * (-m3 + 6.0 * m2 - 15.0 * m1 + 20.0 * z0 - 15.0 * p1 + 6.0 * p2 - p3);
   if constexpr (BND) {
-    if constexpr (D1) d1 = d1_42<true>(A, o, s, q, x.e, x.lf, x.hf, x.ih2, x.ih12);
-    if constexpr (D2) d2 = d2_42<true>(A, o, s, q, x.e, x.lf, x.hf, x.isq, x.isq12);
-    ad = adv_42<true>(A, o, s, q, x.e, x.lf, x.hf, beta, x.ih2, x.ih12);
-    ko = ko_42<true>(A, o, s, q, x.e, x.lf, x.hf, x.pre, x.smr1, x.smr2, x.smr3);
-  } else {
-    const int c = o + q * s;
-    const double m3 = A[c - 3 * s], m2 = A[c - 2 * s], m1 = A[c - s], z0 = A[c], p1 = A[c + s], p2 = A[c + 2 * s],
-                 p3 = A[c + 3 * s];
-    if constexpr (D1) d1 = (m2 - 8.0 * m1 + 8.0 * p1 - p2) * x.ih12;
-    if constexpr (D2) d2 = (-m2 + 16.0 * m1 - 30.0 * z0 + 16.0 * p1 - p2) * x.isq12;
-    if (beta > 0.0)
-      ad = (-3.0 * m1 - 10.0 * z0 + 18.0 * p1 - 6.0 * p2 + p3) * x.ih12;
-    else
-      ad = (-m3 + 6.0 * m2 - 18.0 * m1 + 10.0 * z0 + 3.0 * p1) * x.ih12;
-    ko = x.pre * (-m3 + 6.0 * m2 - 15.0 * m1 + 20.0 * z0 - 15.0 * p1 + 6.0 * p2 - p3);
+    // closures: one generic row evaluation per operator whose form differs
+    // from the interior one at this point
+    if (near_face(q, x)) {
+      using namespace dgr_forms;
+      const double w[7] = {m3, m2, m1, z0, p1, p2, p3};
+#pragma unroll 1
+      for (int op = D1 ? 0 : (D2 ? 1 : 2); op < 4; ++op) {
+        if (op == 1 && !D2) continue;
+        const int f = op == 0 ? form_d1(q, x.e, x.lf, x.hf)
+                    : op == 1 ? form_d2(q, x.e, x.lf, x.hf)
+                    : op == 2 ? form_adv(q, x.e, x.lf, x.hf, beta > 0.0)
+                              : form_ko(q, x.e, x.lf, x.hf);
+        if (f == F_D1 || f == F_D2 || f == F_ADV_P || f == F_ADV_N || f == F_KO) continue;
+        const Form& F = s_forms[f];
+        const double sum = F.desc ? form_sum_down(F.c, w) : form_sum_up(F.c, w);
+        const double r = form_scale(sum, F.sel, &x.ih2);
+        if (op == 0) d1 = r;
+        else if (op == 1) d2 = r;
+        else if (op == 2) ad = r;
+        else ko = r;
+      }
+    }
   }
 }
 
@@ -299,11 +283,11 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
     // sgx[a + TX*(b + GXY*cc)] holds d_x u at block coords (i0+a, j0-2+b, k0-2+cc);
     // sgy[a + TX*(b + TY*cc)] holds d_y u at (i0+a, j0+b, k0-2+cc).
     const int oxy = ti + TX * ((2 - T.j0) + GXY * (tk + 2));
-    d2[1] = d1_42<BND>(sgx, oxy, TX, gj, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
+    d2[1] = d1_42<BND>(sgx, oxy, TX, gj, ax[1]);
     const int oxz = ti + TX * ((tj + 2) + GXY * (2 - T.k0));
-    d2[2] = d1_42<BND>(sgx, oxz, TX * GXY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
+    d2[2] = d1_42<BND>(sgx, oxz, TX * GXY, gk, ax[2]);
     const int oyz = ti + TX * (tj + TY * (2 - T.k0));
-    d2[4] = d1_42<BND>(sgy, oyz, TX * TY, gk, ax[2].e, ax[2].lf, ax[2].hf, ax[2].ih2, ax[2].ih12);
+    d2[4] = d1_42<BND>(sgy, oyz, TX * TY, gk, ax[2]);
     __builtin_amdgcn_sched_barrier(0);
   }
   // upwinded derivatives contracted with beta, summed in the order of the
@@ -339,7 +323,7 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
     park(io, C, V, ex);
   }
   if constexpr (V >= V_BETA0 && V <= V_BETA2) {
-    s.B[V - V_BETA0] = io->u.v[V_B0 + (V - V_BETA0)][C.o];  // B^k only enters b_rhs^k and B_rhs^k
+    s.B[V - V_BETA0] = gmem(io->u.v[V_B0 + (V - V_BETA0)])[C.o];  // B^k only enters b_rhs^k and B_rhs^k
     double gto[6], chio = 0.0;
     emit(io, C, V, s.add_beta(C.P, V - V_BETA0, d1, d2, adv, gto, &chio), ex);
     if constexpr (V == V_BETA2) {
@@ -372,7 +356,7 @@ __device__ __forceinline__ void deriv_planes(const double* su, double* sgx, doub
     const int cc = e / (TX * GXY);
     if (a < T.nx && b < T.ny + 4 && cc < T.nz + 4) {
       const int o = (a + 3) + SX * ((b + 1) + SY * (cc + 1)) - (T.i0 + a);
-      sgx[e] = d1_42<BND>(su, o, 1, T.i0 + a, ax[0].e, ax[0].lf, ax[0].hf, ax[0].ih2, ax[0].ih12);
+      sgx[e] = d1_42<BND>(su, o, 1, T.i0 + a, ax[0]);
     }
   }
   // sgy: d_y u at block coords (i0+a, j0+b, k0-2+cc), su coords (a+3, b+3, cc+1)
@@ -382,7 +366,7 @@ __device__ __forceinline__ void deriv_planes(const double* su, double* sgx, doub
     const int cc = e / (TX * TY);
     if (a < T.nx && b < T.ny && cc < T.nz + 4) {
       const int o = (a + 3) + SX * ((b + 3) + SY * (cc + 1)) - (T.j0 + b) * SX;
-      sgy[e] = d1_42<BND>(su, o, SX, T.j0 + b, ax[1].e, ax[1].lf, ax[1].hf, ax[1].ih2, ax[1].ih12);
+      sgy[e] = d1_42<BND>(su, o, SX, T.j0 + b, ax[1]);
     }
   }
 }
@@ -419,6 +403,13 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
   __shared__ AxisC s_ax[3];         // stencil constants of the current block
 
   const int t = threadIdx.x;
+  if constexpr (BND) {
+    // closure rows for the boundary stencils (visible after the first
+    // barrier of the tile loop)
+    constexpr int nd = dgr_forms::F_NFORMS * (int)(sizeof(dgr_forms::Form) / sizeof(double));
+    static_assert(sizeof(dgr_forms::Form) % sizeof(double) == 0, "Form is a whole number of doubles");
+    for (int i = t; i < nd; i += NT) reinterpret_cast<double*>(s_forms)[i] = reinterpret_cast<const double*>(c_forms)[i];
+  }
   Ctx C;
   // point of this thread in the tile, in the bank-conflict-free order of
   // tile_map.h; threads beyond the tile's 343 points stay inactive
@@ -482,15 +473,15 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
                   ((bf & 8u) && C.gj == T.sz1 - 4) || ((bf & 16u) && C.gk == 3) ||
                   ((bf & 32u) && C.gk == T.sz2 - 4);
       const uint64_t o = C.o;
-      s.alpha = s_io.u.v[dgr::V_ALPHA][o];
-      s.chi = s_io.u.v[dgr::V_CHI][o];
-      s.K = s_io.u.v[dgr::V_K][o];
+      s.alpha = gmem(s_io.u.v[dgr::V_ALPHA])[o];
+      s.chi = gmem(s_io.u.v[dgr::V_CHI])[o];
+      s.K = gmem(s_io.u.v[dgr::V_K])[o];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) s.beta[i] = s_io.u.v[dgr::V_BETA0 + i][o];
+      for (int i = 0; i < 3; ++i) s.beta[i] = gmem(s_io.u.v[dgr::V_BETA0 + i])[o];
 #pragma unroll
       for (int p = 0; p < 6; ++p) {
-        s.gt[p] = s_io.u.v[dgr::V_SG0 + p][o];
-        s.At[p] = s_io.u.v[dgr::V_SA0 + p][o];
+        s.gt[p] = gmem(s_io.u.v[dgr::V_SG0 + p])[o];
+        s.At[p] = gmem(s_io.u.v[dgr::V_SA0 + p])[o];
       }
       s.begin();
     }

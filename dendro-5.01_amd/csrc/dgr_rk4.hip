@@ -23,7 +23,9 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -39,9 +41,41 @@ namespace {
 constexpr int RK_NT = 256;
 enum { V_ALPHA = 0, V_CHI = 1, V_SG0 = 12, V_SA0 = 18 };
 
+// The 24 variable arrays of one state.  Kernels take these tables by value:
+// pointers in the kernel-argument segment are known to be global and cannot
+// be clobbered by the kernel's own stores, so they stay in scalar registers
+// instead of being re-read after every store.
 struct Ptrs24 {
   double* v[24];
 };
+
+// RK4 work memory: one allocation of 3 x 24 arrays of dof doubles, the
+// running final sum (acc), the stage state (stg) and the RHS (k), addressed
+// arithmetically so that only the caller's state needs a pointer table.
+struct Work {
+  double* base;
+  uint64_t dof;
+};
+
+// Per-thread cursor over the 24 variables of one point in the work memory.
+// The cursor is advanced one variable at a time and kept opaque to the
+// optimiser, so the 72 per-variable base addresses are not hoisted into
+// scalar registers (which would spill).
+typedef __attribute__((address_space(1))) double gdouble;  // global memory
+struct WorkCursor {
+  double* q;        // acc of variable v at point o
+  uint64_t s24;     // 24 * dof
+  __device__ __forceinline__ WorkCursor(const Work& W, uint64_t o) : q(W.base + o), s24(24 * W.dof) {}
+  __device__ __forceinline__ gdouble& acc() { return ((gdouble*)q)[0]; }
+  __device__ __forceinline__ gdouble& stg() { return ((gdouble*)q)[s24]; }
+  __device__ __forceinline__ gdouble& k() { return ((gdouble*)q)[2 * s24]; }
+  __device__ __forceinline__ void next(uint64_t dof) {
+    q += dof;
+    asm volatile("" : "+v"(q));
+  }
+};
+
+enum StageKind { ST_FIRST = 0, ST_MID = 1, ST_LAST = 2 };
 
 // enforce_bssn_constraints on one point (bssn_constraints.cpp:12-172):
 // rescale gt to unit determinant, remove the trace of At, floor chi and
@@ -92,50 +126,82 @@ __constant__ double c_rkC[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
 
 // Stage s of the RK4 step.  gdt[g] is the time step of block group g; the
 // coefficients are formed as the reference forms them (RK4_U[s+1] * dt,
-// RK4_C[s] * dt) and then multiply k.
+// RK4_C[s] * dt) and then multiply k.  u_prev is always the state at the
+// start of the step; KIND fixes whether the running sum starts from it
+// (stage 0) and where the new values go (the stage buffer, or the state
+// after the last stage).  Every operand of a point is loaded before the
+// first store.
+template <int KIND>
 __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ blocks, const DevTile* __restrict__ tiles,
-                                              int ntiles, const Ptrs24* __restrict__ tab, const double* __restrict__ gdt,
-                                              double chi_floor, int stage, unsigned* __restrict__ flag) {
-  // tab[0] = u_prev (state), tab[1] = k_s, tab[2] = running sum, tab[3] = u_{s+1}
-  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+                                              int ntiles, int chunk, Ptrs24 state, Work W,
+                                              const double* __restrict__ gdt, double chi_floor, int stage,
+                                              unsigned* __restrict__ flag) {
+  // contiguous runs of tiles per workgroup keep each workgroup on the same
+  // pages of the arrays it touches
+  const int wend = min(ntiles, (blockIdx.x + 1) * chunk);
+  for (int w = blockIdx.x * chunk; w < wend; ++w) {
     const DevTile tl = tiles[w];
     const DevBlock& B = blocks[tl.block];
     uint64_t o;
     if (!tile_point(tl, B, threadIdx.x, o)) continue;
     const double dt = gdt[B.group];
-    double un[24];
-    if (stage < 3) {
+    double un[24], kk[24];
+    if (KIND != ST_LAST) {
       const double cu = c_rkU[stage] * dt, cc = c_rkC[stage] * dt;
+      double acc[24];
+      WorkCursor c(W, o);
+#pragma unroll
+      for (int v = 0; v < 24; ++v) {
+        un[v] = state.v[v][o];  // u_prev: the state at the start of the step
+        kk[v] = c.k();
+        acc[v] = KIND == ST_FIRST ? un[v] : c.acc();
+        c.next(W.dof);
+      }
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
 #pragma clang fp contract(off)
-        const double up = tab[0].v[v][o];
-        const double k = tab[1].v[v][o];
-        const double acc = (stage == 0) ? up : tab[2].v[v][o];
-        tab[2].v[v][o] = acc + cc * k;
-        un[v] = up + cu * k;
+        acc[v] = acc[v] + cc * kk[v];
+        un[v] = un[v] + cu * kk[v];
       }
-      const int bad = enforce_point(un, chi_floor);
-      if (bad) atomicOr(flag, (unsigned)bad);
+      WorkCursor d(W, o);
 #pragma unroll
-      for (int v = 0; v < 24; ++v) tab[3].v[v][o] = un[v];
+      for (int v = 0; v < 24; ++v) {
+        d.acc() = acc[v];
+        d.next(W.dof);
+      }
     } else {
       const double cc = c_rkC[3] * dt;
+      WorkCursor c(W, o);
+#pragma unroll
+      for (int v = 0; v < 24; ++v) {
+        un[v] = c.acc();
+        kk[v] = c.k();
+        c.next(W.dof);
+      }
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
 #pragma clang fp contract(off)
-        un[v] = tab[2].v[v][o] + cc * tab[1].v[v][o];
+        un[v] = un[v] + cc * kk[v];
       }
-      const int bad = enforce_point(un, chi_floor);
-      if (bad) atomicOr(flag, (unsigned)bad);
+    }
+    const int bad = enforce_point(un, chi_floor);
+    if (bad) atomicOr(flag, (unsigned)bad);
+    if (KIND == ST_LAST) {
 #pragma unroll
-      for (int v = 0; v < 24; ++v) tab[0].v[v][o] = un[v];
+      for (int v = 0; v < 24; ++v) state.v[v][o] = un[v];
+    } else {
+      WorkCursor d(W, o);
+#pragma unroll
+      for (int v = 0; v < 24; ++v) {
+        d.stg() = un[v];
+        d.next(W.dof);
+      }
     }
   }
 }
 
 __global__ void __launch_bounds__(384) k_enforce(const DevBlock* __restrict__ blocks, const DevTile* __restrict__ tiles,
-                                                int ntiles, const Ptrs24* __restrict__ tab, double chi_floor,
+                                                int ntiles, Ptrs24 tab, double chi_floor,
                                                 unsigned* __restrict__ flag) {
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const DevTile tl = tiles[w];
@@ -144,12 +210,12 @@ __global__ void __launch_bounds__(384) k_enforce(const DevBlock* __restrict__ bl
     if (!tile_point(tl, B, threadIdx.x, o)) continue;
     double u[24];
 #pragma unroll
-    for (int v = 0; v < 24; ++v) u[v] = tab->v[v][o];
+    for (int v = 0; v < 24; ++v) u[v] = tab.v[v][o];
     const int bad = enforce_point(u, chi_floor);
     if (bad) atomicOr(flag, (unsigned)bad);
     if (bad & 1) continue;  // the reference exits before touching this node
 #pragma unroll
-    for (int v = 0; v < 24; ++v) tab->v[v][o] = u[v];
+    for (int v = 0; v < 24; ++v) tab.v[v][o] = u[v];
   }
 }
 
@@ -157,7 +223,7 @@ __global__ void __launch_bounds__(384) k_enforce(const DevBlock* __restrict__ bl
 // the same physical point in the interior of the same-level neighbour.
 __global__ void __launch_bounds__(RK_NT) k_ghost_refill(const DevBlock* __restrict__ blocks,
                                                        const int32_t* __restrict__ nbr, int nblocks,
-                                                       const Ptrs24* __restrict__ tab) {
+                                                       Ptrs24 tab) {
   for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
     const DevBlock& B = blocks[b];
     const int s0 = B.sz[0], s1 = B.sz[1], s2 = B.sz[2];
@@ -175,14 +241,18 @@ __global__ void __launch_bounds__(RK_NT) k_ghost_refill(const DevBlock* __restri
       const int si = i - di * (s0 - 7), sj = j - dj * (s1 - 7), sk = k - dk * (s2 - 7);
       const uint64_t src = N.offset + (uint64_t)si + (uint64_t)s0 * (sj + (uint64_t)s1 * sk);
       const uint64_t dst = B.offset + (uint64_t)e;
-#pragma unroll 4
-      for (int v = 0; v < 24; ++v) tab->v[v][dst] = tab->v[v][src];
+      double t[24];  // all loads before the stores (see k_stage)
+#pragma unroll
+      for (int v = 0; v < 24; ++v) t[v] = tab.v[v][src];
+#pragma unroll
+      for (int v = 0; v < 24; ++v) tab.v[v][dst] = t[v];
     }
   }
 }
 
+// One workgroup per tile or block, capped well above what fills the chip.
 int grid_for(dgr_plan_t plan, int n) {
-  const int g = plan->grid * 4;
+  const int g = plan->grid * 64;
   return n < g ? (n > 0 ? n : 1) : g;
 }
 
@@ -192,10 +262,7 @@ struct dgr_rk4_s {
   dgr_plan_t plan = nullptr;
   uint64_t dof = 0;
   double* buf = nullptr;        // acc, stage state, k: 3 x 24 x dof
-  Ptrs24 state{};
-  Ptrs24* d_tab = nullptr;      // [4] per stage kind, see k_stage
-  Ptrs24* d_state = nullptr;    // the state alone (refill / enforce)
-  Ptrs24* d_stage = nullptr;    // the stage state alone
+  Ptrs24 state{}, acc{}, stg{}, k{};  // the state and the work arrays in buf
   IOPtrs* d_io = nullptr;       // RHS pointer tables: [0] (u -> k), [1] (stage -> k)
   double* d_gdt = nullptr;      // time step per block group
   std::vector<double> h_gdt;
@@ -217,10 +284,22 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
     // k_s = RHS(u_s): stage 0 reads the state, later stages the stage buffer
     int rc = launch_rhs(p, r->d_io + (s == 0 ? 0 : 1), st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_stage, dim3(grid_for(p, nt)), dim3(384), 0, st, p->d_blocks, p->d_tiles, nt, r->d_tab,
-                       r->d_gdt, p->chi_floor, s, r->d_flag);
+    // stage combine + projection, then the same-level ghost refill
+    const int grid = std::min(nt, std::max(1, p->grid * 2));
+    const int chunk = (nt + grid - 1) / grid;
+    const Work W = {r->buf, r->dof};
+    const dim3 g((nt + chunk - 1) / chunk), b(384);
+    if (s == 0)
+      hipLaunchKernelGGL(k_stage<ST_FIRST>, g, b, 0, st, p->d_blocks, p->d_tiles, nt, chunk, r->state, W, r->d_gdt,
+                         p->chi_floor, s, r->d_flag);
+    else if (s < 3)
+      hipLaunchKernelGGL(k_stage<ST_MID>, g, b, 0, st, p->d_blocks, p->d_tiles, nt, chunk, r->state, W, r->d_gdt,
+                         p->chi_floor, s, r->d_flag);
+    else
+      hipLaunchKernelGGL(k_stage<ST_LAST>, g, b, 0, st, p->d_blocks, p->d_tiles, nt, chunk, r->state, W, r->d_gdt,
+                         p->chi_floor, s, r->d_flag);
     hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(p, (int)p->nblocks)), dim3(RK_NT), 0, st, p->d_blocks,
-                       p->d_nbr, (int)p->nblocks, s < 3 ? r->d_stage : r->d_state);
+                       p->d_nbr, (int)p->nblocks, s < 3 ? r->stg : r->state);
   }
   DGR_HIP(hipGetLastError());
   return DGR_OK;
@@ -244,11 +323,6 @@ int set_group_dt(dgr_rk4_s* r, const double* dt, uint32_t n) {
   return DGR_OK;
 }
 
-int upload_tab(Ptrs24* d, const Ptrs24* h, int n, hipStream_t st) {
-  DGR_HIP(hipMemcpyAsync(d, h, sizeof(Ptrs24) * n, hipMemcpyHostToDevice, st));
-  return DGR_OK;
-}
-
 int check_ptrs(double* const* u) {
   if (!u) return DGR_E_INVALID;
   for (int v = 0; v < 24; ++v)
@@ -256,11 +330,10 @@ int check_ptrs(double* const* u) {
   return DGR_OK;
 }
 
-// Small device table of 24 pointers for a one-off kernel call; freed after
-// the stream has consumed it.
-struct TmpTab {
-  Ptrs24* d = nullptr;
-  ~TmpTab() { (void)hipFree(d); }
+// Device flag word of a one-off kernel call, freed on return.
+struct TmpFlag {
+  unsigned* d = nullptr;
+  ~TmpFlag() { (void)hipFree(d); }
 };
 
 void destroy(dgr_rk4_s* r) {
@@ -268,7 +341,6 @@ void destroy(dgr_rk4_s* r) {
   if (r->graph) (void)hipGraphDestroy(r->graph);
   if (r->st) (void)hipStreamDestroy(r->st);
   if (r->own_buf) (void)hipFree(r->buf);
-  (void)hipFree(r->d_tab);
   (void)hipFree(r->d_io);
   (void)hipFree(r->d_gdt);
   (void)hipFree(r->d_flag);
@@ -282,11 +354,8 @@ extern "C" int dgr_ghost_refill(dgr_plan_t plan, double* const* u, void* stream)
   hipStream_t st = (hipStream_t)stream;
   Ptrs24 h;
   for (int v = 0; v < 24; ++v) h.v[v] = u[v];
-  TmpTab t;
-  DGR_HIP(hipMalloc(&t.d, sizeof(Ptrs24)));
-  DGR_HIP(hipMemcpyAsync(t.d, &h, sizeof(Ptrs24), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(plan, (int)plan->nblocks)), dim3(RK_NT), 0, st, plan->d_blocks,
-                     plan->d_nbr, (int)plan->nblocks, t.d);
+                     plan->d_nbr, (int)plan->nblocks, h);
   DGR_HIP(hipGetLastError());
   DGR_HIP(hipStreamSynchronize(st));
   return DGR_OK;
@@ -297,14 +366,12 @@ extern "C" int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* 
   hipStream_t st = (hipStream_t)stream;
   Ptrs24 h;
   for (int v = 0; v < 24; ++v) h.v[v] = u[v];
-  TmpTab t;
-  unsigned* d_flag = nullptr;
-  DGR_HIP(hipMalloc(&t.d, sizeof(Ptrs24) + 256));
-  d_flag = (unsigned*)((char*)t.d + sizeof(Ptrs24));
-  DGR_HIP(hipMemcpyAsync(t.d, &h, sizeof(Ptrs24), hipMemcpyHostToDevice, st));
+  TmpFlag t;
+  DGR_HIP(hipMalloc(&t.d, 256));
+  unsigned* d_flag = t.d;
   DGR_HIP(hipMemsetAsync(d_flag, 0, sizeof(unsigned), st));
   const int nt = (int)plan->ntiles;
-  hipLaunchKernelGGL(k_enforce, dim3(grid_for(plan, nt)), dim3(384), 0, st, plan->d_blocks, plan->d_tiles, nt, t.d,
+  hipLaunchKernelGGL(k_enforce, dim3(grid_for(plan, nt)), dim3(384), 0, st, plan->d_blocks, plan->d_tiles, nt, h,
                      plan->chi_floor, d_flag);
   DGR_HIP(hipGetLastError());
   unsigned flag = 0;
@@ -328,19 +395,19 @@ extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double*
   r->own_buf = work == nullptr;
   r->buf = work;
   if ((r->own_buf && hipMalloc(&r->buf, nb * 24 * 3) != hipSuccess) ||
-      hipMalloc(&r->d_tab, sizeof(Ptrs24) * 6) != hipSuccess || hipMalloc(&r->d_io, sizeof(IOPtrs) * 2) != hipSuccess ||
+      hipMalloc(&r->d_io, sizeof(IOPtrs) * 2) != hipSuccess ||
       hipMalloc(&r->d_gdt, sizeof(double) * plan->ngroups) != hipSuccess || hipMalloc(&r->d_flag, 256) != hipSuccess) {
     destroy(r);
     return DGR_E_NOMEM;
   }
-  Ptrs24 acc, stg, k;
+  Ptrs24& acc = r->acc;
+  Ptrs24& stg = r->stg;
+  Ptrs24& k = r->k;
   for (int v = 0; v < 24; ++v) {
     acc.v[v] = r->buf + (size_t)v * r->dof;
     stg.v[v] = r->buf + (size_t)(24 + v) * r->dof;
     k.v[v] = r->buf + (size_t)(48 + v) * r->dof;
   }
-  // tables: [0..3] k_stage (state, k, acc, stage), [4] state, [5] stage
-  Ptrs24 h[6] = {r->state, k, acc, stg, r->state, stg};
   IOPtrs io[2];
   for (int v = 0; v < 24; ++v) {
     io[0].u.v[v] = r->state.v[v];
@@ -348,10 +415,7 @@ extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double*
     io[1].u.v[v] = stg.v[v];
     io[1].r.v[v] = k.v[v];
   }
-  r->d_state = r->d_tab + 4;
-  r->d_stage = r->d_tab + 5;
   hipError_t e = hipStreamCreateWithFlags(&r->st, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMemcpy(r->d_tab, h, sizeof(h), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(r->d_io, io, sizeof(io), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(r->d_flag, 0, 256);
   // ghosts the refill never reaches (no same-level neighbour) keep the

@@ -1,0 +1,182 @@
+/*
+ * deriv42 stencils near flagged faces as coefficient tables.
+ *
+ * Every deriv42 operator at block coordinate q along an axis reads only the
+ * seven points q-3 .. q+3 (the one-sided closures at the first / last three
+ * interior points included), so each operator at each position is a row of
+ * seven coefficients over that window plus a final scale:
+ *
+ *   interior   deriv42_x/xx, deriv42adv_x, ko_deriv42_x
+ *              (BSSN_GR/src/derivs.cpp:11-88, 244-328, 494-634, 914-1079)
+ *   closures   the i == 3, 4, 5 and ie-1, ie-2, ie-3 branches of the same
+ *              functions, taken when the face's bflag bit is set.
+ *
+ * A row is summed in the reference's term order with FP contraction off:
+ * ascending window position for all forms except the ko_deriv42 low-face
+ * closures, which the reference writes from the highest point down
+ * (derivs.cpp:1002-1020).  Window points with a zero coefficient are skipped
+ * by a select, so their values (possibly unset padding) never enter, and the
+ * sum starts from +0, so each result equals the reference expression
+ * (a leading +0 and skipped terms change at most the sign of a zero).  The
+ * final step multiplies by the axis constant the reference multiplies by, or
+ * divides by smr1/2/3 where it divides.
+ *
+ * Used by the boundary-tile kernel in place of per-case branches: one code
+ * path per operator instead of up to seven divergent ones.
+ */
+#ifndef DGR_STENCIL_FORMS_H
+#define DGR_STENCIL_FORMS_H
+
+#ifdef __HIPCC__
+#define DGR_HD __host__ __device__
+#else
+#define DGR_HD
+#endif
+
+namespace dgr_forms {
+
+// Scale selectors: the first eight doubles of dgr_internal::AxisC in order.
+enum Scale { S_IH2 = 0, S_IH12, S_ISQ, S_ISQ12, S_PRE, S_SMR1, S_SMR2, S_SMR3 };
+
+struct Form {
+  double c[7];  // coefficients of points q-3 .. q+3
+  int sel;      // Scale; S_SMR* divide, the others multiply
+  int desc;     // 1: sum from q+3 down to q-3
+};
+
+enum FormId {
+  // deriv42_x (derivs.cpp:11-88)
+  F_D1 = 0, F_D1_L3, F_D1_L4, F_D1_H2, F_D1_H1,
+  // deriv42_xx (derivs.cpp:244-328)
+  F_D2, F_D2_L3, F_D2_L4, F_D2_H2, F_D2_H1,
+  // deriv42adv_x (derivs.cpp:494-634): P = beta > 0, N = otherwise
+  F_ADV_P, F_ADV_N, F_ADV_L3, F_ADV_L4P, F_ADV_L4N, F_ADV_L5P, F_ADV_L5N,
+  F_ADV_H1, F_ADV_H2P, F_ADV_H2N, F_ADV_H3P, F_ADV_H3N,
+  // ko_deriv42_x (derivs.cpp:914-1079)
+  F_KO, F_KO_L3, F_KO_L4, F_KO_L5, F_KO_H3, F_KO_H2, F_KO_H1,
+  F_NFORMS
+};
+
+// Window index w: 0 = q-3, 3 = q, 6 = q+3.
+#define DGR_FORMS_TABLE                                                                   \
+  {                                                                                       \
+    /* F_D1    (u[-2] - 8u[-1] + 8u[+1] - u[+2]) ih12 */                                  \
+    {{0, 1, -8, 0, 8, -1, 0}, ::dgr_forms::S_IH12, 0},                                                 \
+    /* F_D1_L3 (-3u3 + 4u4 - u5) ih2, q = 3 */                                            \
+    {{0, 0, 0, -3, 4, -1, 0}, ::dgr_forms::S_IH2, 0},                                                  \
+    /* F_D1_L4 (-u3 + u5) ih2, q = 4 */                                                   \
+    {{0, 0, -1, 0, 1, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    /* F_D1_H2 (-u[e-3] + u[e-1]) ih2, q = e-2 */                                         \
+    {{0, 0, -1, 0, 1, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    /* F_D1_H1 (u[e-3] - 4u[e-2] + 3u[e-1]) ih2, q = e-1 */                               \
+    {{0, 1, -4, 3, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    /* F_D2    (-u[-2] + 16u[-1] - 30u0 + 16u[+1] - u[+2]) isq12 */                       \
+    {{0, -1, 16, -30, 16, -1, 0}, ::dgr_forms::S_ISQ12, 0},                                            \
+    /* F_D2_L3 (2u3 - 5u4 + 4u5 - u6) isq */                                              \
+    {{0, 0, 0, 2, -5, 4, -1}, ::dgr_forms::S_ISQ, 0},                                                  \
+    /* F_D2_L4 (u3 - 2u4 + u5) isq */                                                     \
+    {{0, 0, 1, -2, 1, 0, 0}, ::dgr_forms::S_ISQ, 0},                                                   \
+    /* F_D2_H2 (u[e-3] - 2u[e-2] + u[e-1]) isq */                                         \
+    {{0, 0, 1, -2, 1, 0, 0}, ::dgr_forms::S_ISQ, 0},                                                   \
+    /* F_D2_H1 (-u[e-4] + 4u[e-3] - 5u[e-2] + 2u[e-1]) isq */                             \
+    {{-1, 4, -5, 2, 0, 0, 0}, ::dgr_forms::S_ISQ, 0},                                                  \
+    /* F_ADV_P (-3u[-1] - 10u0 + 18u[+1] - 6u[+2] + u[+3]) ih12 */                        \
+    {{0, 0, -3, -10, 18, -6, 1}, ::dgr_forms::S_IH12, 0},                                              \
+    /* F_ADV_N (-u[-3] + 6u[-2] - 18u[-1] + 10u0 + 3u[+1]) ih12 */                        \
+    {{-1, 6, -18, 10, 3, 0, 0}, ::dgr_forms::S_IH12, 0},                                               \
+    /* F_ADV_L3 (-3u3 + 4u4 - u5) ih2 */                                                  \
+    {{0, 0, 0, -3, 4, -1, 0}, ::dgr_forms::S_IH2, 0},                                                  \
+    /* F_ADV_L4P (-3u4 + 4u5 - u6) ih2 */                                                 \
+    {{0, 0, 0, -3, 4, -1, 0}, ::dgr_forms::S_IH2, 0},                                                  \
+    /* F_ADV_L4N (-u3 + u5) ih2 */                                                        \
+    {{0, 0, -1, 0, 1, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    /* F_ADV_L5P (-3u4 - 10u5 + 18u6 - 6u7 + u8) ih12 */                                  \
+    {{0, 0, -3, -10, 18, -6, 1}, ::dgr_forms::S_IH12, 0},                                              \
+    /* F_ADV_L5N (u3 - 4u4 + 3u5) ih2 */                                                  \
+    {{0, 1, -4, 3, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    /* F_ADV_H1 (u[e-3] - 4u[e-2] + 3u[e-1]) ih2 */                                       \
+    {{0, 1, -4, 3, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    /* F_ADV_H2P (-u[e-3] + u[e-1]) ih2 */                                                \
+    {{0, 0, -1, 0, 1, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    /* F_ADV_H2N (u[e-4] - 4u[e-3] + 3u[e-2]) ih2 */                                      \
+    {{0, 1, -4, 3, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    /* F_ADV_H3P (-3u[e-3] + 4u[e-2] - u[e-1]) ih2 */                                     \
+    {{0, 0, 0, -3, 4, -1, 0}, ::dgr_forms::S_IH2, 0},                                                  \
+    /* F_ADV_H3N (-u[e-6] + 6u[e-5] - 18u[e-4] + 10u[e-3] + 3u[e-2]) ih12 */              \
+    {{-1, 6, -18, 10, 3, 0, 0}, ::dgr_forms::S_IH12, 0},                                               \
+    /* F_KO    pre (-u[-3] + 6u[-2] - 15u[-1] + 20u0 - 15u[+1] + 6u[+2] - u[+3]) */       \
+    {{-1, 6, -15, 20, -15, 6, -1}, ::dgr_forms::S_PRE, 0},                                             \
+    /* F_KO_L3 (u6 - 3u5 + 3u4 - u3) / smr3, summed downwards */                          \
+    {{0, 0, 0, -1, 3, -3, 1}, ::dgr_forms::S_SMR3, 1},                                                 \
+    /* F_KO_L4 (u7 - 6u6 + 12u5 - 10u4 + 3u3) / smr2, downwards */                        \
+    {{0, 0, 3, -10, 12, -6, 1}, ::dgr_forms::S_SMR2, 1},                                               \
+    /* F_KO_L5 (u8 - 6u7 + 15u6 - 19u5 + 12u4 - 3u3) / smr1, downwards */                 \
+    {{0, -3, 12, -19, 15, -6, 1}, ::dgr_forms::S_SMR1, 1},                                             \
+    /* F_KO_H3 (u[e-6] - 6u[e-5] + 15u[e-4] - 19u[e-3] + 12u[e-2] - 3u[e-1]) / smr1 */    \
+    {{1, -6, 15, -19, 12, -3, 0}, ::dgr_forms::S_SMR1, 0},                                             \
+    /* F_KO_H2 (u[e-5] - 6u[e-4] + 12u[e-3] - 10u[e-2] + 3u[e-1]) / smr2 */               \
+    {{1, -6, 12, -10, 3, 0, 0}, ::dgr_forms::S_SMR2, 0},                                               \
+    /* F_KO_H1 (u[e-4] - 3u[e-3] + 3u[e-2] - u[e-1]) / smr3 */                            \
+    {{1, -3, 3, -1, 0, 0, 0}, ::dgr_forms::S_SMR3, 0},                                                 \
+  }
+
+// Form of each operator at position q of an axis ending at e = sz-3 (the
+// first padding point), with the low / high face flags lf, hf.  The order of
+// the tests follows the reference's branches.
+DGR_HD inline int form_d1(int q, int e, bool lf, bool hf) {
+  if (hf && q == e - 1) return F_D1_H1;
+  if (hf && q == e - 2) return F_D1_H2;
+  if (lf && q == 4) return F_D1_L4;
+  if (lf && q == 3) return F_D1_L3;
+  return F_D1;
+}
+DGR_HD inline int form_d2(int q, int e, bool lf, bool hf) {
+  if (hf && q == e - 1) return F_D2_H1;
+  if (hf && q == e - 2) return F_D2_H2;
+  if (lf && q == 4) return F_D2_L4;
+  if (lf && q == 3) return F_D2_L3;
+  return F_D2;
+}
+DGR_HD inline int form_adv(int q, int e, bool lf, bool hf, bool pos) {
+  if (hf && q == e - 1) return F_ADV_H1;
+  if (hf && q == e - 2) return pos ? F_ADV_H2P : F_ADV_H2N;
+  if (hf && q == e - 3) return pos ? F_ADV_H3P : F_ADV_H3N;
+  if (lf && q == 3) return F_ADV_L3;
+  if (lf && q == 4) return pos ? F_ADV_L4P : F_ADV_L4N;
+  if (lf && q == 5) return pos ? F_ADV_L5P : F_ADV_L5N;
+  return pos ? F_ADV_P : F_ADV_N;
+}
+DGR_HD inline int form_ko(int q, int e, bool lf, bool hf) {
+  if (hf && q == e - 1) return F_KO_H1;
+  if (hf && q == e - 2) return F_KO_H2;
+  if (hf && q == e - 3) return F_KO_H3;
+  if (lf && q == 3) return F_KO_L3;
+  if (lf && q == 4) return F_KO_L4;
+  if (lf && q == 5) return F_KO_L5;
+  return F_KO;
+}
+
+// Sum of one form over the window w[0..6] (points q-3 .. q+3) in ascending
+// order; the caller takes the descending sum for desc forms.
+DGR_HD inline double form_sum_up(const double* c, const double (&w)[7]) {
+#pragma clang fp contract(off)
+  double t = 0.0;
+  for (int m = 0; m < 7; ++m) t = t + (c[m] != 0.0 ? c[m] * w[m] : 0.0);
+  return t;
+}
+DGR_HD inline double form_sum_down(const double* c, const double (&w)[7]) {
+#pragma clang fp contract(off)
+  double t = 0.0;
+  for (int m = 6; m >= 0; --m) t = t + (c[m] != 0.0 ? c[m] * w[m] : 0.0);
+  return t;
+}
+// Final scale: scales[] are ih2, ih12, isq, isq12, pre, smr1, smr2, smr3.
+DGR_HD inline double form_scale(double sum, int sel, const double* scales) {
+#pragma clang fp contract(off)
+  const double s = scales[sel];
+  return sel >= S_SMR1 ? sum / s : sum * s;
+}
+
+}  // namespace dgr_forms
+
+#endif

@@ -10,6 +10,10 @@ job is one block group, so ghost refill never crosses jobs and each job keeps
 its own dt).  Jobs arriving while others run are admitted at the next
 iteration boundary as memory frees up, so their work overlaps on the device.
 
+Admission also keeps the resident interior points under `max_resident_points`
+(enough to fill the GPU): past that, adding jobs only stretches every resident
+job's latency without raising throughput.
+
 Because every kernel computes each point from its own block (and its group's
 neighbours) only, a job's result is bitwise the same alone and in any mix.
 """
@@ -109,8 +113,10 @@ class PoolAllocator:
 class Server:
     """Runs jobs to completion; returns per-job latency and throughput."""
 
-    def __init__(self, hbm_budget_bytes, device="cuda:0", use_graph=True, params=None, keep_results=True):
+    def __init__(self, hbm_budget_bytes, device="cuda:0", use_graph=True, params=None, keep_results=True,
+                 max_resident_points=None):
         self.device = torch.device(device)
+        self.max_resident_points = max_resident_points
         self.capacity = int(hbm_budget_bytes // BYTES_PER_POINT)
         self.U = torch.zeros((24, self.capacity), dtype=torch.float64, device=self.device)
         self.work = torch.empty(3 * 24 * self.capacity, dtype=torch.float64, device=self.device)
@@ -147,13 +153,18 @@ class Server:
         resident = []
         done = []
         plan = rk = None
+        host_s = 0.0
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
         while pending or resident:
             changed = False
             # admit in arrival order while the head of the queue fits
+            th = time.perf_counter()
             while pending and (pending[0].arrival <= now() or not realtime):
                 j = pending[0]
+                if (self.max_resident_points and resident
+                        and sum(r.points for r in resident) + j.points > self.max_resident_points):
+                    break
                 got = self.alloc.alloc(j.dof)
                 if got is None:
                     if not resident:
@@ -174,6 +185,7 @@ class Server:
                     rk.close()
                     plan.close()
                 plan, rk = self._rebuild(resident)
+            host_s += time.perf_counter() - th
             rk.run(1)
             torch.cuda.current_stream().synchronize()
             finished = []
@@ -207,6 +219,8 @@ class Server:
             "makespan_s": makespan,
             "rhs_points_per_s": rhs_points / makespan if makespan > 0 else None,
             "plan_rebuilds": self.rebuilds,
+            "admission_and_rebuild_s": host_s,
+            "max_resident_points": self.max_resident_points,
         }
 
 

@@ -1,0 +1,72 @@
+"""The coefficient-table stencils of the boundary-tile kernel
+(csrc/stencil_forms.h), built for the host, against the oracle's deriv42
+operators: bitwise on every interior point, every face-flag combination,
+both signs of the shift, blocks of 13^3 and 19x13x25."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "native", "libforms_host.so")
+
+
+@pytest.fixture(scope="module")
+def forms():
+    if not os.path.exists(LIB):
+        pytest.fail("tests/native/libforms_host.so missing: run dendro-5.01_amd/build.py --all")
+    L = ctypes.CDLL(LIB)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.forms_deriv.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_double, ctypes.POINTER(ctypes.c_uint),
+                              ctypes.c_uint, dp]
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@pytest.mark.parametrize("sz", [(13, 13, 13), (19, 13, 25)])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_forms_match_oracle_bitwise(forms, kind, sz):
+    rng = np.random.default_rng(kind * 7 + sz[0])
+    n = sz[0] * sz[1] * sz[2]
+    u = rng.standard_normal(n)
+    beta = rng.standard_normal(n)
+    beta[::5] = 0.0  # beta == 0 takes the "not positive" branch
+    h = 1.0 / 48
+    szc = (ctypes.c_uint * 3)(*sz)
+    ii = np.zeros(sz[::-1], dtype=bool)
+    ii[3:-3, 3:-3, 3:-3] = True
+    ii = ii.ravel()
+    for axis in range(3):
+        for flags in range(4):
+            bflag = (flags & 1) << (2 * axis) | ((flags >> 1) & 1) << (2 * axis + 1)
+            bflag |= 0b010101 & ~(3 << (2 * axis))  # other axes' low faces: must not matter
+            ref = po.deriv(kind, axis, u, h, sz, bflag=bflag, beta=beta if kind == 2 else None)
+            got = np.full(n, np.nan)
+            forms.forms_deriv(kind, axis, _p(got), _p(u), h, szc, bflag, _p(beta))
+            assert np.array_equal(got[ii], ref[ii]), (kind, axis, bflag, np.abs(got[ii] - ref[ii]).max())
+
+
+def test_forms_ignore_unused_padding(forms):
+    """Closures never read past the face: NaN in the padding beyond a
+    flagged face does not reach any interior result."""
+    sz = (13, 13, 13)
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(13 ** 3).reshape(13, 13, 13)
+    clean = u.copy()
+    u[:, :, :3] = np.nan
+    u[:, :, -3:] = np.nan
+    beta = rng.standard_normal(13 ** 3)
+    szc = (ctypes.c_uint * 3)(*sz)
+    for kind in range(4):
+        got = np.full(13 ** 3, np.nan)
+        forms.forms_deriv(kind, 0, _p(got), _p(np.ascontiguousarray(u.ravel())), 1.0 / 48, szc, 0b11, _p(beta))
+        ref = po.deriv(kind, 0, clean.ravel(), 1.0 / 48, sz, bflag=0b11, beta=beta if kind == 2 else None)
+        g = got.reshape(13, 13, 13)[3:-3, 3:-3, 3:-3]
+        r = ref.reshape(13, 13, 13)[3:-3, 3:-3, 3:-3]
+        assert np.array_equal(g, r), kind
