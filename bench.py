@@ -17,6 +17,10 @@ Also reported on the same line:
   per_block      the plain one-launch-per-block RHS path;
   uneven         3000 blocks of padded sizes 13/19/25/37 (60/25/10/5 %,
                  seeded), flat space plus Gaussian bumps: RHS and RK4 step;
+  autotune       the schedule (tile extent per size class, tile order, grid)
+                 the autotuner picked for each workload, its time against the
+                 library default, and the tuning time (outside every timed
+                 region; cached per device and workload shape);
   serving        a bounded Poisson stream (20 jobs/s) of 50-step evolutions
                  of 64..4096 blocks under a 200 GB HBM budget: p50/p99 job
                  latency and aggregate RHS points/s;
@@ -129,15 +133,29 @@ def time_rk4(rk, stream, steps, warm):
     return e0.elapsed_time(e1) / steps, wall
 
 
-def uneven(stream, steps):
+def tuned_schedule(blocks, params, enabled):
+    """Autotuned schedule (None = library default) and its report."""
+    if not enabled:
+        return None, None
+    from pydgr import autotune
+    sched, rep = autotune.tune(blocks, params, device="cuda:0", log=log)
+    short = {k: rep[k] for k in ("schedule", "best_ms", "default_ms", "tuning_s", "sample_blocks", "cached")}
+    log("autotune: %s (%.3f ms vs default %.3f ms, %s)" % (rep["schedule"], rep["best_ms"], rep["default_ms"],
+                                                        "cached" if rep["cached"] else "%.1f s" % rep["tuning_s"]))
+    return sched, short
+
+
+def uneven(stream, steps, autotune_on):
     import torch
     from pydgr import RK4, Plan, data, layout, make_params
     blocks, dof = layout.mixed_blocks(3000, seed=0)
     pts = layout.interior_points(blocks)
     dev = torch.device("cuda:0")
+    prm = make_params(ko_sigma=0.4, eta_const=2.0)
+    sched, tune_rep = tuned_schedule(blocks, prm, autotune_on)
     U = data.flat_bumps(blocks, dof, seed=0, amp=1e-3, width=0.1, device=dev)
     R = torch.zeros_like(U)
-    plan = Plan(blocks, make_params(ko_sigma=0.4, eta_const=2.0))
+    plan = Plan(blocks, prm, schedule=sched)
     ms = time_rhs(plan, R, U, stream, steps, 2)
     ms_pb = time_rhs(plan, R, U, stream, 1, 0, per_block=True)
     nbytes = algorithmic_bytes(blocks)
@@ -156,7 +174,8 @@ def uneven(stream, steps):
            "rhs_hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
            "rhs_fp64_frac": REF_FLOP_PER_POINT * pts / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
            "per_block_ms": ms_pb, "batched_speedup": ms_pb / ms,
-           "rk4_ms_per_step": ms_rk, "rk4_rhs_points_per_s": 4 * pts / (ms_rk * 1e-3)}
+           "rk4_ms_per_step": ms_rk, "rk4_rhs_points_per_s": 4 * pts / (ms_rk * 1e-3),
+           "autotune": tune_rep}
     del U
     torch.cuda.empty_cache()
     return out
@@ -190,6 +209,7 @@ def main():
     ap.add_argument("--serving-max-points", type=int, default=3_000_000,
                     help="cap on resident interior points (0: memory budget only)")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--no-autotune", action="store_true", help="library default schedule instead of the autotuner's")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box here has one GPU); "
@@ -205,7 +225,9 @@ def main():
     pts = layout.interior_points(blocks)
     U = data.gauge_wave(blocks, dof, amp=0.01, noise=1e-6, seed=0, device=dev)
     R = torch.zeros_like(U)
-    plan = Plan(blocks, make_params(ko_sigma=0.4, eta_const=2.0))
+    prm = make_params(ko_sigma=0.4, eta_const=2.0)
+    sched, tune_rep = tuned_schedule(blocks, prm, not a.no_autotune)
+    plan = Plan(blocks, prm, schedule=sched)
     stream = torch.cuda.current_stream()
     log("%d blocks 13^3, %d interior points, %.2f GB state" % (len(blocks), pts, U.numel() * 8 / 1e9))
 
@@ -252,6 +274,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_bssn_rhs", "kernel_ms": rhs_ms, "bytes_per_launch": nbytes},
         "rhs": {"ms": rhs_ms, "points_per_s": rhs_pps},
+        "autotune": tune_rep,
         "fp64": {"reference_flop_per_point": REF_FLOP_PER_POINT,
                  "achieved_tflops": REF_FLOP_PER_POINT * rhs_pps / 1e12,
                  "frac_of_peak": REF_FLOP_PER_POINT * rhs_pps / 1e12 / FP64_PEAK_TFLOPS},
@@ -261,7 +284,7 @@ def main():
         "wall_s_timed_region": wall,
     }
     if not a.no_uneven:
-        line["uneven"] = uneven(stream, a.steps)
+        line["uneven"] = uneven(stream, a.steps, not a.no_autotune)
         log("uneven: RHS %.4f ms, RK4 %.4f ms" % (line["uneven"]["rhs_ms"], line["uneven"]["rk4_ms_per_step"]))
     if a.serving_jobs > 0:
         line["serving"] = serving(a.serving_jobs, a.serving_budget_gb, a.serving_max_points or None)

@@ -508,6 +508,7 @@ namespace {
 void free_plan(dgr_plan_s* plan) {
   (void)hipFree(plan->d_blocks);
   (void)hipFree(plan->d_tiles);
+  (void)hipFree(plan->d_tiles_blk);
   (void)hipFree(plan->d_io);
   (void)hipFree(plan->d_nbr);
   delete plan;
@@ -565,9 +566,10 @@ void build_neighbours(const dgr_block* blocks, const std::vector<DevBlock>& hb, 
 }
 
 // Split n interior points into chunks of at most 7, as even as possible.
-void split(int n, std::vector<std::pair<int, int>>& out) {
+// Cut n interior points into the fewest near-equal runs of at most tmax.
+void split(int n, int tmax, std::vector<std::pair<int, int>>& out) {
   out.clear();
-  const int parts = (n + TX - 1) / TX;
+  const int parts = (n + tmax - 1) / tmax;
   int start = 3;
   for (int p = 0; p < parts; ++p) {
     const int len = n / parts + (p < n % parts ? 1 : 0);
@@ -598,13 +600,74 @@ extern "C" int dgr_init(int device) {
   return DGR_OK;
 }
 
+extern "C" void dgr_schedule_default(dgr_schedule* s) {
+  if (!s) return;
+  for (int c = 0; c < DGR_SIZE_CLASSES; ++c) s->tile_max[c] = TX;
+  s->order = DGR_ORDER_BLOCK;
+  s->grid_mult = 1;
+}
+
+namespace {
+int size_class(const dgr_block& b) {
+  const uint32_t m = std::max(b.sz[0], std::max(b.sz[1], b.sz[2]));
+  return m <= 13 ? 0 : m <= 19 ? 1 : m <= 25 ? 2 : 3;
+}
+
+// Scheduled order of the tiles [begin, end) of `blk` (block order, a block's
+// tiles contiguous).  XCD-aware: workgroup w of a launch runs on XCD w % 8
+// and the persistent loop hands it tiles w, w + grid, ...; so a tile at
+// position t runs on XCD t % 8 whenever the grid is a multiple of 8.  Blocks
+// are dealt to 8 queues (fewest tiles first) and the queues interleaved,
+// which keeps each block's tiles, and the halo they share, on one L2.
+void order_tiles(std::vector<DevTile>& t, size_t begin, size_t end, uint32_t order, const dgr_block* blocks) {
+  if (order == DGR_ORDER_LARGEST) {
+    std::stable_sort(t.begin() + begin, t.begin() + end, [&](const DevTile& a, const DevTile& b) {
+      const dgr_block &A = blocks[a.block], &B = blocks[b.block];
+      return (uint64_t)A.sz[0] * A.sz[1] * A.sz[2] > (uint64_t)B.sz[0] * B.sz[1] * B.sz[2];
+    });
+  } else if (order == DGR_ORDER_XCD) {
+    constexpr int NXCD = 8;
+    std::vector<std::vector<DevTile>> q(NXCD);
+    size_t i = begin;
+    while (i < end) {
+      size_t j = i;
+      while (j < end && t[j].block == t[i].block) ++j;
+      int best = 0;
+      for (int x = 1; x < NXCD; ++x)
+        if (q[x].size() < q[best].size()) best = x;
+      q[best].insert(q[best].end(), t.begin() + i, t.begin() + j);
+      i = j;
+    }
+    std::vector<size_t> pos(NXCD, 0);
+    size_t o = begin;
+    while (o < end)
+      for (int x = 0; x < NXCD && o < end; ++x)
+        if (pos[x] < q[x].size()) t[o++] = q[x][pos[x]++];
+  }
+}
+}  // namespace
+
 extern "C" int dgr_plan_create(dgr_plan_t* plan_out, const dgr_block* blocks, uint32_t nblocks,
                                const dgr_bssn_params* p) {
+  return dgr_plan_create_ex(plan_out, blocks, nblocks, p, nullptr);
+}
+
+extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks, uint32_t nblocks,
+                                  const dgr_bssn_params* p, const dgr_schedule* sched) {
   if (!plan_out || !blocks || !p || nblocks == 0) return DGR_E_INVALID;
   if (p->deriv_order != 42) return DGR_E_UNSUPPORTED;
+  dgr_schedule sc;
+  dgr_schedule_default(&sc);
+  if (sched) {
+    sc = *sched;
+    for (int c = 0; c < DGR_SIZE_CLASSES; ++c)
+      if (sc.tile_max[c] < 3 || sc.tile_max[c] > (uint32_t)TX) return DGR_E_INVALID;
+    if (sc.order > DGR_ORDER_LARGEST || sc.grid_mult > 64) return DGR_E_INVALID;
+  }
   std::vector<DevBlock> hb(nblocks);
   std::vector<DevTile> ht;
   dgr_plan_s* plan = new dgr_plan_s();
+  plan->grid_mult = sc.grid_mult;
   plan->block_tile_begin.resize(nblocks);
   plan->block_tile_count.resize(nblocks);
   std::vector<std::pair<int, int>> sx, sy, sz;
@@ -630,9 +693,10 @@ extern "C" int dgr_plan_create(dgr_plan_t* plan_out, const dgr_block* blocks, ui
       db.pad = 0;
       plan->ngroups = std::max<uint32_t>(plan->ngroups, bl.group + 1);
       plan->block_tile_begin[b] = (uint32_t)ht.size();
-      split(bl.sz[0] - 6, sx);
-      split(bl.sz[1] - 6, sy);
-      split(bl.sz[2] - 6, sz);
+      const int tmax = (int)sc.tile_max[size_class(bl)];
+      split(bl.sz[0] - 6, tmax, sx);
+      split(bl.sz[1] - 6, tmax, sy);
+      split(bl.sz[2] - 6, tmax, sz);
       for (auto& cz : sz)
         for (auto& cy : sy)
           for (auto& cx : sx) {
@@ -650,6 +714,11 @@ extern "C" int dgr_plan_create(dgr_plan_t* plan_out, const dgr_block* blocks, ui
   }
   plan->nblocks = nblocks;
   plan->ntiles = (uint32_t)ht.size();
+  // batched launches walk the tiles in the scheduled order; per-block
+  // launches keep the block-order table
+  std::vector<DevTile> hs = ht;
+  order_tiles(hs, 0, plan->ntiles_interior, sc.order, blocks);
+  order_tiles(hs, plan->ntiles_interior, hs.size(), sc.order, blocks);
   for (uint32_t b = 0; b < nblocks; ++b)
     plan->dof_end = std::max<uint64_t>(plan->dof_end, blocks[b].offset + (uint64_t)blocks[b].sz[0] * blocks[b].sz[1] *
                                                                            blocks[b].sz[2]);
@@ -671,13 +740,16 @@ extern "C" int dgr_plan_create(dgr_plan_t* plan_out, const dgr_block* blocks, ui
   plan->grid = ncu;
   if (hipMalloc(&plan->d_blocks, sizeof(DevBlock) * nblocks) != hipSuccess ||
       hipMalloc(&plan->d_tiles, sizeof(DevTile) * ht.size()) != hipSuccess ||
+      hipMalloc(&plan->d_tiles_blk, sizeof(DevTile) * ht.size()) != hipSuccess ||
       hipMalloc(&plan->d_io, sizeof(IOPtrs)) != hipSuccess ||
       hipMalloc(&plan->d_nbr, sizeof(int32_t) * plan->nbr.size()) != hipSuccess) {
     free_plan(plan);
     return DGR_E_NOMEM;
   }
   e = hipMemcpy(plan->d_blocks, hb.data(), sizeof(DevBlock) * nblocks, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(plan->d_tiles, ht.data(), sizeof(DevTile) * ht.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(plan->d_tiles, hs.data(), sizeof(DevTile) * hs.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(plan->d_tiles_blk, ht.data(), sizeof(DevTile) * ht.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(plan->d_nbr, plan->nbr.data(), sizeof(int32_t) * plan->nbr.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -720,12 +792,13 @@ int upload_ptrs(dgr_plan_t plan, double* const* rhs, const double* const* uin, h
 int dgr_internal::launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st) {
   const int ni = (int)plan->ntiles_interior;
   const int nbd = (int)plan->ntiles - ni;
+  auto grid = [&](int n) { return plan->grid_mult == 0 ? n : std::min(plan->grid * (int)plan->grid_mult, n); };
   if (ni > 0)
-    hipLaunchKernelGGL(k_bssn_rhs<false>, dim3(std::min(plan->grid, ni)), dim3(NT), 0, st, plan->d_blocks,
-                       plan->d_tiles, 0, ni, d_io, plan->params);
+    hipLaunchKernelGGL(k_bssn_rhs<false>, dim3(grid(ni)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, 0, ni,
+                       d_io, plan->params);
   if (nbd > 0)
-    hipLaunchKernelGGL(k_bssn_rhs<true>, dim3(std::min(plan->grid, nbd)), dim3(NT), 0, st, plan->d_blocks,
-                       plan->d_tiles, ni, nbd, d_io, plan->params);
+    hipLaunchKernelGGL(k_bssn_rhs<true>, dim3(grid(nbd)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, ni, nbd,
+                       d_io, plan->params);
   DGR_HIP(hipGetLastError());
   return DGR_OK;
 }
@@ -748,10 +821,10 @@ extern "C" int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const
     const int tb = (int)plan->block_tile_begin[b];
     const int nt = (int)plan->block_tile_count[b];
     if (tb < (int)plan->ntiles_interior)
-      hipLaunchKernelGGL(k_bssn_rhs<false>, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, tb, nt,
+      hipLaunchKernelGGL(k_bssn_rhs<false>, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, tb, nt,
                          plan->d_io, plan->params);
     else
-      hipLaunchKernelGGL(k_bssn_rhs<true>, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, tb, nt,
+      hipLaunchKernelGGL(k_bssn_rhs<true>, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, tb, nt,
                          plan->d_io, plan->params);
   }
   DGR_HIP(hipGetLastError());

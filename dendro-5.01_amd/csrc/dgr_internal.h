@@ -79,10 +79,12 @@ struct dgr_plan_s {
   uint32_t ntiles = 0;
   uint64_t points = 0;
   dgr_internal::DevBlock* d_blocks = nullptr;
-  dgr_internal::DevTile* d_tiles = nullptr;
+  dgr_internal::DevTile* d_tiles = nullptr;      // scheduled order (batched launches)
+  dgr_internal::DevTile* d_tiles_blk = nullptr;  // block order (per-block launches)
   uint32_t ntiles_interior = 0;           // tiles [0, n) belong to blocks with bflag 0
-  std::vector<uint32_t> block_tile_begin;  // first tile of block b
+  std::vector<uint32_t> block_tile_begin;  // first tile of block b in d_tiles_blk
   std::vector<uint32_t> block_tile_count;
+  uint32_t grid_mult = 1;                 // persistent grid = grid_mult x CUs; 0: one workgroup per tile
   dgr_internal::DevParams params{};
   int grid = 0;                          // persistent grid size
   dgr_internal::IOPtrs* d_io = nullptr;  // device copy of the 48 array pointers

@@ -290,14 +290,14 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
     const Work W = {r->buf, r->dof};
     const dim3 g((nt + chunk - 1) / chunk), b(384);
     if (s == 0)
-      hipLaunchKernelGGL(k_stage<ST_FIRST>, g, b, 0, st, p->d_blocks, p->d_tiles, nt, chunk, r->state, W, r->d_gdt,
-                         p->chi_floor, s, r->d_flag);
+      hipLaunchKernelGGL(k_stage<ST_FIRST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, chunk, r->state, W,
+                         r->d_gdt, p->chi_floor, s, r->d_flag);
     else if (s < 3)
-      hipLaunchKernelGGL(k_stage<ST_MID>, g, b, 0, st, p->d_blocks, p->d_tiles, nt, chunk, r->state, W, r->d_gdt,
-                         p->chi_floor, s, r->d_flag);
+      hipLaunchKernelGGL(k_stage<ST_MID>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, chunk, r->state, W,
+                         r->d_gdt, p->chi_floor, s, r->d_flag);
     else
-      hipLaunchKernelGGL(k_stage<ST_LAST>, g, b, 0, st, p->d_blocks, p->d_tiles, nt, chunk, r->state, W, r->d_gdt,
-                         p->chi_floor, s, r->d_flag);
+      hipLaunchKernelGGL(k_stage<ST_LAST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, chunk, r->state, W,
+                         r->d_gdt, p->chi_floor, s, r->d_flag);
     hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(p, (int)p->nblocks)), dim3(RK_NT), 0, st, p->d_blocks,
                        p->d_nbr, (int)p->nblocks, s < 3 ? r->stg : r->state);
   }
@@ -371,7 +371,7 @@ extern "C" int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* 
   unsigned* d_flag = t.d;
   DGR_HIP(hipMemsetAsync(d_flag, 0, sizeof(unsigned), st));
   const int nt = (int)plan->ntiles;
-  hipLaunchKernelGGL(k_enforce, dim3(grid_for(plan, nt)), dim3(384), 0, st, plan->d_blocks, plan->d_tiles, nt, h,
+  hipLaunchKernelGGL(k_enforce, dim3(grid_for(plan, nt)), dim3(384), 0, st, plan->d_blocks, plan->d_tiles_blk, nt, h,
                      plan->chi_floor, d_flag);
   DGR_HIP(hipGetLastError());
   unsigned flag = 0;

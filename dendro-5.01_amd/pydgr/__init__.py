@@ -1,5 +1,5 @@
 """Python side of the MI355X BSSN RHS library: ctypes binding of the C ABI
 (include/dgr_bssn.h), block tables and seeded synthetic data."""
-from .native import (NUM_VARS, VAR_NAMES, BSSNParams, Block, DgrError, Plan, RK4, lib, make_params,  # noqa: F401
-                     rhs_blocks)
+from .native import (NUM_VARS, VAR_NAMES, BSSNParams, Block, DgrError, Plan, RK4, Schedule,  # noqa: F401
+                     default_schedule, lib, make_params, rhs_blocks)
 from . import layout, data  # noqa: F401

@@ -45,6 +45,38 @@ class BSSNParams(ctypes.Structure):
                 ("reserved", ctypes.c_uint32)]
 
 
+SIZE_CLASSES = 4          # padded block extent <= 13, <= 19, <= 25, larger
+ORDER_BLOCK, ORDER_XCD, ORDER_LARGEST = 0, 1, 2
+
+
+class Schedule(ctypes.Structure):
+    """dgr_schedule: tile extent per size class, tile order, grid size."""
+    _fields_ = [("tile_max", ctypes.c_uint32 * SIZE_CLASSES), ("order", ctypes.c_uint32),
+                ("grid_mult", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {"tile_max": list(self.tile_max), "order": int(self.order), "grid_mult": int(self.grid_mult)}
+
+    @classmethod
+    def from_dict(cls, d):
+        s = cls()
+        s.tile_max[:] = [int(x) for x in d["tile_max"]]
+        s.order = int(d["order"])
+        s.grid_mult = int(d["grid_mult"])
+        return s
+
+
+def default_schedule():
+    s = Schedule()
+    lib().dgr_schedule_default(ctypes.byref(s))
+    return s
+
+
+def size_class(sz):
+    m = max(sz)
+    return 0 if m <= 13 else 1 if m <= 19 else 2 if m <= 25 else 3
+
+
 def make_params(lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0), eta_const=2.0, eta_r0=30.0, eta_damping_exp=1.0,
                 ko_sigma=0.4, chi_floor=0.1, deriv_order=42):
     p = BSSNParams()
@@ -76,6 +108,10 @@ def lib():
         L.dgr_init.argtypes = [ctypes.c_int]
         L.dgr_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Block), ctypes.c_uint32,
                                       ctypes.POINTER(BSSNParams)]
+        L.dgr_plan_create_ex.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Block), ctypes.c_uint32,
+                                         ctypes.POINTER(BSSNParams), ctypes.POINTER(Schedule)]
+        L.dgr_schedule_default.argtypes = [ctypes.POINTER(Schedule)]
+        L.dgr_schedule_default.restype = None
         L.dgr_plan_destroy.argtypes = [vp]
         L.dgr_plan_points.restype = ctypes.c_uint64
         L.dgr_plan_points.argtypes = [vp]
@@ -150,13 +186,16 @@ def _stream_ptr(stream):
 class Plan:
     """Device-resident block table + tile work table (dgr_plan_create)."""
 
-    def __init__(self, blocks, params, device=0):
+    def __init__(self, blocks, params, device=0, schedule=None):
         L = lib()
         check(L.dgr_init(device), "dgr_init")
         self._barr = blocks_array(blocks)
         self.params = params
+        self.schedule = schedule
         h = ctypes.c_void_p()
-        check(L.dgr_plan_create(ctypes.byref(h), self._barr, len(blocks), ctypes.byref(params)), "dgr_plan_create")
+        sp = ctypes.byref(schedule) if schedule is not None else None
+        check(L.dgr_plan_create_ex(ctypes.byref(h), self._barr, len(blocks), ctypes.byref(params), sp),
+              "dgr_plan_create_ex")
         self.handle = h
         self.nblocks = len(blocks)
 

@@ -72,6 +72,25 @@ typedef struct dgr_bssn_params {
 /* Opaque plan: the block table and the tile work table on the device. */
 typedef struct dgr_plan_s* dgr_plan_t;
 
+/* How a plan cuts blocks into tiles and orders them for the persistent
+ * kernel; chosen per workload by the autotuner (pydgr.autotune).  No
+ * counterpart in the reference, whose GPU path launches a fixed thread
+ * block per unzipped block (BSSN_GR/cuda_gr/src/rhs_cuda.cu:35-143).
+ * Size classes by the largest padded extent of a block:
+ *   0: <= 13, 1: <= 19, 2: <= 25, 3: larger. */
+#define DGR_SIZE_CLASSES 4
+enum {
+  DGR_ORDER_BLOCK = 0,   /* tiles in block order */
+  DGR_ORDER_XCD = 1,     /* a block's tiles on one XCD (shared L2 for the halos) */
+  DGR_ORDER_LARGEST = 2  /* tiles of the largest blocks first */
+};
+typedef struct dgr_schedule {
+  uint32_t tile_max[DGR_SIZE_CLASSES]; /* largest tile extent per axis, 3..7 (default 7) */
+  uint32_t order;                      /* DGR_ORDER_* (default DGR_ORDER_BLOCK) */
+  uint32_t grid_mult;                  /* persistent grid = grid_mult x CUs; 0 = one workgroup per tile (default 1) */
+} dgr_schedule;
+void dgr_schedule_default(dgr_schedule* s);
+
 const char* dgr_error_string(int code);
 
 /* Select the device for this thread and load the kernels. */
@@ -83,6 +102,10 @@ int dgr_init(int device);
  * Blocks need sz >= 13 in every direction. */
 int dgr_plan_create(dgr_plan_t* plan, const dgr_block* blocks, uint32_t nblocks,
                     const dgr_bssn_params* params);
+/* Same with an explicit schedule (NULL = default); results are identical
+ * for every schedule, only the speed differs. */
+int dgr_plan_create_ex(dgr_plan_t* plan, const dgr_block* blocks, uint32_t nblocks,
+                       const dgr_bssn_params* params, const dgr_schedule* sched);
 int dgr_plan_destroy(dgr_plan_t plan);
 /* Interior points covered by the plan. */
 uint64_t dgr_plan_points(dgr_plan_t plan);
