@@ -270,27 +270,32 @@ extern "C" void orc_bssnrhs(double* const* rhs, const double* const* uz, uint64_
   for (int v = 0; v < 24; ++v) g2slot[v] = -1;
   for (int k = 0; k < 11; ++k) g2slot[kGrad2Vars[k]] = k;
 
+  // operator set: deriv42 (BSSN_GR/include/rhs.h:23-37) or the 6th-order
+  // option deriv64 / deriv64adv (= centred deriv64) with ko_deriv42
+  // (QuadGrav/include/derivs.h:80-96)
+  const bool six = p->deriv_order == 64;
+  const int kFirst = six ? FIRST6 : FIRST, kSecond = six ? SECOND6 : SECOND, kAdv = six ? ADV6 : ADV;
   // first and pure second derivatives (bssnrhs_derivs.h:1-105)
   for (int d = 0; d < 3; ++d)
     for (int v = 0; v < 24; ++v) {
-      orc_deriv(FIRST, d, G(d, v), u[v], h[d], sz, bflag, nullptr);
+      orc_deriv(kFirst, d, G(d, v), u[v], h[d], sz, bflag, nullptr);
       if (g2slot[v] >= 0) {
         const int pair = d == 0 ? 0 : (d == 1 ? 3 : 5);
-        orc_deriv(SECOND, d, G2(pair, g2slot[v]), u[v], h[d], sz, bflag, nullptr);
+        orc_deriv(kSecond, d, G2(pair, g2slot[v]), u[v], h[d], sz, bflag, nullptr);
       }
     }
   // mixed derivatives by composition (bssnrhs_derivs.h:106-138):
   // d01 = d_y(d_x f), d02 = d_z(d_x f), d12 = d_z(d_y f)
   for (int k = 0; k < 11; ++k) {
     const int v = kGrad2Vars[k];
-    orc_deriv(FIRST, 1, G2(1, k), G(0, v), hy, sz, bflag, nullptr);
-    orc_deriv(FIRST, 2, G2(2, k), G(0, v), hz, sz, bflag, nullptr);
-    orc_deriv(FIRST, 2, G2(4, k), G(1, v), hz, sz, bflag, nullptr);
+    orc_deriv(kFirst, 1, G2(1, k), G(0, v), hy, sz, bflag, nullptr);
+    orc_deriv(kFirst, 2, G2(2, k), G(0, v), hz, sz, bflag, nullptr);
+    orc_deriv(kFirst, 2, G2(4, k), G(1, v), hz, sz, bflag, nullptr);
   }
   // upwinded derivatives (bssnrhs_derivs_adv.h)
   const double* beta[3] = {u[BETA0], u[BETA1], u[BETA2]};
   for (int d = 0; d < 3; ++d)
-    for (int v = 0; v < 24; ++v) orc_deriv(ADV, d, AG(d, v), u[v], h[d], sz, bflag, beta[d]);
+    for (int v = 0; v < 24; ++v) orc_deriv(kAdv, d, AG(d, v), u[v], h[d], sz, bflag, beta[d]);
 
   // per-input pointer table in bssn_point_io.h order
   const double* D[DGR_POINT_NIN];

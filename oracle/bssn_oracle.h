@@ -35,6 +35,9 @@ typedef struct {
   double eta_r0;      /* ETA_R0 */
   double eta_damping_exp; /* ETA_DAMPING_EXP */
   double ko_sigma;    /* KO_DISS_SIGMA */
+  int deriv_order;    /* 42: deriv42 (BSSN_GR/include/rhs.h:23-37); 64: deriv64 first/second/
+                         advective with ko_deriv42 (QuadGrav/include/derivs.h:80-96) */
+  int pad;
 } orc_params;
 
 /* Single-operator restatements (derivs.cpp).  axis: 0=x, 1=y, 2=z.

@@ -18,12 +18,14 @@ dp = ctypes.POINTER(ctypes.c_double)
 class Params(ctypes.Structure):
     _fields_ = [("lambda_", ctypes.c_double * 4), ("lambda_f", ctypes.c_double * 2),
                 ("eta_const", ctypes.c_double), ("eta_r0", ctypes.c_double),
-                ("eta_damping_exp", ctypes.c_double), ("ko_sigma", ctypes.c_double)]
+                ("eta_damping_exp", ctypes.c_double), ("ko_sigma", ctypes.c_double),
+                ("deriv_order", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
 def params(lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0), eta_const=2.0, eta_r0=30.0, eta_damping_exp=1.0,
-           ko_sigma=0.4):
+           ko_sigma=0.4, deriv_order=42):
     p = Params()
+    p.deriv_order = deriv_order
     p.lambda_[:] = [float(x) for x in lambda_]
     p.lambda_f[:] = [float(x) for x in lambda_f]
     p.eta_const, p.eta_r0, p.eta_damping_exp, p.ko_sigma = eta_const, eta_r0, eta_damping_exp, ko_sigma

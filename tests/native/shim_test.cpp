@@ -88,6 +88,7 @@ int main() {
   for (int i = 0; i < 4; ++i) op.lambda[i] = 1.0;
   op.lambda_f[0] = 1.0; op.lambda_f[1] = 0.0;
   op.eta_const = 2.0; op.eta_r0 = 30.0; op.eta_damping_exp = 1.0; op.ko_sigma = 0.4;
+  op.deriv_order = 42; op.pad = 0;
   for (int v = 0; v < 24; ++v) rp[v] = r_orc[v].data();
   for (const auto& b : blocks) orc_bssnrhs(rp, up, b.offset, b.ptmin, b.ptmax, b.sz, b.bflag, &op);
 

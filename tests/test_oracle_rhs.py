@@ -23,10 +23,11 @@ def coords_np(blocks):
     return x.numpy(), y.numpy(), z.numpy(), idx.numpy()
 
 
-def test_flat_space_is_exactly_static():
+@pytest.mark.parametrize("order", [42, 64])
+def test_flat_space_is_exactly_static(order):
     blocks, dof = layout.uniform_grid(2, dx=1.0 / 64)
     U = data.flat_bumps(blocks, dof, amp=0.0).numpy()
-    R = po.rhs_blocks(U, blocks, fill=0.0)
+    R = po.rhs_blocks(U, blocks, po.params(deriv_order=order), fill=0.0)
     assert np.abs(R).max() == 0.0
 
 
@@ -41,7 +42,8 @@ def test_padding_untouched():
     assert np.all(np.isfinite(R[:, ii]))
 
 
-def test_manufactured_shift_terms():
+@pytest.mark.parametrize("order", [42, 64])
+def test_manufactured_shift_terms(order):
     """gt = delta, chi = alpha = 1, K = At = Gt = B = 0 and a bilinear shift
     beta = s (xy, yz, zx).  Then (CodeGen/bssn.py:60-82)
       b_rhs^i  = beta^k d_k beta^i
@@ -56,7 +58,7 @@ def test_manufactured_shift_terms():
     x, y, z, idx = coords_np(blocks)
     U = data.flat_bumps(blocks, dof, amp=0.0).numpy()
     U[6, idx], U[7, idx], U[8, idx] = s * x * y, s * y * z, s * z * x
-    R = po.rhs_blocks(U, blocks, po.params(ko_sigma=0.0), fill=0.0)
+    R = po.rhs_blocks(U, blocks, po.params(ko_sigma=0.0, deriv_order=order), fill=0.0)
     ii = layout.interior_index(blocks)
     X, Y, Z = x[ii], y[ii], z[ii]
     b = [s * X * Y, s * Y * Z, s * Z * X]
@@ -73,7 +75,8 @@ def test_manufactured_shift_terms():
     np.testing.assert_allclose(R[:, ii], want, atol=1e-13)
 
 
-def test_manufactured_lapse_terms():
+@pytest.mark.parametrize("order", [42, 64])
+def test_manufactured_lapse_terms(order):
     """alpha a cubic, K constant, beta constant, flat metric:
     a_rhs = lambda0 beta.grad(alpha) - 2 alpha K, K_rhs = -lap(alpha) + alpha K^2/3."""
     blocks, dof = layout.uniform_grid(2, dx=1.0 / 64, outer_bflags=False)
@@ -85,7 +88,7 @@ def test_manufactured_lapse_terms():
     beta = (0.1, -0.2, 0.05)
     for k in range(3):
         U[6 + k, idx] = beta[k]
-    R = po.rhs_blocks(U, blocks, po.params(ko_sigma=0.0), fill=0.0)
+    R = po.rhs_blocks(U, blocks, po.params(ko_sigma=0.0, deriv_order=order), fill=0.0)
     ii = layout.interior_index(blocks)
     X, Y, Z = x[ii], y[ii], z[ii]
     A = 1.0 + 0.01 * (X ** 3 + 2 * X * Y * Z - Y ** 2 + 0.5 * Z)
@@ -95,26 +98,46 @@ def test_manufactured_lapse_terms():
     np.testing.assert_allclose(R[2, ii], -lap + A * 0.04 / 3.0, atol=1e-12)
 
 
+def test_sixth_order_option_changes_operators():
+    """deriv_order 64 switches the operator set: same data, different RHS,
+    and for a degree-7 lapse only the 6th-order K_rhs is exact."""
+    blocks, dof = layout.uniform_grid(2, dx=1.0 / 16, outer_bflags=False)
+    x, y, z, idx = coords_np(blocks)
+    U = data.flat_bumps(blocks, dof, amp=0.0).numpy()
+    U[0, idx] = 1.0 + 0.01 * x ** 7
+    ii = layout.interior_index(blocks)
+    X = x[ii]
+    want = -0.01 * 42 * X ** 5  # K_rhs = -lap(alpha) with K = 0
+    got = {o: po.rhs_blocks(U, blocks, po.params(ko_sigma=0.0, deriv_order=o), fill=0.0)[2, ii] for o in (42, 64)}
+    err42 = np.abs(got[42] - want).max()
+    err64 = np.abs(got[64] - want).max()
+    assert err64 < 1e-11 < err42
+
+
 PERM_XY = {3: 4, 4: 3, 6: 7, 7: 6, 9: 10, 10: 9, 12: 15, 15: 12, 14: 16, 16: 14, 18: 21, 21: 18, 20: 22, 22: 20}
 
 
-def test_axis_swap_symmetry():
+@pytest.mark.parametrize("order", [42, 64])
+def test_axis_swap_symmetry(order):
     """Swapping x and y in the data (including vector/tensor components and
     the boundary flags) swaps them in the right-hand side."""
     blocks, dof = layout.uniform_grid(1, dx=1.0 / 64)
     blocks[0]["bflag"] = 0b000101  # -x, -y faces
     U = data.flat_bumps(blocks, dof, seed=3, amp=2e-2, width=0.2).numpy()
     # mirror the beta signs too so the upwind choice is exercised both ways
-    R = po.rhs_blocks(U, blocks, fill=0.0)
+    prm = po.params(deriv_order=order)
+    R = po.rhs_blocks(U, blocks, prm, fill=0.0)
     sz = 13
     perm = [PERM_XY.get(v, v) for v in range(24)]
     U2 = U[perm].reshape(24, sz, sz, sz).transpose(0, 1, 3, 2).reshape(24, -1).copy()
     b2 = [dict(blocks[0])]
     b2[0]["bflag"] = 0b000101
-    R2 = po.rhs_blocks(U2, b2, fill=0.0)
+    R2 = po.rhs_blocks(U2, b2, prm, fill=0.0)
     R2b = R2.reshape(24, sz, sz, sz).transpose(0, 1, 3, 2).reshape(24, -1)[perm]
     ii = layout.interior_index(blocks)
-    assert rel_err_per_var(R2b, R, ii).max() < 1e-13
+    # mixed derivatives compose d_y(d_x u) in both runs, i.e. d_x(d_y u) of
+    # the swapped data: equal up to rounding, larger for the 6th-order weights
+    assert rel_err_per_var(R2b, R, ii).max() < (1e-13 if order == 42 else 1e-12)
 
 
 def test_sommerfeld_faces():

@@ -18,7 +18,7 @@ import pytest
 
 import pyoracle as po
 
-FIRST, SECOND, ADV, KO = 0, 1, 2, 3
+FIRST, SECOND, ADV, KO, FIRST6, SECOND6, ADV6 = 0, 1, 2, 3, 4, 5, 6
 SZ = 13
 E = SZ - 3
 SMR3, SMR2, SMR1 = 59.0 / 48.0 * 64, 43.0 / 48.0 * 64, 49.0 / 48.0 * 64  # derivs.cpp:918-925, h = 1
@@ -135,7 +135,45 @@ def test_kreiss_oliger_rows(axis):
         assert abs(mom[3] + m3) < 1e-14
 
 
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_sixth_order_first_rows(axis):
+    """deriv64_x/y/z (QuadGrav/src/derivs.cpp:2141-2420): 6th-order centred
+    interior, 4th-order one-sided rows at the three points next to a
+    flagged face; deriv64adv is the same operator whatever the shift."""
+    for q in range(6, E - 3):
+        check_row(coeffs(FIRST6, axis, q), q, [q - 3, q - 2, q - 1, q + 1, q + 2, q + 3], 6, 1)
+    bf = LOW[axis] | HIGH[axis]
+    check_row(coeffs(FIRST6, axis, 3, bf), 3, [3, 4, 5, 6, 7], 4, 1)
+    check_row(coeffs(FIRST6, axis, 4, bf), 4, [3, 4, 5, 6, 7], 4, 1)
+    check_row(coeffs(FIRST6, axis, 5, bf), 5, [3, 4, 6, 7], 4, 1)
+    check_row(coeffs(FIRST6, axis, E - 3, bf), E - 3, [E - 5, E - 4, E - 2, E - 1], 4, 1)
+    check_row(coeffs(FIRST6, axis, E - 2, bf), E - 2, list(range(E - 5, E)), 4, 1)
+    check_row(coeffs(FIRST6, axis, E - 1, bf), E - 1, list(range(E - 5, E)), 4, 1)
+    check_row(coeffs(FIRST6, axis, 3), 3, [0, 1, 2, 4, 5, 6], 6, 1)
+    for q in (3, 5, 7, E - 1):
+        for b in (0.7, -0.7, 0.0):
+            np.testing.assert_array_equal(coeffs(ADV6, axis, q, bf, b), coeffs(FIRST6, axis, q, bf))
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_sixth_order_second_rows(axis):
+    """deriv64_xx/yy/zz (QuadGrav/src/derivs.cpp:2569-2910)."""
+    for q in range(6, E - 3):
+        check_row(coeffs(SECOND6, axis, q), q, list(range(q - 3, q + 4)), 7, 2)
+    bf = LOW[axis] | HIGH[axis]
+    check_row(coeffs(SECOND6, axis, 3, bf), 3, [3, 4, 5, 6, 7], 4, 2)
+    check_row(coeffs(SECOND6, axis, 4, bf), 4, [3, 4, 5, 6, 7], 4, 2)
+    check_row(coeffs(SECOND6, axis, 5, bf), 5, [3, 4, 5, 6, 7], 5, 2)
+    check_row(coeffs(SECOND6, axis, E - 3, bf), E - 3, list(range(E - 5, E)), 5, 2)
+    check_row(coeffs(SECOND6, axis, E - 2, bf), E - 2, list(range(E - 5, E)), 4, 2)
+    check_row(coeffs(SECOND6, axis, E - 1, bf), E - 1, list(range(E - 5, E)), 4, 2)
+
+
 @pytest.mark.parametrize("kind,axis,ranges", [
+    (FIRST6, 0, [(3, SZ - 3), (0, SZ), (0, SZ)]),       # QuadGrav/src/derivs.cpp:2151-2158
+    (FIRST6, 1, [(3, SZ - 3), (3, SZ - 3), (0, SZ)]),   # :2293-2300
+    (FIRST6, 2, [(3, SZ - 3)] * 3),
+    (SECOND6, 1, [(3, SZ - 3)] * 3),
     (FIRST, 0, [(3, SZ - 3), (1, SZ - 1), (1, SZ - 1)]),   # derivs.cpp:19-27
     (FIRST, 1, [(3, SZ - 3), (3, SZ - 3), (1, SZ - 1)]),   # derivs.cpp:96-104
     (FIRST, 2, [(3, SZ - 3)] * 3),
