@@ -53,9 +53,22 @@ constexpr int NT = 384;                        // 6 waves
 constexpr int SX = TX + 6, SY = TY + 6, SZ = TZ + 6;  // staged region
 constexpr int NSU = SX * SY * SZ;              // 2197
 constexpr int NPF = (NSU + NT - 1) / NT;       // prefetch registers per thread
-constexpr int GXY = TY + 4, GXZ = TZ + 4;      // widened x-derivative plane
-constexpr int NGX = TX * GXY * GXZ;            // 847
-constexpr int NGY = TX * TY * GXZ;             // 539
+// Derivative planes for the mixed derivatives by composition: d_x u over
+// the tile widened by GW points in y and z, d_y u widened by GW in z.  The
+// reference computes the first derivatives on the widened ranges the second
+// differentiation reads: +-2 for deriv42 (BSSN_GR/src/derivs.cpp:19-27,
+// 96-104), +-3 for deriv64 (QuadGrav/src/derivs.cpp:2151-2158, 2293-2300).
+// The 6th-order planes do not both fit next to the per-point slots, so they
+// share one buffer and are built one after the other.
+template <int ORD>
+struct Planes {
+  static constexpr int GW = ORD == 64 ? 3 : 2;
+  static constexpr int GXY = TY + 2 * GW, GXZ = TZ + 2 * GW;
+  static constexpr int NGX = TX * GXY * GXZ;   // 847 / 1183
+  static constexpr int NGY = TX * TY * GXZ;    // 539 / 637
+  static constexpr bool SHARED = ORD == 64;
+  static constexpr int N = SHARED ? (NGX > NGY ? NGX : NGY) : NGX + NGY;
+};
 
 // ---------------------------------------------------------------------------
 // Stencils.  The value at axis coordinate q' of the current line is
@@ -68,23 +81,25 @@ constexpr int NGY = TX * TY * GXZ;             // 539
 // rows of stencil_forms.h (one code path instead of a branch per case; the
 // rows reproduce the closures bit for bit).
 // ---------------------------------------------------------------------------
-__constant__ dgr_forms::Form c_forms[dgr_forms::F_NFORMS] = DGR_FORMS_TABLE;
+__constant__ dgr_forms::Form c_forms42[dgr_forms::F_NFORMS] = DGR_FORMS_TABLE;
+__constant__ dgr_forms::Form c_forms64[dgr_forms::G_NFORMS] = DGR_FORMS64_TABLE;
+constexpr int NFORMS_MAX = dgr_forms::F_NFORMS > dgr_forms::G_NFORMS ? dgr_forms::F_NFORMS : dgr_forms::G_NFORMS;
 
 // The variable arrays are global memory; saying so turns generic (flat)
 // accesses into global ones, which do not hold up waits on LDS reads.
 typedef __attribute__((address_space(1))) double gdouble;
 __device__ __forceinline__ const gdouble* gmem(const double* p) { return (const gdouble*)p; }
 __device__ __forceinline__ gdouble* gmem(double* p) { return (gdouble*)p; }
-__shared__ dgr_forms::Form s_forms[dgr_forms::F_NFORMS];  // boundary kernel only
+__shared__ dgr_forms::Form s_forms[NFORMS_MAX];  // boundary kernels only: the order's table
 
-// Closure row f over the 7-point window centred at A[c], scaled.  Only
+// Closure row f over the 9-point window centred at A[c], scaled.  Only
 // points with a nonzero coefficient are read (a closure never reaches past
 // its face, where a derivative plane may end).
 __device__ __forceinline__ double eval_form(int f, const double* A, int c, int s, const AxisC& x) {
   const dgr_forms::Form& F = s_forms[f];
-  double w[7];
+  double w[dgr_forms::W];
 #pragma unroll
-  for (int m = 0; m < 7; ++m) w[m] = F.c[m] != 0.0 ? A[c + (m - 3) * s] : 0.0;
+  for (int m = 0; m < dgr_forms::W; ++m) w[m] = F.c[m] != 0.0 ? A[c + (m - 4) * s] : 0.0;
   const double sum = F.desc ? dgr_forms::form_sum_down(F.c, w) : dgr_forms::form_sum_up(F.c, w);
   return dgr_forms::form_scale(sum, F.sel, &x.ih2);
 }
@@ -94,14 +109,34 @@ __device__ __forceinline__ bool near_face(int q, const AxisC& x) {
   return (x.lf && q <= 5) || (x.hf && q >= x.e - 3);
 }
 
-template <bool BND>
-__device__ __forceinline__ double d1_42(const double* A, int o, int s, int q, const AxisC& x) {
+// Interior first / second derivatives over the window m3..p3 = A[q-3..q+3]:
+// deriv42_x / deriv42_xx (BSSN_GR/src/derivs.cpp:29-65, 265-316) or
+// deriv64_x / deriv64_xx (QuadGrav/src/derivs.cpp:2161-2170, 2585-2596).
+template <int ORD>
+__device__ __forceinline__ double d1_int(double m3, double m2, double m1, double p1, double p2, double p3,
+                                         const AxisC& x) {
 #pragma clang fp contract(off)
+  if constexpr (ORD == 64) return (-m3 + 9.0 * m2 - 45.0 * m1 + 45.0 * p1 - 9.0 * p2 + p3) * x.ih60;
+  return (m2 - 8.0 * m1 + 8.0 * p1 - p2) * x.ih12;
+}
+template <int ORD>
+__device__ __forceinline__ double d2_int(double m3, double m2, double m1, double z0, double p1, double p2,
+                                         double p3, const AxisC& x) {
+#pragma clang fp contract(off)
+  if constexpr (ORD == 64)
+    return (2.0 * m3 - 27.0 * m2 + 270.0 * m1 - 490.0 * z0 + 270.0 * p1 - 27.0 * p2 + 2.0 * p3) * x.isq180;
+  return (-m2 + 16.0 * m1 - 30.0 * z0 + 16.0 * p1 - p2) * x.isq12;
+}
+
+template <bool BND, int ORD>
+__device__ __forceinline__ double d1_op(const double* A, int o, int s, int q, const AxisC& x) {
   const int c = o + q * s;
-  double r = (A[c - 2 * s] - 8.0 * A[c - s] + 8.0 * A[c + s] - A[c + 2 * s]) * x.ih12;
+  const double m2 = A[c - 2 * s], m1 = A[c - s], p1 = A[c + s], p2 = A[c + 2 * s];
+  const double m3 = ORD == 64 ? A[c - 3 * s] : 0.0, p3 = ORD == 64 ? A[c + 3 * s] : 0.0;
+  double r = d1_int<ORD>(m3, m2, m1, p1, p2, p3, x);
   if (BND && near_face(q, x)) {
-    const int f = dgr_forms::form_d1(q, x.e, x.lf, x.hf);
-    if (f != dgr_forms::F_D1) r = eval_form(f, A, c, s, x);
+    const int f = ORD == 64 ? dgr_forms::form64_d1(q, x.e, x.lf, x.hf) : dgr_forms::form_d1(q, x.e, x.lf, x.hf);
+    if (f != (ORD == 64 ? (int)dgr_forms::G_D1 : (int)dgr_forms::F_D1)) r = eval_form(f, A, c, s, x);
   }
   return r;
 }
@@ -211,37 +246,48 @@ using KPoint = dgr::PointT<LdsStore>;
 // registers and every operator is evaluated from it with the reference's
 // operation order (derivs.cpp); in boundary blocks the closure-aware forms
 // above are used.
-template <bool BND, bool D1, bool D2>
+template <bool BND, bool D1, bool D2, int ORD>
 __device__ __forceinline__ void axis_ops(const double* A, int o, int s, int q, const AxisC& x, double beta,
                                          double& d1, double& d2, double& ad, double& ko) {
 #pragma clang fp contract(off)
   const int c = o + q * s;
   const double m3 = A[c - 3 * s], m2 = A[c - 2 * s], m1 = A[c - s], z0 = A[c], p1 = A[c + s], p2 = A[c + 2 * s],
                p3 = A[c + 3 * s];
-  if constexpr (D1) d1 = (m2 - 8.0 * m1 + 8.0 * p1 - p2) * x.ih12;
-  if constexpr (D2) d2 = (-m2 + 16.0 * m1 - 30.0 * z0 + 16.0 * p1 - p2) * x.isq12;
-  if (beta > 0.0)
+  if constexpr (D1) d1 = d1_int<ORD>(m3, m2, m1, p1, p2, p3, x);
+  if constexpr (D2) d2 = d2_int<ORD>(m3, m2, m1, z0, p1, p2, p3, x);
+  if constexpr (ORD == 64) {
+    // deriv64adv is the centred deriv64 (QuadGrav/src/derivs.cpp:2913-2928)
+    ad = D1 ? d1 : d1_int<ORD>(m3, m2, m1, p1, p2, p3, x);
+  } else if (beta > 0.0) {
     ad = (-3.0 * m1 - 10.0 * z0 + 18.0 * p1 - 6.0 * p2 + p3) * x.ih12;
-  else
+  } else {
     ad = (-m3 + 6.0 * m2 - 18.0 * m1 + 10.0 * z0 + 3.0 * p1) * x.ih12;
+  }
   ko = x.pre * (-m3 + 6.0 * m2 - 15.0 * m1 + 20.0 * z0 - 15.0 * p1 + 6.0 * p2 - p3);
   if constexpr (BND) {
     // closures: one generic row evaluation per operator whose form differs
     // from the interior one at this point
     if (near_face(q, x)) {
       using namespace dgr_forms;
-      const double w[7] = {m3, m2, m1, z0, p1, p2, p3};
 #pragma unroll 1
       for (int op = D1 ? 0 : (D2 ? 1 : 2); op < 4; ++op) {
         if (op == 1 && !D2) continue;
-        const int f = op == 0 ? form_d1(q, x.e, x.lf, x.hf)
-                    : op == 1 ? form_d2(q, x.e, x.lf, x.hf)
-                    : op == 2 ? form_adv(q, x.e, x.lf, x.hf, beta > 0.0)
-                              : form_ko(q, x.e, x.lf, x.hf);
-        if (f == F_D1 || f == F_D2 || f == F_ADV_P || f == F_ADV_N || f == F_KO) continue;
-        const Form& F = s_forms[f];
-        const double sum = F.desc ? form_sum_down(F.c, w) : form_sum_up(F.c, w);
-        const double r = form_scale(sum, F.sel, &x.ih2);
+        int f;
+        bool interior;
+        if constexpr (ORD == 64) {
+          f = op == 0 || op == 2 ? form64_d1(q, x.e, x.lf, x.hf)
+            : op == 1            ? form64_d2(q, x.e, x.lf, x.hf)
+                                 : form64_ko(q, x.e, x.lf, x.hf);
+          interior = f == G_D1 || f == G_D2 || f == G_KO;
+        } else {
+          f = op == 0 ? form_d1(q, x.e, x.lf, x.hf)
+            : op == 1 ? form_d2(q, x.e, x.lf, x.hf)
+            : op == 2 ? form_adv(q, x.e, x.lf, x.hf, beta > 0.0)
+                      : form_ko(q, x.e, x.lf, x.hf);
+          interior = f == F_D1 || f == F_D2 || f == F_ADV_P || f == F_ADV_N || f == F_KO;
+        }
+        if (interior) continue;
+        const double r = eval_form(f, A, c, s, x);
         if (op == 0) d1 = r;
         else if (op == 1) d2 = r;
         else if (op == 2) ad = r;
@@ -251,9 +297,16 @@ __device__ __forceinline__ void axis_ops(const double* A, int o, int s, int q, c
   }
 }
 
-template <int V, bool BND>
+// Mixed-derivative inputs of one point, for the 6th-order path whose two
+// planes are built one after the other in a shared buffer.
+struct Mixed {
+  double d1x, d1y, dxy, dxz, dyz;
+};
+
+template <int V, bool BND, int ORD>
 __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su, const double* sgx,
-                                            const double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io) {
+                                            const double* sgy, const Mixed& M, const TileGeom& T,
+                                            const IOPtrs* __restrict__ io) {
   using namespace dgr;
   if (!C.act) return;
   const AxisC* ax = C.ax;
@@ -269,13 +322,20 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
   double d1[3] = {0.0, 0.0, 0.0};
   double d2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double av[3], kv[3];
-  axis_ops<BND, D1xy, D2>(su, ox, 1, gi, ax[0], s.beta[0], d1[0], d2[0], av[0], kv[0]);
+  axis_ops<BND, D1xy, D2, ORD>(su, ox, 1, gi, ax[0], s.beta[0], d1[0], d2[0], av[0], kv[0]);
   __builtin_amdgcn_sched_barrier(0);
-  axis_ops<BND, D1xy, D2>(su, oy, SX, gj, ax[1], s.beta[1], d1[1], d2[3], av[1], kv[1]);
+  axis_ops<BND, D1xy, D2, ORD>(su, oy, SX, gj, ax[1], s.beta[1], d1[1], d2[3], av[1], kv[1]);
   __builtin_amdgcn_sched_barrier(0);
-  axis_ops<BND, D1z, D2>(su, oz, SX * SY, gk, ax[2], s.beta[2], d1[2], d2[5], av[2], kv[2]);
+  axis_ops<BND, D1z, D2, ORD>(su, oz, SX * SY, gk, ax[2], s.beta[2], d1[2], d2[5], av[2], kv[2]);
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (D2) {
+  if constexpr (D2 && ORD == 64) {
+    d1[0] = M.d1x;
+    d1[1] = M.d1y;
+    d2[1] = M.dxy;
+    d2[2] = M.dxz;
+    d2[4] = M.dyz;
+  } else if constexpr (D2) {
+    constexpr int GXY = Planes<ORD>::GXY;
     // x and y derivatives at the point are read back from the LDS planes
     d1[0] = sgx[ti + TX * ((tj + 2) + GXY * (tk + 2))];
     d1[1] = sgy[ti + TX * (tj + TY * (tk + 2))];
@@ -283,11 +343,11 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
     // sgx[a + TX*(b + GXY*cc)] holds d_x u at block coords (i0+a, j0-2+b, k0-2+cc);
     // sgy[a + TX*(b + TY*cc)] holds d_y u at (i0+a, j0+b, k0-2+cc).
     const int oxy = ti + TX * ((2 - T.j0) + GXY * (tk + 2));
-    d2[1] = d1_42<BND>(sgx, oxy, TX, gj, ax[1]);
+    d2[1] = d1_op<BND, ORD>(sgx, oxy, TX, gj, ax[1]);
     const int oxz = ti + TX * ((tj + 2) + GXY * (2 - T.k0));
-    d2[2] = d1_42<BND>(sgx, oxz, TX * GXY, gk, ax[2]);
+    d2[2] = d1_op<BND, ORD>(sgx, oxz, TX * GXY, gk, ax[2]);
     const int oyz = ti + TX * (tj + TY * (2 - T.k0));
-    d2[4] = d1_42<BND>(sgy, oyz, TX * TY, gk, ax[2]);
+    d2[4] = d1_op<BND, ORD>(sgy, oyz, TX * TY, gk, ax[2]);
     __builtin_amdgcn_sched_barrier(0);
   }
   // upwinded derivatives contracted with beta, summed in the order of the
@@ -346,58 +406,92 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
 
 // Widened first-derivative planes for the mixed derivatives, walked in
 // their fixed LDS shapes (compile-time divisors) and masked to the tile.
-template <bool BND>
-__device__ __forceinline__ void deriv_planes(const double* su, double* sgx, double* sgy, const TileGeom& T,
-                                             const AxisC* ax, int t) {
-  // sgx: d_x u at block coords (i0+a, j0-2+b, k0-2+cc), su coords (a+3, b+1, cc+1)
-  for (int e = t; e < NGX; e += NT) {
+// sgx[a + TX*(b + GXY*cc)]: d_x u at block coords (i0+a, j0-GW+b, k0-GW+cc),
+// su coords (a+3, b+3-GW, cc+3-GW).
+template <bool BND, int ORD>
+__device__ __forceinline__ void plane_x(const double* su, double* sgx, const TileGeom& T, const AxisC* ax, int t) {
+  using G = Planes<ORD>;
+  for (int e = t; e < G::NGX; e += NT) {
     const int a = e % TX;
-    const int b = (e / TX) % GXY;
-    const int cc = e / (TX * GXY);
-    if (a < T.nx && b < T.ny + 4 && cc < T.nz + 4) {
-      const int o = (a + 3) + SX * ((b + 1) + SY * (cc + 1)) - (T.i0 + a);
-      sgx[e] = d1_42<BND>(su, o, 1, T.i0 + a, ax[0]);
+    const int b = (e / TX) % G::GXY;
+    const int cc = e / (TX * G::GXY);
+    if (a < T.nx && b < T.ny + 2 * G::GW && cc < T.nz + 2 * G::GW) {
+      const int o = (a + 3) + SX * ((b + 3 - G::GW) + SY * (cc + 3 - G::GW)) - (T.i0 + a);
+      sgx[e] = d1_op<BND, ORD>(su, o, 1, T.i0 + a, ax[0]);
     }
   }
-  // sgy: d_y u at block coords (i0+a, j0+b, k0-2+cc), su coords (a+3, b+3, cc+1)
-  for (int e = t; e < NGY; e += NT) {
+}
+// sgy[a + TX*(b + TY*cc)]: d_y u at (i0+a, j0+b, k0-GW+cc), su coords
+// (a+3, b+3, cc+3-GW).
+template <bool BND, int ORD>
+__device__ __forceinline__ void plane_y(const double* su, double* sgy, const TileGeom& T, const AxisC* ax, int t) {
+  using G = Planes<ORD>;
+  for (int e = t; e < G::NGY; e += NT) {
     const int a = e % TX;
     const int b = (e / TX) % TY;
     const int cc = e / (TX * TY);
-    if (a < T.nx && b < T.ny && cc < T.nz + 4) {
-      const int o = (a + 3) + SX * ((b + 3) + SY * (cc + 1)) - (T.j0 + b) * SX;
-      sgy[e] = d1_42<BND>(su, o, SX, T.j0 + b, ax[1]);
+    if (a < T.nx && b < T.ny && cc < T.nz + 2 * G::GW) {
+      const int o = (a + 3) + SX * ((b + 3) + SY * (cc + 3 - G::GW)) - (T.j0 + b) * SX;
+      sgy[e] = d1_op<BND, ORD>(su, o, SX, T.j0 + b, ax[1]);
     }
   }
 }
 
-template <int I, bool BND>
+template <int I, bool BND, int ORD>
 __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], double* su, double* sgx,
                                          double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io,
                                          int t) {
   if constexpr (I < 24) {
     constexpr int V = h_order[I];
+    using G = Planes<ORD>;
     stage(pf, su, T, t);
     __syncthreads();
-    if constexpr (has_d2(V)) deriv_planes<BND>(su, sgx, sgy, T, C.ax, t);
-    if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, t);
-    if constexpr (has_d2(V)) __syncthreads();
+    Mixed M;
+    if constexpr (has_d2(V) && G::SHARED) {
+      // d_x plane -> d_x u, d_y d_x u, d_z d_x u; then the d_y plane in the
+      // same buffer -> d_y u, d_z d_y u
+      plane_x<BND, ORD>(su, sgx, T, C.ax, t);
+      if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, t);
+      __syncthreads();
+      if (C.act) {
+        const int ti = C.ti, tj = C.tj, tk = C.tk;
+        M.d1x = sgx[ti + TX * ((tj + G::GW) + G::GXY * (tk + G::GW))];
+        M.dxy = d1_op<BND, ORD>(sgx, ti + TX * ((G::GW - T.j0) + G::GXY * (tk + G::GW)), TX, C.gj, C.ax[1]);
+        M.dxz = d1_op<BND, ORD>(sgx, ti + TX * ((tj + G::GW) + G::GXY * (G::GW - T.k0)), TX * G::GXY, C.gk, C.ax[2]);
+      }
+      __syncthreads();
+      plane_y<BND, ORD>(su, sgx, T, C.ax, t);
+      __syncthreads();
+      if (C.act) {
+        const int ti = C.ti, tj = C.tj, tk = C.tk;
+        M.d1y = sgx[ti + TX * (tj + TY * (tk + G::GW))];
+        M.dyz = d1_op<BND, ORD>(sgx, ti + TX * (tj + TY * (G::GW - T.k0)), TX * TY, C.gk, C.ax[2]);
+      }
+    } else {
+      if constexpr (has_d2(V)) {
+        plane_x<BND, ORD>(su, sgx, T, C.ax, t);
+        plane_y<BND, ORD>(su, sgy, T, C.ax, t);
+      }
+      if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, t);
+      if constexpr (has_d2(V)) __syncthreads();
+    }
     if constexpr (V == dgr::V_CHI) {
       if (C.act) s.finish_metric();
     }
-    process_var<V, BND>(s, C, su, sgx, sgy, T, io);
+    process_var<V, BND, ORD>(s, C, su, sgx, sgy, M, T, io);
     __syncthreads();
-    run_vars<I + 1, BND>(s, C, pf, su, sgx, sgy, T, io, t);
+    run_vars<I + 1, BND, ORD>(s, C, pf, su, sgx, sgy, T, io, t);
   }
 }
 
-template <bool BND>
+template <bool BND, int ORD>
 __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ blocks,
                                                  const DevTile* __restrict__ tiles, int tile_begin, int ntiles,
                                                  const IOPtrs* __restrict__ io, DevParams P) {
   __shared__ double su[NSU];
-  __shared__ double sgx[NGX];
-  __shared__ double sgy[NGY];
+  __shared__ double splanes[Planes<ORD>::N];
+  double* const sgx = splanes;
+  double* const sgy = Planes<ORD>::SHARED ? splanes : splanes + Planes<ORD>::NGX;
   __shared__ double sacc[NSLOT * NPT];
   __shared__ IOPtrs s_io;           // array pointers of this launch
   __shared__ AxisC s_ax[3];         // stencil constants of the current block
@@ -406,9 +500,11 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
   if constexpr (BND) {
     // closure rows for the boundary stencils (visible after the first
     // barrier of the tile loop)
-    constexpr int nd = dgr_forms::F_NFORMS * (int)(sizeof(dgr_forms::Form) / sizeof(double));
+    constexpr int nf = ORD == 64 ? dgr_forms::G_NFORMS : dgr_forms::F_NFORMS;
+    constexpr int nd = nf * (int)(sizeof(dgr_forms::Form) / sizeof(double));
     static_assert(sizeof(dgr_forms::Form) % sizeof(double) == 0, "Form is a whole number of doubles");
-    for (int i = t; i < nd; i += NT) reinterpret_cast<double*>(s_forms)[i] = reinterpret_cast<const double*>(c_forms)[i];
+    const double* src = reinterpret_cast<const double*>(ORD == 64 ? c_forms64 : c_forms42);
+    for (int i = t; i < nd; i += NT) reinterpret_cast<double*>(s_forms)[i] = src[i];
   }
   Ctx C;
   // point of this thread in the tile, in the bank-conflict-free order of
@@ -485,7 +581,7 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
       }
       s.begin();
     }
-    run_vars<0, BND>(s, C, pf, su, sgx, sgy, T, &s_io, t);
+    run_vars<0, BND, ORD>(s, C, pf, su, sgx, sgy, T, &s_io, t);
   }
 }
 
@@ -528,6 +624,8 @@ AxisC host_axis(double h, int sz, unsigned bflag, int a) {
   c.smr3 = 59.0 / 48.0 * 64 * h;
   c.smr2 = 43.0 / 48.0 * 64 * h;
   c.smr1 = 49.0 / 48.0 * 64 * h;
+  c.ih60 = ih / 60.0;        // QuadGrav/src/derivs.cpp:2147
+  c.isq180 = c.isq / 180.0;  // QuadGrav/src/derivs.cpp:2574
   c.e = sz - 3;
   c.lf = (bflag >> (2 * a)) & 1u;
   c.hf = (bflag >> (2 * a + 1)) & 1u;
@@ -655,7 +753,7 @@ extern "C" int dgr_plan_create(dgr_plan_t* plan_out, const dgr_block* blocks, ui
 extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks, uint32_t nblocks,
                                   const dgr_bssn_params* p, const dgr_schedule* sched) {
   if (!plan_out || !blocks || !p || nblocks == 0) return DGR_E_INVALID;
-  if (p->deriv_order != 42) return DGR_E_UNSUPPORTED;
+  if (p->deriv_order != 42 && p->deriv_order != 64) return DGR_E_UNSUPPORTED;
   dgr_schedule sc;
   dgr_schedule_default(&sc);
   if (sched) {
@@ -668,6 +766,7 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
   std::vector<DevTile> ht;
   dgr_plan_s* plan = new dgr_plan_s();
   plan->grid_mult = sc.grid_mult;
+  plan->order = (int)p->deriv_order;
   plan->block_tile_begin.resize(nblocks);
   plan->block_tile_count.resize(nblocks);
   std::vector<std::pair<int, int>> sx, sy, sz;
@@ -793,12 +892,15 @@ int dgr_internal::launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st
   const int ni = (int)plan->ntiles_interior;
   const int nbd = (int)plan->ntiles - ni;
   auto grid = [&](int n) { return plan->grid_mult == 0 ? n : std::min(plan->grid * (int)plan->grid_mult, n); };
+  const bool six = plan->order == 64;
+  const auto kin = six ? k_bssn_rhs<false, 64> : k_bssn_rhs<false, 42>;
+  const auto kbd = six ? k_bssn_rhs<true, 64> : k_bssn_rhs<true, 42>;
   if (ni > 0)
-    hipLaunchKernelGGL(k_bssn_rhs<false>, dim3(grid(ni)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, 0, ni,
-                       d_io, plan->params);
+    hipLaunchKernelGGL(kin, dim3(grid(ni)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, 0, ni, d_io,
+                       plan->params);
   if (nbd > 0)
-    hipLaunchKernelGGL(k_bssn_rhs<true>, dim3(grid(nbd)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, ni, nbd,
-                       d_io, plan->params);
+    hipLaunchKernelGGL(kbd, dim3(grid(nbd)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, ni, nbd, d_io,
+                       plan->params);
   DGR_HIP(hipGetLastError());
   return DGR_OK;
 }
@@ -820,12 +922,11 @@ extern "C" int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const
   for (uint32_t b = 0; b < plan->nblocks; ++b) {
     const int tb = (int)plan->block_tile_begin[b];
     const int nt = (int)plan->block_tile_count[b];
-    if (tb < (int)plan->ntiles_interior)
-      hipLaunchKernelGGL(k_bssn_rhs<false>, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, tb, nt,
-                         plan->d_io, plan->params);
-    else
-      hipLaunchKernelGGL(k_bssn_rhs<true>, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, tb, nt,
-                         plan->d_io, plan->params);
+    const bool six = plan->order == 64;
+    const auto k = tb < (int)plan->ntiles_interior ? (six ? k_bssn_rhs<false, 64> : k_bssn_rhs<false, 42>)
+                                                   : (six ? k_bssn_rhs<true, 64> : k_bssn_rhs<true, 42>);
+    hipLaunchKernelGGL(k, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, tb, nt, plan->d_io,
+                       plan->params);
   }
   DGR_HIP(hipGetLastError());
   return DGR_OK;

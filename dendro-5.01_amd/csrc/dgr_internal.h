@@ -15,7 +15,7 @@ namespace dgr_internal {
 // Per-axis stencil constants, computed on the host exactly as derivs.cpp
 // computes them (IEEE double, no contraction).
 struct AxisC {
-  double ih2, ih12, isq, isq12, pre, smr1, smr2, smr3;
+  double ih2, ih12, isq, isq12, pre, smr1, smr2, smr3, ih60, isq180;
   int e;
   int lf, hf;
   int pad;
@@ -95,6 +95,7 @@ struct dgr_plan_s {
   uint64_t dof_end = 0;                  // one past the last padded point of any block
   double chi_floor = 0.1;                // CHI_FLOOR of the constraint projection
   uint32_t ngroups = 1;                  // 1 + largest block group
+  int order = 42;                        // stencil set: 42 (deriv42) or 64 (deriv64 + ko_deriv42)
 };
 
 namespace dgr_internal {

@@ -1,13 +1,16 @@
 /*
- * deriv42 stencils near flagged faces as coefficient tables.
+ * Finite-difference stencils near flagged faces as coefficient tables.
  *
- * Every deriv42 operator at block coordinate q along an axis reads only the
- * seven points q-3 .. q+3 (the one-sided closures at the first / last three
- * interior points included), so each operator at each position is a row of
- * seven coefficients over that window plus a final scale:
+ * Every operator at block coordinate q along an axis reads only points of
+ * the window q-4 .. q+4 (deriv42 and ko_deriv42 stay within q-3 .. q+3; the
+ * 4th-order one-sided rows of deriv64 reach one further), so each operator at
+ * each position is a row of nine coefficients over that window plus a final
+ * scale:
  *
- *   interior   deriv42_x/xx, deriv42adv_x, ko_deriv42_x
+ *   deriv42    deriv42_x/xx, deriv42adv_x, ko_deriv42_x
  *              (BSSN_GR/src/derivs.cpp:11-88, 244-328, 494-634, 914-1079)
+ *   deriv64    deriv64_x/xx (QuadGrav/src/derivs.cpp:2141-2280, 2569-2680);
+ *              deriv64adv_x is deriv64_x (:2913-2928); KO stays ko_deriv42
  *   closures   the i == 3, 4, 5 and ie-1, ie-2, ie-3 branches of the same
  *              functions, taken when the face's bflag bit is set.
  *
@@ -35,13 +38,15 @@
 
 namespace dgr_forms {
 
-// Scale selectors: the first eight doubles of dgr_internal::AxisC in order.
-enum Scale { S_IH2 = 0, S_IH12, S_ISQ, S_ISQ12, S_PRE, S_SMR1, S_SMR2, S_SMR3 };
+// Scale selectors: the first ten doubles of dgr_internal::AxisC in order.
+enum Scale { S_IH2 = 0, S_IH12, S_ISQ, S_ISQ12, S_PRE, S_SMR1, S_SMR2, S_SMR3, S_IH60, S_ISQ180 };
+
+constexpr int W = 9;  // window q-4 .. q+4
 
 struct Form {
-  double c[7];  // coefficients of points q-3 .. q+3
-  int sel;      // Scale; S_SMR* divide, the others multiply
-  int desc;     // 1: sum from q+3 down to q-3
+  double c[W];  // coefficients of points q-4 .. q+4
+  int sel;      // Scale; S_SMR1..3 divide, the others multiply
+  int desc;     // 1: sum from q+4 down to q-4
 };
 
 enum FormId {
@@ -57,67 +62,124 @@ enum FormId {
   F_NFORMS
 };
 
+// Rows of the 6th-order option: deriv64 first (also the advective
+// operator) and second derivatives; the KO rows are those of the deriv42
+// table at the same ids.
+enum FormId64 {
+  G_D1 = 0, G_D1_L3, G_D1_L4, G_D1_L5, G_D1_H3, G_D1_H2, G_D1_H1,
+  G_D2, G_D2_L3, G_D2_L4, G_D2_L5, G_D2_H3, G_D2_H2, G_D2_H1,
+  G_KO, G_KO_L3, G_KO_L4, G_KO_L5, G_KO_H3, G_KO_H2, G_KO_H1,
+  G_NFORMS
+};
+
 // Window index w: 0 = q-3, 3 = q, 6 = q+3.
+// Window index: 0 = q-4, 4 = q, 8 = q+4.
 #define DGR_FORMS_TABLE                                                                   \
   {                                                                                       \
     /* F_D1    (u[-2] - 8u[-1] + 8u[+1] - u[+2]) ih12 */                                  \
-    {{0, 1, -8, 0, 8, -1, 0}, ::dgr_forms::S_IH12, 0},                                                 \
+    {{0, 0, 1, -8, 0, 8, -1, 0, 0}, ::dgr_forms::S_IH12, 0},                              \
     /* F_D1_L3 (-3u3 + 4u4 - u5) ih2, q = 3 */                                            \
-    {{0, 0, 0, -3, 4, -1, 0}, ::dgr_forms::S_IH2, 0},                                                  \
+    {{0, 0, 0, 0, -3, 4, -1, 0, 0}, ::dgr_forms::S_IH2, 0},                               \
     /* F_D1_L4 (-u3 + u5) ih2, q = 4 */                                                   \
-    {{0, 0, -1, 0, 1, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    {{0, 0, 0, -1, 0, 1, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                \
     /* F_D1_H2 (-u[e-3] + u[e-1]) ih2, q = e-2 */                                         \
-    {{0, 0, -1, 0, 1, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    {{0, 0, 0, -1, 0, 1, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                \
     /* F_D1_H1 (u[e-3] - 4u[e-2] + 3u[e-1]) ih2, q = e-1 */                               \
-    {{0, 1, -4, 3, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    {{0, 0, 1, -4, 3, 0, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                \
     /* F_D2    (-u[-2] + 16u[-1] - 30u0 + 16u[+1] - u[+2]) isq12 */                       \
-    {{0, -1, 16, -30, 16, -1, 0}, ::dgr_forms::S_ISQ12, 0},                                            \
+    {{0, 0, -1, 16, -30, 16, -1, 0, 0}, ::dgr_forms::S_ISQ12, 0},                         \
     /* F_D2_L3 (2u3 - 5u4 + 4u5 - u6) isq */                                              \
-    {{0, 0, 0, 2, -5, 4, -1}, ::dgr_forms::S_ISQ, 0},                                                  \
+    {{0, 0, 0, 0, 2, -5, 4, -1, 0}, ::dgr_forms::S_ISQ, 0},                               \
     /* F_D2_L4 (u3 - 2u4 + u5) isq */                                                     \
-    {{0, 0, 1, -2, 1, 0, 0}, ::dgr_forms::S_ISQ, 0},                                                   \
+    {{0, 0, 0, 1, -2, 1, 0, 0, 0}, ::dgr_forms::S_ISQ, 0},                                \
     /* F_D2_H2 (u[e-3] - 2u[e-2] + u[e-1]) isq */                                         \
-    {{0, 0, 1, -2, 1, 0, 0}, ::dgr_forms::S_ISQ, 0},                                                   \
+    {{0, 0, 0, 1, -2, 1, 0, 0, 0}, ::dgr_forms::S_ISQ, 0},                                \
     /* F_D2_H1 (-u[e-4] + 4u[e-3] - 5u[e-2] + 2u[e-1]) isq */                             \
-    {{-1, 4, -5, 2, 0, 0, 0}, ::dgr_forms::S_ISQ, 0},                                                  \
+    {{0, -1, 4, -5, 2, 0, 0, 0, 0}, ::dgr_forms::S_ISQ, 0},                               \
     /* F_ADV_P (-3u[-1] - 10u0 + 18u[+1] - 6u[+2] + u[+3]) ih12 */                        \
-    {{0, 0, -3, -10, 18, -6, 1}, ::dgr_forms::S_IH12, 0},                                              \
+    {{0, 0, 0, -3, -10, 18, -6, 1, 0}, ::dgr_forms::S_IH12, 0},                           \
     /* F_ADV_N (-u[-3] + 6u[-2] - 18u[-1] + 10u0 + 3u[+1]) ih12 */                        \
-    {{-1, 6, -18, 10, 3, 0, 0}, ::dgr_forms::S_IH12, 0},                                               \
+    {{0, -1, 6, -18, 10, 3, 0, 0, 0}, ::dgr_forms::S_IH12, 0},                            \
     /* F_ADV_L3 (-3u3 + 4u4 - u5) ih2 */                                                  \
-    {{0, 0, 0, -3, 4, -1, 0}, ::dgr_forms::S_IH2, 0},                                                  \
+    {{0, 0, 0, 0, -3, 4, -1, 0, 0}, ::dgr_forms::S_IH2, 0},                               \
     /* F_ADV_L4P (-3u4 + 4u5 - u6) ih2 */                                                 \
-    {{0, 0, 0, -3, 4, -1, 0}, ::dgr_forms::S_IH2, 0},                                                  \
+    {{0, 0, 0, 0, -3, 4, -1, 0, 0}, ::dgr_forms::S_IH2, 0},                               \
     /* F_ADV_L4N (-u3 + u5) ih2 */                                                        \
-    {{0, 0, -1, 0, 1, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    {{0, 0, 0, -1, 0, 1, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                \
     /* F_ADV_L5P (-3u4 - 10u5 + 18u6 - 6u7 + u8) ih12 */                                  \
-    {{0, 0, -3, -10, 18, -6, 1}, ::dgr_forms::S_IH12, 0},                                              \
+    {{0, 0, 0, -3, -10, 18, -6, 1, 0}, ::dgr_forms::S_IH12, 0},                           \
     /* F_ADV_L5N (u3 - 4u4 + 3u5) ih2 */                                                  \
-    {{0, 1, -4, 3, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    {{0, 0, 1, -4, 3, 0, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                \
     /* F_ADV_H1 (u[e-3] - 4u[e-2] + 3u[e-1]) ih2 */                                       \
-    {{0, 1, -4, 3, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    {{0, 0, 1, -4, 3, 0, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                \
     /* F_ADV_H2P (-u[e-3] + u[e-1]) ih2 */                                                \
-    {{0, 0, -1, 0, 1, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    {{0, 0, 0, -1, 0, 1, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                \
     /* F_ADV_H2N (u[e-4] - 4u[e-3] + 3u[e-2]) ih2 */                                      \
-    {{0, 1, -4, 3, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                                   \
+    {{0, 0, 1, -4, 3, 0, 0, 0, 0}, ::dgr_forms::S_IH2, 0},                                \
     /* F_ADV_H3P (-3u[e-3] + 4u[e-2] - u[e-1]) ih2 */                                     \
-    {{0, 0, 0, -3, 4, -1, 0}, ::dgr_forms::S_IH2, 0},                                                  \
+    {{0, 0, 0, 0, -3, 4, -1, 0, 0}, ::dgr_forms::S_IH2, 0},                               \
     /* F_ADV_H3N (-u[e-6] + 6u[e-5] - 18u[e-4] + 10u[e-3] + 3u[e-2]) ih12 */              \
-    {{-1, 6, -18, 10, 3, 0, 0}, ::dgr_forms::S_IH12, 0},                                               \
+    {{0, -1, 6, -18, 10, 3, 0, 0, 0}, ::dgr_forms::S_IH12, 0},                            \
     /* F_KO    pre (-u[-3] + 6u[-2] - 15u[-1] + 20u0 - 15u[+1] + 6u[+2] - u[+3]) */       \
-    {{-1, 6, -15, 20, -15, 6, -1}, ::dgr_forms::S_PRE, 0},                                             \
+    {{0, -1, 6, -15, 20, -15, 6, -1, 0}, ::dgr_forms::S_PRE, 0},                          \
     /* F_KO_L3 (u6 - 3u5 + 3u4 - u3) / smr3, summed downwards */                          \
-    {{0, 0, 0, -1, 3, -3, 1}, ::dgr_forms::S_SMR3, 1},                                                 \
+    {{0, 0, 0, 0, -1, 3, -3, 1, 0}, ::dgr_forms::S_SMR3, 1},                              \
     /* F_KO_L4 (u7 - 6u6 + 12u5 - 10u4 + 3u3) / smr2, downwards */                        \
-    {{0, 0, 3, -10, 12, -6, 1}, ::dgr_forms::S_SMR2, 1},                                               \
+    {{0, 0, 0, 3, -10, 12, -6, 1, 0}, ::dgr_forms::S_SMR2, 1},                            \
     /* F_KO_L5 (u8 - 6u7 + 15u6 - 19u5 + 12u4 - 3u3) / smr1, downwards */                 \
-    {{0, -3, 12, -19, 15, -6, 1}, ::dgr_forms::S_SMR1, 1},                                             \
+    {{0, 0, -3, 12, -19, 15, -6, 1, 0}, ::dgr_forms::S_SMR1, 1},                          \
     /* F_KO_H3 (u[e-6] - 6u[e-5] + 15u[e-4] - 19u[e-3] + 12u[e-2] - 3u[e-1]) / smr1 */    \
-    {{1, -6, 15, -19, 12, -3, 0}, ::dgr_forms::S_SMR1, 0},                                             \
+    {{0, 1, -6, 15, -19, 12, -3, 0, 0}, ::dgr_forms::S_SMR1, 0},                          \
     /* F_KO_H2 (u[e-5] - 6u[e-4] + 12u[e-3] - 10u[e-2] + 3u[e-1]) / smr2 */               \
-    {{1, -6, 12, -10, 3, 0, 0}, ::dgr_forms::S_SMR2, 0},                                               \
+    {{0, 1, -6, 12, -10, 3, 0, 0, 0}, ::dgr_forms::S_SMR2, 0},                            \
     /* F_KO_H1 (u[e-4] - 3u[e-3] + 3u[e-2] - u[e-1]) / smr3 */                            \
-    {{1, -3, 3, -1, 0, 0, 0}, ::dgr_forms::S_SMR3, 0},                                                 \
+    {{0, 1, -3, 3, -1, 0, 0, 0, 0}, ::dgr_forms::S_SMR3, 0},                              \
+  }
+
+#define DGR_FORMS64_TABLE                                                                 \
+  {                                                                                       \
+    /* G_D1    (-u[-3] + 9u[-2] - 45u[-1] + 45u[+1] - 9u[+2] + u[+3]) ih60 */             \
+    {{0, -1, 9, -45, 0, 45, -9, 1, 0}, ::dgr_forms::S_IH60, 0},                           \
+    /* G_D1_L3 (-25u3 + 48u4 - 36u5 + 16u6 - 3u7) ih12 */                                 \
+    {{0, 0, 0, 0, -25, 48, -36, 16, -3}, ::dgr_forms::S_IH12, 0},                         \
+    /* G_D1_L4 (-3u3 - 10u4 + 18u5 - 6u6 + u7) ih12 */                                    \
+    {{0, 0, 0, -3, -10, 18, -6, 1, 0}, ::dgr_forms::S_IH12, 0},                           \
+    /* G_D1_L5 (u3 - 8u4 + 8u6 - u7) ih12 */                                              \
+    {{0, 0, 1, -8, 0, 8, -1, 0, 0}, ::dgr_forms::S_IH12, 0},                              \
+    /* G_D1_H3 (u[e-5] - 8u[e-4] + 8u[e-2] - u[e-1]) ih12, q = e-3 */                     \
+    {{0, 0, 1, -8, 0, 8, -1, 0, 0}, ::dgr_forms::S_IH12, 0},                              \
+    /* G_D1_H2 (-u[e-5] + 6u[e-4] - 18u[e-3] + 10u[e-2] + 3u[e-1]) ih12, q = e-2 */       \
+    {{0, -1, 6, -18, 10, 3, 0, 0, 0}, ::dgr_forms::S_IH12, 0},                            \
+    /* G_D1_H1 (3u[e-5] - 16u[e-4] + 36u[e-3] - 48u[e-2] + 25u[e-1]) ih12, q = e-1 */     \
+    {{3, -16, 36, -48, 25, 0, 0, 0, 0}, ::dgr_forms::S_IH12, 0},                          \
+    /* G_D2    (2u[-3] - 27u[-2] + 270u[-1] - 490u0 + 270u[+1] - 27u[+2] + 2u[+3]) isq180 */ \
+    {{0, 2, -27, 270, -490, 270, -27, 2, 0}, ::dgr_forms::S_ISQ180, 0},                   \
+    /* G_D2_L3 (35u3 - 104u4 + 114u5 - 56u6 + 11u7) isq12 */                              \
+    {{0, 0, 0, 0, 35, -104, 114, -56, 11}, ::dgr_forms::S_ISQ12, 0},                      \
+    /* G_D2_L4 (11u3 - 20u4 + 6u5 + 4u6 - u7) isq12 */                                    \
+    {{0, 0, 0, 11, -20, 6, 4, -1, 0}, ::dgr_forms::S_ISQ12, 0},                           \
+    /* G_D2_L5 (-u3 + 16u4 - 30u5 + 16u6 - u7) isq12 */                                   \
+    {{0, 0, -1, 16, -30, 16, -1, 0, 0}, ::dgr_forms::S_ISQ12, 0},                         \
+    /* G_D2_H3 (-u[e-5] + 16u[e-4] - 30u[e-3] + 16u[e-2] - u[e-1]) isq12 */               \
+    {{0, 0, -1, 16, -30, 16, -1, 0, 0}, ::dgr_forms::S_ISQ12, 0},                         \
+    /* G_D2_H2 (-u[e-5] + 4u[e-4] + 6u[e-3] - 20u[e-2] + 11u[e-1]) isq12 */               \
+    {{0, -1, 4, 6, -20, 11, 0, 0, 0}, ::dgr_forms::S_ISQ12, 0},                           \
+    /* G_D2_H1 (11u[e-5] - 56u[e-4] + 114u[e-3] - 104u[e-2] + 35u[e-1]) isq12 */          \
+    {{11, -56, 114, -104, 35, 0, 0, 0, 0}, ::dgr_forms::S_ISQ12, 0},                      \
+    /* G_KO    pre (-u[-3] + 6u[-2] - 15u[-1] + 20u0 - 15u[+1] + 6u[+2] - u[+3]) */       \
+    {{0, -1, 6, -15, 20, -15, 6, -1, 0}, ::dgr_forms::S_PRE, 0},                          \
+    /* G_KO_L3 (u6 - 3u5 + 3u4 - u3) / smr3, summed downwards */                          \
+    {{0, 0, 0, 0, -1, 3, -3, 1, 0}, ::dgr_forms::S_SMR3, 1},                              \
+    /* G_KO_L4 (u7 - 6u6 + 12u5 - 10u4 + 3u3) / smr2, downwards */                        \
+    {{0, 0, 0, 3, -10, 12, -6, 1, 0}, ::dgr_forms::S_SMR2, 1},                            \
+    /* G_KO_L5 (u8 - 6u7 + 15u6 - 19u5 + 12u4 - 3u3) / smr1, downwards */                 \
+    {{0, 0, -3, 12, -19, 15, -6, 1, 0}, ::dgr_forms::S_SMR1, 1},                          \
+    /* G_KO_H3 (u[e-6] - 6u[e-5] + 15u[e-4] - 19u[e-3] + 12u[e-2] - 3u[e-1]) / smr1 */    \
+    {{0, 1, -6, 15, -19, 12, -3, 0, 0}, ::dgr_forms::S_SMR1, 0},                          \
+    /* G_KO_H2 (u[e-5] - 6u[e-4] + 12u[e-3] - 10u[e-2] + 3u[e-1]) / smr2 */               \
+    {{0, 1, -6, 12, -10, 3, 0, 0, 0}, ::dgr_forms::S_SMR2, 0},                            \
+    /* G_KO_H1 (u[e-4] - 3u[e-3] + 3u[e-2] - u[e-1]) / smr3 */                            \
+    {{0, 1, -3, 3, -1, 0, 0, 0, 0}, ::dgr_forms::S_SMR3, 0},                              \
   }
 
 // Form of each operator at position q of an axis ending at e = sz-3 (the
@@ -156,25 +218,47 @@ DGR_HD inline int form_ko(int q, int e, bool lf, bool hf) {
   return F_KO;
 }
 
-// Sum of one form over the window w[0..6] (points q-3 .. q+3) in ascending
+// deriv64 rows (QuadGrav/src/derivs.cpp: high-face rows tested first).
+DGR_HD inline int form64_d1(int q, int e, bool lf, bool hf) {
+  if (hf && q == e - 1) return G_D1_H1;
+  if (hf && q == e - 2) return G_D1_H2;
+  if (hf && q == e - 3) return G_D1_H3;
+  if (lf && q == 3) return G_D1_L3;
+  if (lf && q == 4) return G_D1_L4;
+  if (lf && q == 5) return G_D1_L5;
+  return G_D1;
+}
+DGR_HD inline int form64_d2(int q, int e, bool lf, bool hf) {
+  if (hf && q == e - 1) return G_D2_H1;
+  if (hf && q == e - 2) return G_D2_H2;
+  if (hf && q == e - 3) return G_D2_H3;
+  if (lf && q == 3) return G_D2_L3;
+  if (lf && q == 4) return G_D2_L4;
+  if (lf && q == 5) return G_D2_L5;
+  return G_D2;
+}
+DGR_HD inline int form64_ko(int q, int e, bool lf, bool hf) { return form_ko(q, e, lf, hf) - F_KO + G_KO; }
+
+// Sum of one form over the window w[0..8] (points q-4 .. q+4) in ascending
 // order; the caller takes the descending sum for desc forms.
-DGR_HD inline double form_sum_up(const double* c, const double (&w)[7]) {
+DGR_HD inline double form_sum_up(const double* c, const double (&w)[W]) {
 #pragma clang fp contract(off)
   double t = 0.0;
-  for (int m = 0; m < 7; ++m) t = t + (c[m] != 0.0 ? c[m] * w[m] : 0.0);
+  for (int m = 0; m < W; ++m) t = t + (c[m] != 0.0 ? c[m] * w[m] : 0.0);
   return t;
 }
-DGR_HD inline double form_sum_down(const double* c, const double (&w)[7]) {
+DGR_HD inline double form_sum_down(const double* c, const double (&w)[W]) {
 #pragma clang fp contract(off)
   double t = 0.0;
-  for (int m = 6; m >= 0; --m) t = t + (c[m] != 0.0 ? c[m] * w[m] : 0.0);
+  for (int m = W - 1; m >= 0; --m) t = t + (c[m] != 0.0 ? c[m] * w[m] : 0.0);
   return t;
 }
-// Final scale: scales[] are ih2, ih12, isq, isq12, pre, smr1, smr2, smr3.
+// Final scale: scales[] are ih2, ih12, isq, isq12, pre, smr1, smr2, smr3,
+// ih60, isq180.
 DGR_HD inline double form_scale(double sum, int sel, const double* scales) {
 #pragma clang fp contract(off)
   const double s = scales[sel];
-  return sel >= S_SMR1 ? sum / s : sum * s;
+  return (sel >= S_SMR1 && sel <= S_SMR3) ? sum / s : sum * s;
 }
 
 }  // namespace dgr_forms

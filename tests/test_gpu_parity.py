@@ -112,3 +112,44 @@ def test_repeat_calls_deterministic(cuda):
         outs.append(R)
     torch.cuda.synchronize()
     assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
+
+
+# 6th-order option: deriv64 first / second / advective derivatives with
+# ko_deriv42 (QuadGrav/include/derivs.h:80-96), same bar.
+
+def test_sixth_order_flat_bumps_with_faces(cuda):
+    blocks, dof = layout.uniform_grid(4, dx=1.0 / 64, outer_bflags=True)
+    run_case(cuda, blocks, dof, data.flat_bumps(blocks, dof, seed=11), {"deriv_order": 64})
+
+
+def test_sixth_order_mixed_sizes_random_faces(cuda):
+    blocks, dof = layout.mixed_blocks(12, seed=12, dist=((13, 0.25), (19, 0.25), (25, 0.25), (37, 0.25)))
+    rng = np.random.default_rng(12)
+    for b in blocks:
+        b["bflag"] = int(rng.integers(0, 64))
+    run_case(cuda, blocks, dof, data.flat_bumps(blocks, dof, seed=13, amp=1e-2, width=0.3), {"deriv_order": 64})
+
+
+def test_sixth_order_brill_lindquist(cuda):
+    blocks, dof = layout.uniform_grid(4, dx=0.05, outer_bflags=True)
+    for b in blocks:
+        b["pmin"][0] += 4.0
+        b["pmax"][0] += 4.0
+    run_case(cuda, blocks, dof, data.brill_lindquist(blocks, dof), {"deriv_order": 64})
+
+
+def test_sixth_order_differs_from_fourth(cuda):
+    """The option really switches operators (not the same kernel twice)."""
+    blocks, dof = layout.uniform_grid(2, dx=1.0 / 64, outer_bflags=True)
+    U = data.flat_bumps(blocks, dof, seed=14, amp=1e-2, width=0.1).to(cuda)
+    out = {}
+    for order in (42, 64):
+        R = torch.full_like(U, SENTINEL)
+        plan = Plan(blocks, make_params(deriv_order=order))
+        plan.rhs(R, U)
+        R2 = torch.full_like(U, SENTINEL)
+        plan.rhs(R2, U, per_block=True)
+        torch.cuda.synchronize()
+        assert torch.equal(R, R2)
+        out[order] = R
+    assert not torch.equal(out[42], out[64])

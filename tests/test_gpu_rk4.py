@@ -47,9 +47,9 @@ def test_enforce_constraints_matches_oracle(cuda):
     assert np.array_equal(G[:, mask], U[:, mask])
 
 
-def _evolve_gpu(cuda, blocks, U, dt, steps, use_graph):
+def _evolve_gpu(cuda, blocks, U, dt, steps, use_graph, order=42):
     Ud = torch.from_numpy(U.copy()).to(cuda)
-    plan = Plan(blocks, make_params())
+    plan = Plan(blocks, make_params(deriv_order=order))
     rk = RK4(plan, Ud, use_graph=use_graph)
     for _ in range(steps):
         rk.step(dt)
@@ -59,8 +59,8 @@ def _evolve_gpu(cuda, blocks, U, dt, steps, use_graph):
     return out, ok
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_rk4_steps_match_oracle(cuda, use_graph):
+@pytest.mark.parametrize("use_graph,order", [(False, 42), (True, 42), (True, 64)])
+def test_rk4_steps_match_oracle(cuda, use_graph, order):
     nb = 3
     dx = 1.0 / 32
     blocks, dof = layout.uniform_grid(nb, dx=dx)
@@ -68,9 +68,9 @@ def test_rk4_steps_match_oracle(cuda, use_graph):
     dt = 0.25 * dx
     ref = U.copy()
     for _ in range(2):
-        ref, bad = po.rk4_step_uniform(ref, blocks, nb, dt)
+        ref, bad = po.rk4_step_uniform(ref, blocks, nb, dt, po.params(deriv_order=order))
         assert bad == 0
-    got, ok = _evolve_gpu(cuda, blocks, U, dt, 2, use_graph)
+    got, ok = _evolve_gpu(cuda, blocks, U, dt, 2, use_graph, order)
     assert ok
     ii = layout.interior_index(blocks)
     change = np.abs(ref[:, ii] - U[:, ii]).max(1)
