@@ -21,6 +21,8 @@ Also reported on the same line:
                  the autotuner picked for each workload, its time against the
                  library default, and the tuning time (outside every timed
                  region; cached per device and workload shape);
+  adaptive       Brill-Lindquist punctures on a 2:1-balanced octree of about
+                 21000 blocks of 13^3 over 6 levels: RHS and 10 RK4 steps;
   serving        a bounded Poisson stream (20 jobs/s) of 50-step evolutions
                  of 64..4096 blocks under a 200 GB HBM budget: p50/p99 job
                  latency and aggregate RHS points/s;
@@ -181,6 +183,44 @@ def uneven(stream, steps, autotune_on):
     return out
 
 
+def adaptive(stream, steps, autotune_on):
+    """Brill-Lindquist punctures (m1 = m2 = 0.5, separation 8) on a
+    2:1-balanced octree of 13^3 blocks refined toward the punctures over 6
+    levels; RHS and 10 hipGraph RK4 steps (CFL 0.25 on the finest level)."""
+    import torch
+    from pydgr import RK4, Plan, data, layout, make_params
+    blocks, dof, lev = layout.adaptive_octree(half_width=32.0, base_level=3, levels=6, factor=4.5)
+    pts = layout.interior_points(blocks)
+    dev = torch.device("cuda:0")
+    prm = make_params(ko_sigma=0.4, eta_const=2.0, chi_floor=1e-4)
+    sched, tune_rep = tuned_schedule(blocks, prm, autotune_on)
+    U = data.brill_lindquist(blocks, dof, m1=0.5, m2=0.5, sep=8.0, device=dev)
+    R = torch.zeros_like(U)
+    plan = Plan(blocks, prm, schedule=sched)
+    ms = time_rhs(plan, R, U, stream, steps, 2)
+    nbytes = algorithmic_bytes(blocks)
+    del R
+    dx_min = min((b["pmax"][0] - b["pmin"][0]) / (b["sz"][0] - 1) for b in blocks)
+    rk = RK4(plan, U, use_graph=True)
+    rk.step(0.25 * dx_min, 0)
+    ms_rk, _ = time_rk4(rk, stream, 10, 1)
+    rk.close()
+    plan.close()
+    del U
+    torch.cuda.empty_cache()
+    counts = {}
+    for l in lev:
+        counts[l] = counts.get(l, 0) + 1
+    return {"workload": "Brill-Lindquist m1=m2=0.5, separation 8, on %d blocks of 13^3 over levels %s (2:1 balanced "
+                        "octree of [-32,32]^3), chi floor 1e-4" % (len(blocks), counts),
+            "interior_points": pts, "rhs_ms": ms, "rhs_points_per_s": pts / (ms * 1e-3),
+            "rhs_hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "rk4_steps": 10, "rk4_ms_per_step": ms_rk, "rk4_rhs_points_per_s": 4 * pts / (ms_rk * 1e-3),
+            "rk4_note": "ghost layers are refilled from same-level neighbours only; ghosts at level jumps keep "
+                        "their initial values (no coarse/fine interpolation)",
+            "autotune": tune_rep}
+
+
 def serving(njobs, budget_gb, max_points):
     import torch
     from pydgr.serving import Server, poisson_jobs
@@ -204,6 +244,7 @@ def main():
     ap.add_argument("--per-block-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-adaptive", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=24, help="0 skips the serving run")
     ap.add_argument("--serving-budget-gb", type=float, default=200.0)
     ap.add_argument("--serving-max-points", type=int, default=3_000_000,
@@ -286,6 +327,9 @@ def main():
     if not a.no_uneven:
         line["uneven"] = uneven(stream, a.steps, not a.no_autotune)
         log("uneven: RHS %.4f ms, RK4 %.4f ms" % (line["uneven"]["rhs_ms"], line["uneven"]["rk4_ms_per_step"]))
+    if not a.no_adaptive:
+        line["adaptive"] = adaptive(stream, a.steps, not a.no_autotune)
+        log("adaptive: RHS %.4f ms, RK4 %.4f ms" % (line["adaptive"]["rhs_ms"], line["adaptive"]["rk4_ms_per_step"]))
     if a.serving_jobs > 0:
         line["serving"] = serving(a.serving_jobs, a.serving_budget_gb, a.serving_max_points or None)
         log("serving: p50 %.3f s p99 %.3f s" % (line["serving"]["p50_latency_s"], line["serving"]["p99_latency_s"]))

@@ -79,3 +79,87 @@ def interior_index(blocks):
         k, j, i = np.meshgrid(np.arange(3, s2 - 3), np.arange(3, s1 - 3), np.arange(3, s0 - 3), indexing="ij")
         out.append((b["offset"] + i + s0 * (j + s1 * k)).ravel())
     return np.concatenate(out).astype(np.int64)
+
+
+def adaptive_octree(half_width=32.0, base_level=3, levels=6, centers=((4.0, 0.0, 0.0), (-4.0, 0.0, 0.0)),
+                    factor=2.0, ele_order=6):
+    """2:1-balanced octree of blocks refined toward `centers` (punctures).
+
+    The cube [-half_width, half_width]^3 is cut into octants of level
+    base_level; an octant of level l < base_level + levels - 1 is split
+    while the distance from its centre to the nearest centre is below
+    `factor` times its edge, then the leaves are 2:1 balanced across faces,
+    edges and corners (a leaf is split while any neighbouring leaf is more
+    than one level finer).  Each leaf becomes one block of one order-
+    `ele_order` element (sz = ele_order + 7) with dx = edge / ele_order;
+    blocks on the domain boundary carry the flags of their outer faces.
+    Returns (blocks, dof, levels_of_blocks)."""
+    top = base_level + levels - 1
+    cen = np.asarray(centers, dtype=float)
+
+    def edge(l):
+        return 2.0 * half_width / (1 << l)
+
+    def centre(l, i, j, k):
+        e = edge(l)
+        return np.array([-half_width + (i + 0.5) * e, -half_width + (j + 0.5) * e, -half_width + (k + 0.5) * e])
+
+    n0 = 1 << base_level
+    leaves = {(base_level, i, j, k) for i in range(n0) for j in range(n0) for k in range(n0)}
+    for l in range(base_level, top):
+        split = [o for o in leaves if o[0] == l and
+                 np.min(np.linalg.norm(cen - centre(*o), axis=1)) < factor * edge(l)]
+        for o in split:
+            leaves.remove(o)
+            _, i, j, k = o
+            for c in range(8):
+                leaves.add((l + 1, 2 * i + (c & 1), 2 * j + ((c >> 1) & 1), 2 * k + (c >> 2)))
+
+    def leaf_at(l, i, j, k):
+        """Leaf containing octant (l, i, j, k) if it is that octant or an
+        ancestor; None if the region is refined further or outside."""
+        n = 1 << l
+        if not (0 <= i < n and 0 <= j < n and 0 <= k < n):
+            return None
+        while l >= base_level:
+            if (l, i, j, k) in leaves:
+                return (l, i, j, k)
+            l, i, j, k = l - 1, i >> 1, j >> 1, k >> 1
+        return None
+
+    changed = True
+    while changed:
+        changed = False
+        for o in sorted(leaves, key=lambda t: -t[0]):
+            if o not in leaves:
+                continue
+            l, i, j, k = o
+            for d in range(27):
+                if d == 13:
+                    continue
+                nb = leaf_at(l, i + d % 3 - 1, j + (d // 3) % 3 - 1, k + d // 9 - 1)
+                if nb is not None and nb[0] < l - 1:
+                    leaves.remove(nb)
+                    bl, bi, bj, bk = nb
+                    for c in range(8):
+                        leaves.add((bl + 1, 2 * bi + (c & 1), 2 * bj + ((c >> 1) & 1), 2 * bk + (c >> 2)))
+                    changed = True
+
+    sz = ele_order + 1 + 2 * GHOST
+    blocks, lev = [], []
+    off = 0
+    for o in sorted(leaves):
+        l, i, j, k = o
+        e = edge(l)
+        n = 1 << l
+        bf = 0
+        for d, c in enumerate((i, j, k)):
+            if c == 0:
+                bf |= 1 << (2 * d)
+            if c == n - 1:
+                bf |= 1 << (2 * d + 1)
+        corner = [-half_width + i * e, -half_width + j * e, -half_width + k * e]
+        blocks.append(block_dict(off, [sz] * 3, bf, corner, e / ele_order, [ele_order] * 3))
+        lev.append(l)
+        off += sz ** 3
+    return blocks, off, lev

@@ -122,7 +122,8 @@ class Server:
         self.work = torch.empty(3 * 24 * self.capacity, dtype=torch.float64, device=self.device)
         self.alloc = PoolAllocator(self.capacity)
         self.use_graph = use_graph
-        self.params = params or make_params(ko_sigma=0.4)
+        # chi floor of the reference's puncture parameter files (1e-4)
+        self.params = params or make_params(ko_sigma=0.4, chi_floor=1e-4)
         self.keep_results = keep_results
         self.rebuilds = 0
 

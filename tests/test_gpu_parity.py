@@ -153,3 +153,13 @@ def test_sixth_order_differs_from_fourth(cuda):
         assert torch.equal(R, R2)
         out[order] = R
     assert not torch.equal(out[42], out[64])
+
+
+def test_adaptive_octree_brill_lindquist(cuda):
+    """Blocks of 4 refinement levels (2:1 balanced) around a puncture, with
+    domain-face flags, in one launch; both stencil orders."""
+    blocks, dof, _ = layout.adaptive_octree(half_width=4.0, base_level=1, levels=4,
+                                            centers=((0.7, 0.1, -0.2),), factor=1.5)
+    U = data.brill_lindquist(blocks, dof, m1=0.5, m2=0.0, sep=1.4)
+    for order in (42, 64):
+        run_case(cuda, blocks, dof, U, {"deriv_order": order})
