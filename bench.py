@@ -117,6 +117,29 @@ def time_rhs(plan, R, U, stream, steps, warm, per_block=False):
     return e0.elapsed_time(e1) / steps
 
 
+def pcie_inclusive(plan, R, U, stream, pts, reps=3):
+    """The host-array form of the boundary (what the C++ bssnRHS shim does
+    with host pointers): the 24 input arrays copied in from pinned host
+    memory, one RHS, the 24 output arrays copied back.  Reported beside the
+    device-resident rate, never as `value`."""
+    import torch
+    hu = torch.empty(U.shape, dtype=U.dtype, pin_memory=True)
+    hr = torch.empty(R.shape, dtype=R.dtype, pin_memory=True)
+    hu.copy_(U)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        U.copy_(hu, non_blocking=True)
+        plan.rhs(R, U, stream)
+        hr.copy_(R, non_blocking=True)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"ms": ms, "points_per_s": pts / (ms * 1e-3), "bytes_over_pcie": 2 * U.numel() * 8}
+
+
 def time_rk4(rk, stream, steps, warm):
     """(ms per RK4 step from HIP events, wall seconds of the timed region)."""
     import torch
@@ -279,6 +302,8 @@ def main():
         ms_pb = time_rhs(plan, R, U, stream, a.per_block_steps, 1, per_block=True)
         log("per-block RHS %.4f ms" % ms_pb)
     U_cpu = U.cpu() if not a.no_cpu else None
+    pcie = pcie_inclusive(plan, R, U, stream, pts)
+    log("PCIe-inclusive RHS (host arrays staged in and out) %.2f ms" % pcie["ms"])
     del R
     rk = RK4(plan, U, use_graph=True)
     rk.step(0.25 * dx, 0)  # sets dt for every block
@@ -316,6 +341,7 @@ def main():
                      "kernel": "k_bssn_rhs", "kernel_ms": rhs_ms, "bytes_per_launch": nbytes},
         "rhs": {"ms": rhs_ms, "points_per_s": rhs_pps},
         "autotune": tune_rep,
+        "pcie_inclusive": pcie,
         "fp64": {"reference_flop_per_point": REF_FLOP_PER_POINT,
                  "achieved_tflops": REF_FLOP_PER_POINT * rhs_pps / 1e12,
                  "frac_of_peak": REF_FLOP_PER_POINT * rhs_pps / 1e12 / FP64_PEAK_TFLOPS},
