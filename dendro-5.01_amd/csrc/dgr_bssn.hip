@@ -177,8 +177,16 @@ struct TileGeom {
 // The region is walked in the fixed SX x SY x SZ layout of the LDS buffer
 // (compile-time divisors); elements outside a partial tile's region are
 // skipped and their LDS slots are never read.
-__device__ __forceinline__ void prefetch(double (&r)[NPF], const double* __restrict__ g, const TileGeom& T,
-                                         int t) {
+//
+// Gather mode (RK4 stages): a ghost point with a same-level neighbour is
+// read from the neighbour's interior instead of the block's own padding,
+// sd[direction] being the index distance to that source (0 for interior
+// points and for ghosts without a neighbour, which keep their own padding).
+// This is the same-level refill done on the fly, so the RK4 step needs no
+// refill kernel between stages.
+template <bool GATHER>
+__device__ __forceinline__ void prefetch(double (&r)[NPF], const double* __restrict__ g, const TileGeom& T, int t,
+                                         const long long* sd, uint32_t gcode) {
   const double* base = g + T.off + (T.i0 - 3) + (long)T.sz0 * ((T.j0 - 3) + (long)T.sz1 * (T.k0 - 3));
 #pragma unroll
   for (int m = 0; m < NPF; ++m) {
@@ -186,9 +194,29 @@ __device__ __forceinline__ void prefetch(double (&r)[NPF], const double* __restr
     const int a = e % SX;
     const int b = (e / SX) % SY;
     const int c = e / (SX * SY);
-    if (e < NSU && a < T.rx && b < T.ry && c < T.rz)
-      r[m] = __builtin_nontemporal_load(gmem(base) + a + (long)T.sz0 * (b + (long)T.sz1 * c));
+    if (e < NSU && a < T.rx && b < T.ry && c < T.rz) {
+      if constexpr (GATHER)
+        r[m] = __builtin_nontemporal_load(gmem(base) + a + (long)T.sz0 * (b + (long)T.sz1 * c) +
+                                          sd[(gcode >> (5 * m)) & 31u]);
+      else
+        r[m] = __builtin_nontemporal_load(gmem(base) + a + (long)T.sz0 * (b + (long)T.sz1 * c));
+    }
   }
+}
+
+// Ghost direction of every prefetch slot of this thread (gather mode).
+__device__ __forceinline__ uint32_t gather_codes(const TileGeom& T, int t) {
+  uint32_t code = 0;
+#pragma unroll
+  for (int m = 0; m < NPF; ++m) {
+    const int e = t + m * NT;
+    const int i = T.i0 - 3 + e % SX, j = T.j0 - 3 + (e / SX) % SY, k = T.k0 - 3 + e / (SX * SY);
+    const int dx = i < 3 ? 0 : (i >= T.sz0 - 3 ? 2 : 1);
+    const int dy = j < 3 ? 0 : (j >= T.sz1 - 3 ? 2 : 1);
+    const int dz = k < 3 ? 0 : (k >= T.sz2 - 3 ? 2 : 1);
+    code |= (uint32_t)(dx + 3 * dy + 9 * dz) << (5 * m);
+  }
+  return code;
 }
 
 __device__ __forceinline__ void stage(const double (&r)[NPF], double* su, const TileGeom& T, int t) {
@@ -437,10 +465,10 @@ __device__ __forceinline__ void plane_y(const double* su, double* sgy, const Til
   }
 }
 
-template <int I, bool BND, int ORD>
+template <int I, bool BND, int ORD, bool GATHER>
 __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], double* su, double* sgx,
                                          double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io,
-                                         int t) {
+                                         int t, const long long* sd, uint32_t gcode) {
   if constexpr (I < 24) {
     constexpr int V = h_order[I];
     using G = Planes<ORD>;
@@ -451,7 +479,7 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
       // d_x plane -> d_x u, d_y d_x u, d_z d_x u; then the d_y plane in the
       // same buffer -> d_y u, d_z d_y u
       plane_x<BND, ORD>(su, sgx, T, C.ax, t);
-      if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, t);
+      if constexpr (I + 1 < 24) prefetch<GATHER>(pf, io->u.v[h_order[I + 1]], T, t, sd, gcode);
       __syncthreads();
       if (C.act) {
         const int ti = C.ti, tj = C.tj, tk = C.tk;
@@ -472,7 +500,7 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
         plane_x<BND, ORD>(su, sgx, T, C.ax, t);
         plane_y<BND, ORD>(su, sgy, T, C.ax, t);
       }
-      if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, t);
+      if constexpr (I + 1 < 24) prefetch<GATHER>(pf, io->u.v[h_order[I + 1]], T, t, sd, gcode);
       if constexpr (has_d2(V)) __syncthreads();
     }
     if constexpr (V == dgr::V_CHI) {
@@ -480,14 +508,15 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
     }
     process_var<V, BND, ORD>(s, C, su, sgx, sgy, M, T, io);
     __syncthreads();
-    run_vars<I + 1, BND, ORD>(s, C, pf, su, sgx, sgy, T, io, t);
+    run_vars<I + 1, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, io, t, sd, gcode);
   }
 }
 
-template <bool BND, int ORD>
+template <bool BND, int ORD, bool GATHER>
 __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ blocks,
                                                  const DevTile* __restrict__ tiles, int tile_begin, int ntiles,
-                                                 const IOPtrs* __restrict__ io, DevParams P) {
+                                                 const IOPtrs* __restrict__ io, DevParams P,
+                                                 const int32_t* __restrict__ nbr) {
   __shared__ double su[NSU];
   __shared__ double splanes[Planes<ORD>::N];
   double* const sgx = splanes;
@@ -495,6 +524,7 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
   __shared__ double sacc[NSLOT * NPT];
   __shared__ IOPtrs s_io;           // array pointers of this launch
   __shared__ AxisC s_ax[3];         // stencil constants of the current block
+  __shared__ long long s_delta[GATHER ? NNBR : 1];  // gather mode: source distance per ghost direction
 
   const int t = threadIdx.x;
   if constexpr (BND) {
@@ -541,8 +571,24 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
     else if (t < 51) s_ax[t - 48] = B.ax[t - 48];
     C.ax = s_ax;
 
+    if constexpr (GATHER) {
+      // source distance of each ghost direction: the same point of the
+      // same-level neighbour's interior (corners sz-7 apart), or 0
+      if (t < NNBR) {
+        const int n = nbr[(size_t)tl.block * NNBR + t];
+        long long dl = 0;
+        if (n >= 0) {
+          const long long dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+          dl = (long long)blocks[n].offset - (long long)T.off -
+               (dx * (T.sz0 - 7) + (long long)T.sz0 * (dy * (T.sz1 - 7) + (long long)T.sz1 * dz * (T.sz2 - 7)));
+        }
+        s_delta[t] = dl;
+      }
+      __syncthreads();
+    }
+    const uint32_t gcode = GATHER ? gather_codes(T, t) : 0u;
     double pf[NPF];
-    prefetch(pf, io->u.v[h_order[0]], T, t);
+    prefetch<GATHER>(pf, io->u.v[h_order[0]], T, t, s_delta, gcode);
     __syncthreads();
 
     // point coordinates, eta (rhs.cpp:214-223) and the Sommerfeld face test
@@ -581,7 +627,7 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
       }
       s.begin();
     }
-    run_vars<0, BND, ORD>(s, C, pf, su, sgx, sgy, T, &s_io, t);
+    run_vars<0, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, s_delta, gcode);
   }
 }
 
@@ -888,19 +934,22 @@ int upload_ptrs(dgr_plan_t plan, double* const* rhs, const double* const* uin, h
 }
 }  // namespace
 
-int dgr_internal::launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st) {
+int dgr_internal::launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st, bool gather) {
   const int ni = (int)plan->ntiles_interior;
   const int nbd = (int)plan->ntiles - ni;
   auto grid = [&](int n) { return plan->grid_mult == 0 ? n : std::min(plan->grid * (int)plan->grid_mult, n); };
   const bool six = plan->order == 64;
-  const auto kin = six ? k_bssn_rhs<false, 64> : k_bssn_rhs<false, 42>;
-  const auto kbd = six ? k_bssn_rhs<true, 64> : k_bssn_rhs<true, 42>;
+  const auto kin = gather ? (six ? k_bssn_rhs<false, 64, true> : k_bssn_rhs<false, 42, true>)
+                          : (six ? k_bssn_rhs<false, 64, false> : k_bssn_rhs<false, 42, false>);
+  const auto kbd = gather ? (six ? k_bssn_rhs<true, 64, true> : k_bssn_rhs<true, 42, true>)
+                          : (six ? k_bssn_rhs<true, 64, false> : k_bssn_rhs<true, 42, false>);
+  const int32_t* nb = gather ? plan->d_nbr : nullptr;
   if (ni > 0)
     hipLaunchKernelGGL(kin, dim3(grid(ni)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, 0, ni, d_io,
-                       plan->params);
+                       plan->params, nb);
   if (nbd > 0)
     hipLaunchKernelGGL(kbd, dim3(grid(nbd)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, ni, nbd, d_io,
-                       plan->params);
+                       plan->params, nb);
   DGR_HIP(hipGetLastError());
   return DGR_OK;
 }
@@ -910,7 +959,7 @@ extern "C" int dgr_bssn_rhs(dgr_plan_t plan, double* const* rhs, const double* c
   hipStream_t st = (hipStream_t)stream;
   int rc = upload_ptrs(plan, rhs, uin, st);
   if (rc) return rc;
-  return launch_rhs(plan, plan->d_io, st);
+  return launch_rhs(plan, plan->d_io, st, false);
 }
 
 extern "C" int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const double* const* uin,
@@ -923,10 +972,11 @@ extern "C" int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const
     const int tb = (int)plan->block_tile_begin[b];
     const int nt = (int)plan->block_tile_count[b];
     const bool six = plan->order == 64;
-    const auto k = tb < (int)plan->ntiles_interior ? (six ? k_bssn_rhs<false, 64> : k_bssn_rhs<false, 42>)
-                                                   : (six ? k_bssn_rhs<true, 64> : k_bssn_rhs<true, 42>);
+    const auto k = tb < (int)plan->ntiles_interior
+                       ? (six ? k_bssn_rhs<false, 64, false> : k_bssn_rhs<false, 42, false>)
+                       : (six ? k_bssn_rhs<true, 64, false> : k_bssn_rhs<true, 42, false>);
     hipLaunchKernelGGL(k, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, tb, nt, plan->d_io,
-                       plan->params);
+                       plan->params, (const int32_t*)nullptr);
   }
   DGR_HIP(hipGetLastError());
   return DGR_OK;

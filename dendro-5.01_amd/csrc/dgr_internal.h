@@ -99,6 +99,8 @@ struct dgr_plan_s {
 };
 
 namespace dgr_internal {
-// Launch the RHS kernels of `plan` with a device pointer table.
-int launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st);
+// Launch the RHS kernels of `plan` with a device pointer table.  With
+// gather, ghost points that have a same-level neighbour are read from that
+// neighbour's interior (the RK4 stages), so no refill is needed before.
+int launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st, bool gather);
 }  // namespace dgr_internal

@@ -10,16 +10,20 @@
  * enforce.  The reference does the combine on zipped nodal vectors; here
  * the state stays unzipped: the combine runs on every block's interior
  * points (shared face nodes are duplicated in both blocks and receive
- * identical values), and the "unzip" is a same-level ghost refill that
- * copies each ghost point from the neighbouring block's interior.
+ * identical values), and the "unzip" is a same-level ghost fill: the RHS
+ * of each stage reads a ghost point that has a same-level neighbour from
+ * that neighbour's interior (gather mode of the RHS kernel), which is what a
+ * refill before it would have copied there.
  *
  * Kernels:
  *   k_stage          fused stage combine + running final sum + constraint
  *                    projection (BSSN_GR/src/bssn_constraints.cpp:12-172)
  *   k_enforce        the projection alone
  *   k_ghost_refill   padding <- neighbour interiors (uniform level only;
- *                    coarse/fine interpolation is not implemented)
- * The four stages are captured once into a hipGraph and replayed.
+ *                    coarse/fine interpolation is not implemented); run on
+ *                    the state once at the end of each evolution call
+ * One step (four RHS + stage launches) is captured once into a hipGraph and
+ * replayed.
  */
 #include <hip/hip_runtime.h>
 
@@ -49,12 +53,15 @@ struct Ptrs24 {
   double* v[24];
 };
 
-// RK4 work memory: one allocation of 3 x 24 arrays of dof doubles, the
-// running final sum (acc), the stage state (stg) and the RHS (k), addressed
-// arithmetically so that only the caller's state needs a pointer table.
+// RK4 work memory: one allocation holding the stage state (stg) and the RHS
+// (k) as 24 padded arrays of dof doubles each, then the running final sum in
+// a compact layout; all addressed arithmetically so that only the caller's
+// state needs a pointer table.
 struct Work {
-  double* base;
+  double* base;   // stage state (24 x dof, padded layout) then k (24 x dof)
   uint64_t dof;
+  double* acc;    // running sum, compact: 343 slots per tile, 24 x nacc
+  uint64_t nacc;
 };
 
 // Per-thread cursor over the 24 variables of one point in the work memory.
@@ -63,14 +70,25 @@ struct Work {
 // scalar registers (which would spill).
 typedef __attribute__((address_space(1))) double gdouble;  // global memory
 struct WorkCursor {
-  double* q;        // acc of variable v at point o
+  double* q;        // stage state of variable v at padded point o
   uint64_t s24;     // 24 * dof
   __device__ __forceinline__ WorkCursor(const Work& W, uint64_t o) : q(W.base + o), s24(24 * W.dof) {}
-  __device__ __forceinline__ gdouble& acc() { return ((gdouble*)q)[0]; }
-  __device__ __forceinline__ gdouble& stg() { return ((gdouble*)q)[s24]; }
-  __device__ __forceinline__ gdouble& k() { return ((gdouble*)q)[2 * s24]; }
+  __device__ __forceinline__ gdouble& stg() { return ((gdouble*)q)[0]; }
+  __device__ __forceinline__ gdouble& k() { return ((gdouble*)q)[s24]; }
   __device__ __forceinline__ void next(uint64_t dof) {
     q += dof;
+    asm volatile("" : "+v"(q));
+  }
+};
+// The running sum is never seen outside k_stage, so it lives in a compact
+// tile-major layout (slot w * 343 + thread of tile w): whole-line reads and
+// writes instead of the 7-point row fragments of the padded layout.
+struct AccCursor {
+  double* q;
+  __device__ __forceinline__ AccCursor(const Work& W, uint64_t co) : q(W.acc + co) {}
+  __device__ __forceinline__ gdouble& acc() { return ((gdouble*)q)[0]; }
+  __device__ __forceinline__ void next(uint64_t n) {
+    q += n;
     asm volatile("" : "+v"(q));
   }
 };
@@ -145,17 +163,20 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
     uint64_t o;
     if (!tile_point(tl, B, threadIdx.x, o)) continue;
     const double dt = gdt[B.group];
+    const uint64_t co = (uint64_t)w * 343 + threadIdx.x;
     double un[24], kk[24];
     if (KIND != ST_LAST) {
       const double cu = c_rkU[stage] * dt, cc = c_rkC[stage] * dt;
       double acc[24];
       WorkCursor c(W, o);
+      AccCursor a(W, co);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
         un[v] = state.v[v][o];  // u_prev: the state at the start of the step
         kk[v] = c.k();
-        acc[v] = KIND == ST_FIRST ? un[v] : c.acc();
+        acc[v] = KIND == ST_FIRST ? un[v] : a.acc();
         c.next(W.dof);
+        a.next(W.nacc);
       }
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
@@ -163,20 +184,22 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
         acc[v] = acc[v] + cc * kk[v];
         un[v] = un[v] + cu * kk[v];
       }
-      WorkCursor d(W, o);
+      AccCursor d(W, co);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
         d.acc() = acc[v];
-        d.next(W.dof);
+        d.next(W.nacc);
       }
     } else {
       const double cc = c_rkC[3] * dt;
       WorkCursor c(W, o);
+      AccCursor a(W, co);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
-        un[v] = c.acc();
+        un[v] = a.acc();
         kk[v] = c.k();
         c.next(W.dof);
+        a.next(W.nacc);
       }
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
@@ -261,8 +284,9 @@ int grid_for(dgr_plan_t plan, int n) {
 struct dgr_rk4_s {
   dgr_plan_t plan = nullptr;
   uint64_t dof = 0;
-  double* buf = nullptr;        // acc, stage state, k: 3 x 24 x dof
-  Ptrs24 state{}, acc{}, stg{}, k{};  // the state and the work arrays in buf
+  double* buf = nullptr;        // stage state, k (24 x dof each), running sum (24 x 343 per tile)
+  Ptrs24 state{}, stg{}, k{};  // the state and the padded work arrays in buf
+  uint64_t nacc = 0;           // compact running-sum slots per variable (343 per tile)
   IOPtrs* d_io = nullptr;       // RHS pointer tables: [0] (u -> k), [1] (stage -> k)
   double* d_gdt = nullptr;      // time step per block group
   std::vector<double> h_gdt;
@@ -282,12 +306,13 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
   const int nt = (int)p->ntiles;
   for (int s = 0; s < 4; ++s) {
     // k_s = RHS(u_s): stage 0 reads the state, later stages the stage buffer
-    int rc = launch_rhs(p, r->d_io + (s == 0 ? 0 : 1), st);
+    int rc = launch_rhs(p, r->d_io + (s == 0 ? 0 : 1), st, true);
     if (rc) return rc;
-    // stage combine + projection, then the same-level ghost refill
+    // stage combine + projection; the next RHS reads ghosts from the
+    // neighbours' interiors (gather), so no refill runs between stages
     const int grid = std::min(nt, std::max(1, p->grid * 2));
     const int chunk = (nt + grid - 1) / grid;
-    const Work W = {r->buf, r->dof};
+    const Work W = {r->buf, r->dof, r->buf + 48 * r->dof, r->nacc};
     const dim3 g((nt + chunk - 1) / chunk), b(384);
     if (s == 0)
       hipLaunchKernelGGL(k_stage<ST_FIRST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, chunk, r->state, W,
@@ -298,8 +323,6 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
     else
       hipLaunchKernelGGL(k_stage<ST_LAST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, chunk, r->state, W,
                          r->d_gdt, p->chi_floor, s, r->d_flag);
-    hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(p, (int)p->nblocks)), dim3(RK_NT), 0, st, p->d_blocks,
-                       p->d_nbr, (int)p->nblocks, s < 3 ? r->stg : r->state);
   }
   DGR_HIP(hipGetLastError());
   return DGR_OK;
@@ -380,7 +403,10 @@ extern "C" int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* 
   return flag ? DGR_E_CONSTRAINT : DGR_OK;
 }
 
-extern "C" uint64_t dgr_rk4_work_doubles(dgr_plan_t plan) { return plan ? 3ull * 24ull * plan->dof_end : 0; }
+// stage state and k in the padded layout, the running sum compact
+extern "C" uint64_t dgr_rk4_work_doubles(dgr_plan_t plan) {
+  return plan ? 48ull * plan->dof_end + 24ull * 343ull * plan->ntiles : 0;
+}
 
 extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double* const* u, double* work,
                                         int use_graph) {
@@ -394,19 +420,18 @@ extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double*
   const size_t nb = sizeof(double) * r->dof;
   r->own_buf = work == nullptr;
   r->buf = work;
-  if ((r->own_buf && hipMalloc(&r->buf, nb * 24 * 3) != hipSuccess) ||
+  r->nacc = 343ull * plan->ntiles;
+  if ((r->own_buf && hipMalloc(&r->buf, sizeof(double) * dgr_rk4_work_doubles(plan)) != hipSuccess) ||
       hipMalloc(&r->d_io, sizeof(IOPtrs) * 2) != hipSuccess ||
       hipMalloc(&r->d_gdt, sizeof(double) * plan->ngroups) != hipSuccess || hipMalloc(&r->d_flag, 256) != hipSuccess) {
     destroy(r);
     return DGR_E_NOMEM;
   }
-  Ptrs24& acc = r->acc;
   Ptrs24& stg = r->stg;
   Ptrs24& k = r->k;
   for (int v = 0; v < 24; ++v) {
-    acc.v[v] = r->buf + (size_t)v * r->dof;
-    stg.v[v] = r->buf + (size_t)(24 + v) * r->dof;
-    k.v[v] = r->buf + (size_t)(48 + v) * r->dof;
+    stg.v[v] = r->buf + (size_t)v * r->dof;
+    k.v[v] = r->buf + (size_t)(24 + v) * r->dof;
   }
   IOPtrs io[2];
   for (int v = 0; v < 24; ++v) {
@@ -484,6 +509,14 @@ extern "C" int dgr_rk4_run(dgr_rk4_t r, int nsteps, void* stream) {
     } else {
       rc = enqueue_step(r, r->st);
     }
+  }
+  if (rc == DGR_OK && nsteps > 0) {
+    // the state's ghost layers, left stale by the gathering stages, are
+    // refilled once at the end of the run
+    hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(r->plan, (int)r->plan->nblocks)), dim3(RK_NT), 0, r->st,
+                       r->plan->d_blocks, r->plan->d_nbr, (int)r->plan->nblocks, r->state);
+    e = hipGetLastError();
+    if (e != hipSuccess) rc = hip_fail(e, "k_ghost_refill");
   }
   if (rc == DGR_OK) {
     e = hipEventRecord(ev, r->st);

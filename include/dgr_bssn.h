@@ -152,11 +152,15 @@ int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* stream);
  * enforce_bssn_constraints, the final u_prev + sum_s RK4_C[s] dt k_s
  * (rkBSSN.h:55-58) and a same-level ghost refill after every stage.  The
  * step is captured into a hipGraph on first use (use_graph != 0) and
- * replayed.  Work buffers: 3 x 24 x dof doubles. */
+ * replayed.  The RHS of stages 1-4 reads ghost points that have a same-level
+ * neighbour from that neighbour's interior, which gives the same values as a
+ * refill after each stage; the state's own ghost layers are refilled once at
+ * the end of every dgr_rk4_run / _step / _steps call.  Work memory:
+ * dgr_rk4_work_doubles(plan) doubles. */
 typedef struct dgr_rk4_s* dgr_rk4_t;
 int dgr_rk4_create(dgr_rk4_t* rk, dgr_plan_t plan, double* const* u, int use_graph);
 /* Same, with caller-owned work memory of at least dgr_rk4_work_doubles(plan)
- * doubles (no allocation inside). */
+ * doubles = 48 x dof + 24 x 343 x tiles (no allocation inside). */
 uint64_t dgr_rk4_work_doubles(dgr_plan_t plan);
 int dgr_rk4_create_with_work(dgr_rk4_t* rk, dgr_plan_t plan, double* const* u, double* work, int use_graph);
 /* Per-group time steps dt[g], g < ngroups (every group in the plan must be
