@@ -158,13 +158,13 @@ def time_rk4(rk, stream, steps, warm):
     return e0.elapsed_time(e1) / steps, wall
 
 
-def tuned_schedule(blocks, params, enabled):
+def tuned_schedule(blocks, params, enabled, mode="rhs"):
     """Autotuned schedule (None = library default) and its report."""
     if not enabled:
         return None, None
     from pydgr import autotune
-    sched, rep = autotune.tune(blocks, params, device="cuda:0", log=log)
-    short = {k: rep[k] for k in ("schedule", "best_ms", "default_ms", "tuning_s", "sample_blocks", "cached")}
+    sched, rep = autotune.tune(blocks, params, device="cuda:0", log=log, mode=mode)
+    short = {k: rep[k] for k in ("schedule", "mode", "best_ms", "default_ms", "tuning_s", "sample_blocks", "cached")}
     log("autotune: %s (%.3f ms vs default %.3f ms, %s)" % (rep["schedule"], rep["best_ms"], rep["default_ms"],
                                                         "cached" if rep["cached"] else "%.1f s" % rep["tuning_s"]))
     return sched, short
@@ -216,7 +216,7 @@ def adaptive(stream, steps, autotune_on):
     pts = layout.interior_points(blocks)
     dev = torch.device("cuda:0")
     prm = make_params(ko_sigma=0.4, eta_const=2.0, chi_floor=1e-4)
-    sched, tune_rep = tuned_schedule(blocks, prm, autotune_on)
+    sched, tune_rep = tuned_schedule(blocks, prm, autotune_on, mode="rk4")
     U = data.brill_lindquist(blocks, dof, m1=0.5, m2=0.5, sep=8.0, device=dev)
     R = torch.zeros_like(U)
     plan = Plan(blocks, prm, schedule=sched)
@@ -290,7 +290,7 @@ def main():
     U = data.gauge_wave(blocks, dof, amp=0.01, noise=1e-6, seed=0, device=dev)
     R = torch.zeros_like(U)
     prm = make_params(ko_sigma=0.4, eta_const=2.0)
-    sched, tune_rep = tuned_schedule(blocks, prm, not a.no_autotune)
+    sched, tune_rep = tuned_schedule(blocks, prm, not a.no_autotune, mode="rk4")
     plan = Plan(blocks, prm, schedule=sched)
     stream = torch.cuda.current_stream()
     log("%d blocks 13^3, %d interior points, %.2f GB state" % (len(blocks), pts, U.numel() * 8 / 1e9))

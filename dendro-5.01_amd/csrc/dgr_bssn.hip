@@ -769,18 +769,54 @@ void order_tiles(std::vector<DevTile>& t, size_t begin, size_t end, uint32_t ord
       const dgr_block &A = blocks[a.block], &B = blocks[b.block];
       return (uint64_t)A.sz[0] * A.sz[1] * A.sz[2] > (uint64_t)B.sz[0] * B.sz[1] * B.sz[2];
     });
-  } else if (order == DGR_ORDER_XCD) {
+  } else if (order == DGR_ORDER_XCD || order == DGR_ORDER_SFC) {
     constexpr int NXCD = 8;
     std::vector<std::vector<DevTile>> q(NXCD);
-    size_t i = begin;
-    while (i < end) {
+    // runs of one block's tiles
+    std::vector<std::pair<size_t, size_t>> runs;
+    for (size_t i = begin; i < end;) {
       size_t j = i;
       while (j < end && t[j].block == t[i].block) ++j;
-      int best = 0;
-      for (int x = 1; x < NXCD; ++x)
-        if (q[x].size() < q[best].size()) best = x;
-      q[best].insert(q[best].end(), t.begin() + i, t.begin() + j);
+      runs.emplace_back(i, j);
       i = j;
+    }
+    if (order == DGR_ORDER_SFC) {
+      // Morton order of the block corners on the finest spacing present,
+      // then 8 contiguous runs of about equal tile counts
+      double hmin = 1e300, lo[3] = {1e300, 1e300, 1e300};
+      for (auto& r : runs) {
+        const dgr_block& B = blocks[t[r.first].block];
+        for (int d = 0; d < 3; ++d) {
+          hmin = std::min(hmin, (B.pmax[d] - B.pmin[d]) / (B.sz[d] - 1));
+          lo[d] = std::min(lo[d], B.pmin[d]);
+        }
+      }
+      auto key = [&](const std::pair<size_t, size_t>& r) {
+        const dgr_block& B = blocks[t[r.first].block];
+        uint64_t c[3], k = 0;
+        for (int d = 0; d < 3; ++d) c[d] = (uint64_t)std::llround((B.pmin[d] - lo[d]) / hmin) & 0x1fffff;
+        for (int bit = 20; bit >= 0; --bit)
+          for (int d = 2; d >= 0; --d) k = (k << 1) | ((c[d] >> bit) & 1u);
+        return k;
+      };
+      std::vector<std::pair<uint64_t, size_t>> ord(runs.size());
+      for (size_t r = 0; r < runs.size(); ++r) ord[r] = {key(runs[r]), r};
+      std::stable_sort(ord.begin(), ord.end());
+      const size_t total = end - begin;
+      size_t acc = 0;
+      for (auto& kr : ord) {
+        const auto& r = runs[kr.second];
+        const int x = (int)std::min<size_t>(NXCD - 1, acc * NXCD / std::max<size_t>(total, 1));
+        q[x].insert(q[x].end(), t.begin() + r.first, t.begin() + r.second);
+        acc += r.second - r.first;
+      }
+    } else {
+      for (auto& r : runs) {
+        int best = 0;
+        for (int x = 1; x < NXCD; ++x)
+          if (q[x].size() < q[best].size()) best = x;
+        q[best].insert(q[best].end(), t.begin() + r.first, t.begin() + r.second);
+      }
     }
     std::vector<size_t> pos(NXCD, 0);
     size_t o = begin;
@@ -806,7 +842,7 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
     sc = *sched;
     for (int c = 0; c < DGR_SIZE_CLASSES; ++c)
       if (sc.tile_max[c] < 3 || sc.tile_max[c] > (uint32_t)TX) return DGR_E_INVALID;
-    if (sc.order > DGR_ORDER_LARGEST || sc.grid_mult > 64) return DGR_E_INVALID;
+    if (sc.order > DGR_ORDER_SFC || sc.grid_mult > 64) return DGR_E_INVALID;
   }
   std::vector<DevBlock> hb(nblocks);
   std::vector<DevTile> ht;

@@ -6,8 +6,10 @@ What is tuned (dgr_schedule, include/dgr_bssn.h):
                interior is cut into the fewest near-equal runs of at most
                tile_max points per axis;
   order        tile order of the persistent launch: block order, XCD-aware
-               (a block's tiles on one XCD so its halo stays in one L2), or
-               largest blocks first;
+               (a block's tiles on one XCD so its halo stays in one L2),
+               largest blocks first, or Morton order cut into one run per
+               XCD (neighbouring blocks share an L2, which the RK4 stages'
+               ghost gathers read);
   grid_mult    persistent grid of grid_mult x CUs, or one workgroup per tile.
 
 Every schedule gives bitwise the same result; only the speed differs.  The
@@ -28,11 +30,11 @@ import numpy as np
 import torch
 
 from . import data, layout
-from .native import (ORDER_BLOCK, ORDER_LARGEST, ORDER_XCD, SIZE_CLASSES, Plan, Schedule, default_schedule,
-                     size_class)
+from .native import (ORDER_BLOCK, ORDER_LARGEST, ORDER_SFC, ORDER_XCD, RK4, SIZE_CLASSES, Plan, Schedule,
+                     default_schedule, size_class)
 
 TILE_CANDIDATES = (7, 6, 5)
-ORDER_CANDIDATES = (ORDER_BLOCK, ORDER_XCD, ORDER_LARGEST)
+ORDER_CANDIDATES = (ORDER_BLOCK, ORDER_XCD, ORDER_LARGEST, ORDER_SFC)
 GRID_CANDIDATES = (1, 2, 0)
 
 
@@ -84,22 +86,33 @@ def _relocate(blocks):
     return out, off
 
 
-def _time_rhs(blocks, dof, params, sched, device, reps):
+def _time_rhs(blocks, dof, params, sched, device, reps, mode="rhs"):
+    """Mean ms of one RHS launch, or (mode "rk4") of one hipGraph RK4 step."""
     U = data.flat_bumps(blocks, dof, seed=1, amp=1e-3, width=0.1, device=device)
-    R = torch.zeros_like(U)
+    R = torch.zeros_like(U) if mode == "rhs" else None
     plan = Plan(blocks, params, schedule=sched)
+    rk = None
     try:
         stream = torch.cuda.current_stream(device)
-        plan.rhs(R, U, stream)
+        if mode == "rk4":
+            dx = min((b["pmax"][0] - b["pmin"][0]) / (b["sz"][0] - 1) for b in blocks)
+            rk = RK4(plan, U, use_graph=True)
+            rk.step(0.25 * dx, 1, stream)
+            run = lambda: rk.run(1, stream)  # noqa: E731
+        else:
+            plan.rhs(R, U, stream)
+            run = lambda: plan.rhs(R, U, stream)  # noqa: E731
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(reps):
-            plan.rhs(R, U, stream)
+            run()
         e1.record(stream)
         torch.cuda.synchronize(device)
         return e0.elapsed_time(e1) / reps
     finally:
+        if rk is not None:
+            rk.close()
         plan.close()
         del U, R
 
@@ -113,11 +126,15 @@ def _sample(blocks, limit, seed=0):
     return [blocks[i] for i in idx]
 
 
-def tune(blocks, params, device="cuda:0", reps=3, probe_blocks=1024, sample_blocks=4096, cache=True, log=None):
-    """Best Schedule for `blocks` (cached).  Returns (schedule, report)."""
+def tune(blocks, params, device="cuda:0", reps=3, probe_blocks=1024, sample_blocks=4096, cache=True, log=None,
+         mode="rhs"):
+    """Best Schedule for `blocks` (cached).  mode "rhs" times RHS launches;
+    "rk4" times the order / grid candidates as hipGraph RK4 steps (whose RHS
+    gathers ghosts from neighbours) on the whole workload, which is then not
+    sampled.  Returns (schedule, report)."""
     device = torch.device(device)
     path = cache_path()
-    key = workload_key(blocks, device)
+    key = workload_key(blocks, device) + "|" + mode
     store = _load_cache(path) if cache else {}
     if key in store:
         return Schedule.from_dict(store[key]["schedule"]), dict(store[key], cached=True)
@@ -142,14 +159,15 @@ def tune(blocks, params, device="cuda:0", reps=3, probe_blocks=1024, sample_bloc
             if log:
                 log("autotune: class %d tile_max %d: %.3f ms (%d blocks)" % (c, tm, ms, len(probe)))
         best.tile_max[c] = min(res, key=res.get)
-    # 2. order and grid on the whole (sampled) workload
-    full, dof = _relocate(_sample(blocks, sample_blocks))
+    # 2. order and grid on the whole (sampled) workload; RK4 steps need the
+    # blocks' neighbours, so that mode keeps every block
+    full, dof = _relocate(blocks if mode == "rk4" else _sample(blocks, sample_blocks))
     pts = layout.interior_points(full)
     res = {}
     for order in ORDER_CANDIDATES:
         for gm in GRID_CANDIDATES:
             s = Schedule.from_dict(dict(best.as_dict(), order=order, grid_mult=gm))
-            ms = _time_rhs(full, dof, params, s, device, reps)
+            ms = _time_rhs(full, dof, params, s, device, reps, mode)
             res[(order, gm)] = ms
             trials.append({"order": order, "grid_mult": gm, "ms": ms, "points_per_s": pts / (ms * 1e-3)})
             if log:
@@ -157,8 +175,8 @@ def tune(blocks, params, device="cuda:0", reps=3, probe_blocks=1024, sample_bloc
     order, gm = min(res, key=res.get)
     best.order = order
     best.grid_mult = gm
-    default_ms = _time_rhs(full, dof, params, default_schedule(), device, reps)
-    report = {"schedule": best.as_dict(), "key": key, "tuning_s": time.perf_counter() - t0,
+    default_ms = _time_rhs(full, dof, params, default_schedule(), device, reps, mode)
+    report = {"schedule": best.as_dict(), "key": key, "mode": mode, "tuning_s": time.perf_counter() - t0,
               "best_ms": res[(order, gm)], "default_ms": default_ms, "sample_blocks": len(full),
               "trials": trials}
     if cache:

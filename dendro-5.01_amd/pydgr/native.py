@@ -46,7 +46,7 @@ class BSSNParams(ctypes.Structure):
 
 
 SIZE_CLASSES = 4          # padded block extent <= 13, <= 19, <= 25, larger
-ORDER_BLOCK, ORDER_XCD, ORDER_LARGEST = 0, 1, 2
+ORDER_BLOCK, ORDER_XCD, ORDER_LARGEST, ORDER_SFC = 0, 1, 2, 3
 
 
 class Schedule(ctypes.Structure):

@@ -82,7 +82,9 @@ typedef struct dgr_plan_s* dgr_plan_t;
 enum {
   DGR_ORDER_BLOCK = 0,   /* tiles in block order */
   DGR_ORDER_XCD = 1,     /* a block's tiles on one XCD (shared L2 for the halos) */
-  DGR_ORDER_LARGEST = 2  /* tiles of the largest blocks first */
+  DGR_ORDER_LARGEST = 2, /* tiles of the largest blocks first */
+  DGR_ORDER_SFC = 3      /* blocks in Morton order cut into 8 contiguous runs, one per XCD:
+                            neighbouring blocks share an L2 (gathered ghosts) */
 };
 typedef struct dgr_schedule {
   uint32_t tile_max[DGR_SIZE_CLASSES]; /* largest tile extent per axis, 3..7 (default 7) */

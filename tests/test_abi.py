@@ -46,7 +46,7 @@ def test_plan_rejects_bad_input_without_device_work():
     b = native.blocks_array([{"offset": 0, "sz": [13, 13, 13], "bflag": 0, "pmin": [0, 0, 0], "pmax": [1, 1, 1]}])
     bad = native.Schedule.from_dict({"tile_max": [8, 7, 7, 7], "order": 0, "grid_mult": 1})
     assert L.dgr_plan_create_ex(ctypes.byref(h), b, 1, ctypes.byref(p), ctypes.byref(bad)) == -1
-    bad = native.Schedule.from_dict({"tile_max": [7, 7, 7, 7], "order": 3, "grid_mult": 1})
+    bad = native.Schedule.from_dict({"tile_max": [7, 7, 7, 7], "order": 4, "grid_mult": 1})
     assert L.dgr_plan_create_ex(ctypes.byref(h), b, 1, ctypes.byref(p), ctypes.byref(bad)) == -1
     p.deriv_order = 66  # only 42 and 64 exist
     assert L.dgr_plan_create(ctypes.byref(h), b, 1, ctypes.byref(p)) == -4
