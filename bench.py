@@ -239,8 +239,9 @@ def adaptive(stream, steps, autotune_on):
             "interior_points": pts, "rhs_ms": ms, "rhs_points_per_s": pts / (ms * 1e-3),
             "rhs_hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "rk4_steps": 10, "rk4_ms_per_step": ms_rk, "rk4_rhs_points_per_s": 4 * pts / (ms_rk * 1e-3),
-            "rk4_note": "ghost layers are refilled from same-level neighbours only; ghosts at level jumps keep "
-                        "their initial values (no coarse/fine interpolation)",
+            "rk4_note": "ghosts with a same-level neighbour are gathered from its interior; ghosts at level jumps "
+                        "are interpolated from the coarser element (order-6 Lagrange) or copied from finer nodes "
+                        "after every stage",
             "autotune": tune_rep}
 
 

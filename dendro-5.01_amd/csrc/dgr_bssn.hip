@@ -653,6 +653,8 @@ void free_plan(dgr_plan_s* plan) {
   (void)hipFree(plan->d_tiles_blk);
   (void)hipFree(plan->d_io);
   (void)hipFree(plan->d_nbr);
+  (void)hipFree(plan->d_fill_items);
+  (void)hipFree(plan->d_fill_copies);
   delete plan;
 }
 
@@ -681,18 +683,19 @@ AxisC host_axis(double h, int sz, unsigned bflag, int a) {
 
 // Same-level neighbour table: blocks of equal size and spacing whose
 // corners are sz-7 points apart along an axis are neighbours.  Corners are
-// compared on the integer lattice pmin/h.
+// compared on the integer lattice pmin/h, spacings by a rounded log2 (equal
+// up to rounding).
 void build_neighbours(const dgr_block* blocks, const std::vector<DevBlock>& hb, uint32_t nblocks,
                       std::vector<int32_t>& nbr) {
-  using Key = std::tuple<long long, long long, long long, uint32_t, uint32_t, uint32_t, uint64_t, uint32_t>;
+  using Key = std::tuple<long long, long long, long long, uint32_t, uint32_t, uint32_t, long long, long long,
+                         long long, uint32_t>;
+  auto lev = [](double h) { return llround(std::log2(h) * 64.0); };
   auto key = [&](uint32_t b, int dx, int dy, int dz) {
     const DevBlock& B = hb[b];
     const int d[3] = {dx, dy, dz};
     long long K[3];
     for (int a = 0; a < 3; ++a) K[a] = llround(B.pmin[a] / B.h[a]) + (long long)d[a] * (long long)(B.sz[a] - 7);
-    uint64_t hb64;
-    std::memcpy(&hb64, &B.h[0], sizeof(hb64));
-    return Key(K[0], K[1], K[2], B.sz[0], B.sz[1], B.sz[2], hb64, B.group);
+    return Key(K[0], K[1], K[2], B.sz[0], B.sz[1], B.sz[2], lev(B.h[0]), lev(B.h[1]), lev(B.h[2]), B.group);
   };
   std::map<Key, int32_t> pos;
   for (uint32_t b = 0; b < nblocks; ++b) pos.emplace(key(b, 0, 0, 0), (int32_t)b);
@@ -703,8 +706,7 @@ void build_neighbours(const dgr_block* blocks, const std::vector<DevBlock>& hb, 
         for (int dx = -1; dx <= 1; ++dx) {
           if (!dx && !dy && !dz) continue;
           auto it = pos.find(key(b, dx, dy, dz));
-          if (it != pos.end() && hb[it->second].h[1] == hb[b].h[1] && hb[it->second].h[2] == hb[b].h[2])
-            nbr[(size_t)b * NNBR + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] = it->second;
+          if (it != pos.end()) nbr[(size_t)b * NNBR + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] = it->second;
         }
   (void)blocks;
 }
@@ -936,6 +938,11 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
   if (e != hipSuccess) {
     free_plan(plan);
     return hip_fail(e, "hipMemcpy(plan)");
+  }
+  const int rc = build_level_fill(plan, blocks, hb);
+  if (rc != DGR_OK) {
+    free_plan(plan);
+    return rc;
   }
   *plan_out = plan;
   return DGR_OK;

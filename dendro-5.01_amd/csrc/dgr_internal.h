@@ -57,6 +57,25 @@ struct DevParams {
   double eta_const, eta_r0, eta_damping_exp, ko_sigma;
 };
 
+// Ghost fill across a level jump, fine side: the ghost box of a block (one
+// of its 26 directions) lies inside one element of a block twice as coarse
+// and is interpolated from that element's 7^3 nodes.
+struct FillItem {
+  uint64_t dst;        // padded index of the box's first point in the fine block
+  uint64_t src;        // padded index of the coarse block's node (3,3,3)
+  uint32_t dsz[3];     // fine block sizes (strides of dst)
+  uint32_t ssz[3];     // coarse block sizes (strides of src)
+  uint8_t n[3];        // box extent per axis (3 or sz-6)
+  uint8_t s0[3];       // position of the box's first point in the coarse element, in half node spacings
+  uint8_t pad[2];
+};
+
+// Ghost fill across a level jump, coarse side: every ghost point coincides
+// with a node of a finer block and copies it.
+struct FillCopy {
+  uint64_t dst, src;
+};
+
 
 // Same-level neighbour of a block in direction (dx,dy,dz) in {-1,0,1}^3,
 // slot (dx+1) + 3 (dy+1) + 9 (dz+1); -1 when there is none (domain
@@ -96,6 +115,11 @@ struct dgr_plan_s {
   double chi_floor = 0.1;                // CHI_FLOOR of the constraint projection
   uint32_t ngroups = 1;                  // 1 + largest block group
   int order = 42;                        // stencil set: 42 (deriv42) or 64 (deriv64 + ko_deriv42)
+  // level jumps (adaptive layouts): interpolation boxes and copies
+  dgr_internal::FillItem* d_fill_items = nullptr;
+  uint32_t nfill_items = 0;
+  dgr_internal::FillCopy* d_fill_copies = nullptr;
+  uint64_t nfill_copies = 0;
 };
 
 namespace dgr_internal {
@@ -103,4 +127,11 @@ namespace dgr_internal {
 // gather, ghost points that have a same-level neighbour are read from that
 // neighbour's interior (the RK4 stages), so no refill is needed before.
 int launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st, bool gather);
+
+// Find the level-jump ghost fills of a plan's blocks and upload them
+// (plan->d_fill_*); blocks without a same-level neighbour in a direction are
+// looked up one level coarser and one level finer.
+int build_level_fill(dgr_plan_t plan, const dgr_block* blocks, const std::vector<DevBlock>& hb);
+// Fill the level-jump ghosts of the 24 arrays u (asynchronous on st).
+int launch_level_fill(dgr_plan_t plan, const OutPtrs& u, hipStream_t st);
 }  // namespace dgr_internal

@@ -323,6 +323,11 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
     else
       hipLaunchKernelGGL(k_stage<ST_LAST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, chunk, r->state, W,
                          r->d_gdt, p->chi_floor, s, r->d_flag);
+    // ghosts across level jumps (adaptive layouts) of the new stage state
+    OutPtrs o;
+    for (int v = 0; v < 24; ++v) o.v[v] = s < 3 ? r->stg.v[v] : r->state.v[v];
+    rc = launch_level_fill(p, o, st);
+    if (rc) return rc;
   }
   DGR_HIP(hipGetLastError());
   return DGR_OK;
@@ -376,10 +381,13 @@ extern "C" int dgr_ghost_refill(dgr_plan_t plan, double* const* u, void* stream)
   if (!plan || check_ptrs(u)) return DGR_E_INVALID;
   hipStream_t st = (hipStream_t)stream;
   Ptrs24 h;
-  for (int v = 0; v < 24; ++v) h.v[v] = u[v];
+  OutPtrs o;
+  for (int v = 0; v < 24; ++v) h.v[v] = o.v[v] = u[v];
   hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(plan, (int)plan->nblocks)), dim3(RK_NT), 0, st, plan->d_blocks,
                      plan->d_nbr, (int)plan->nblocks, h);
   DGR_HIP(hipGetLastError());
+  int rc = launch_level_fill(plan, o, st);
+  if (rc) return rc;
   DGR_HIP(hipStreamSynchronize(st));
   return DGR_OK;
 }

@@ -122,6 +122,7 @@ def lib():
         L.dgr_bssn_rhs_blocks.argtypes = [pp, pp, ctypes.POINTER(Block), ctypes.c_uint32,
                                           ctypes.POINTER(BSSNParams), vp]
         L.dgr_ghost_refill.argtypes = [vp, pp, vp]
+        L.dgr_plan_level_fills.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]
         L.dgr_enforce_constraints.argtypes = [vp, pp, vp]
         L.dgr_rk4_create.argtypes = [ctypes.POINTER(vp), vp, pp, ctypes.c_int]
         L.dgr_rk4_work_doubles.restype = ctypes.c_uint64
@@ -202,6 +203,12 @@ class Plan:
     @property
     def points(self):
         return int(lib().dgr_plan_points(self.handle))
+
+    def level_fills(self):
+        """(interpolation boxes, node copies) filling ghosts at level jumps."""
+        b, c = ctypes.c_uint32(), ctypes.c_uint64()
+        check(lib().dgr_plan_level_fills(self.handle, ctypes.byref(b), ctypes.byref(c)), "dgr_plan_level_fills")
+        return int(b.value), int(c.value)
 
     @property
     def tiles(self):

@@ -130,15 +130,19 @@ int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const double* co
 int dgr_bssn_rhs_blocks(double* const* rhs, const double* const* u, const dgr_block* blocks,
                         uint32_t nblocks, const dgr_bssn_params* params, void* stream);
 
-/* Same-level "unzip": every ghost point of every block (outside
- * [3, sz-3)^3) is overwritten with the same physical point of the
- * neighbouring block's interior, for neighbours of equal size and spacing
- * (corners sz-7 points apart).  Stands in for the ghost exchange + unzip
- * between RK stages (BSSN_GR/src/rkBSSN.cpp:909-914, Mesh::unzip
- * include/mesh.tcc:7436) on uniform layouts; ghosts with no same-level
- * neighbour (domain boundary, level jumps) are left untouched.
- * Synchronous. */
+/* "Unzip" of the padding between RK stages (ghost exchange + unzip,
+ * BSSN_GR/src/rkBSSN.cpp:909-914, Mesh::unzip include/mesh.tcc:7436):
+ * every ghost point of every block (outside [3, sz-3)^3) with a same-level
+ * neighbour (equal size and spacing, corners sz-7 points apart) takes that
+ * neighbour's interior value; on 2:1-balanced layouts of 13^3 blocks, ghosts
+ * next to a coarser block are interpolated from its element (order-6
+ * Lagrange, x then y then z) and ghosts next to finer blocks copy their
+ * nodes.  Ghosts outside the domain are left untouched.  Synchronous. */
 int dgr_ghost_refill(dgr_plan_t plan, double* const* u, void* stream);
+/* Number of level-jump interpolation boxes and node copies the plan found
+ * (0 on uniform layouts).  No counterpart in the reference (its unzip
+ * recomputes these from the mesh on every call). */
+int dgr_plan_level_fills(dgr_plan_t plan, uint32_t* boxes, uint64_t* copies);
 
 /* enforce_bssn_constraints (BSSN_GR/src/bssn_constraints.cpp:12-172) on
  * every interior point: det(gt) = 1, tr(At) = 0, chi and alpha floored at
