@@ -60,50 +60,59 @@ __device__ __forceinline__ void lagrange_table(double (*W)[7], int t) {
   }
 }
 
+// Four variables per pass: four times the work between barriers.
+constexpr int VB = 4;
+
 __global__ void __launch_bounds__(NF) k_fill_interp(const FillItem* __restrict__ items, uint32_t nitems,
                                                      OutPtrs u) {
   __shared__ double W[13][7];
-  __shared__ double sC[343], sX[343], sY[343];
+  __shared__ double sC[VB][343], sX[VB][343], sY[VB][343];
   const int t = threadIdx.x;
   lagrange_table(W, t);
   for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
     const FillItem I = items[it];
     const int nx = I.n[0], ny = I.n[1], nz = I.n[2];
-    for (int v = 0; v < 24; ++v) {
-      const gdouble* src = (const gdouble*)u.v[v] + I.src;
-      __syncthreads();  // previous variable (or item) done with sC, sY; W visible
-      for (int e = t; e < 343; e += NF)
-        sC[e] = src[(e % 7) + (uint64_t)I.ssz[0] * ((e / 7) % 7 + (uint64_t)I.ssz[1] * (e / 49))];
+    const int nX = nx * 49, nY = nx * ny * 7, nZ = nx * ny * nz;
+    for (int v0 = 0; v0 < 24; v0 += VB) {
+      __syncthreads();  // previous pass (or item) done with sC, sY; W visible
+      for (int e = t; e < VB * 343; e += NF) {
+        const int w = e / 343, m = e % 343;
+        const gdouble* src = (const gdouble*)u.v[v0 + w] + I.src;
+        sC[w][m] = src[(m % 7) + (uint64_t)I.ssz[0] * ((m / 7) % 7 + (uint64_t)I.ssz[1] * (m / 49))];
+      }
       __syncthreads();
       // along x: X[p][b][c] = sum_a W[s0x + p][a] C[a][b][c]
-      for (int e = t; e < nx * 49; e += NF) {
-        const int p = e % nx, b = (e / nx) % 7, c = e / (nx * 7);
-        const double* w = W[I.s0[0] + p];
-        const double* cc = sC + 7 * b + 49 * c;
+      for (int e = t; e < VB * nX; e += NF) {
+        const int w = e / nX, f = e % nX;
+        const int p = f % nx, b = (f / nx) % 7, c = f / (nx * 7);
+        const double* wt = W[I.s0[0] + p];
+        const double* cc = sC[w] + 7 * b + 49 * c;
         double r = 0.0;
 #pragma unroll
-        for (int a = 0; a < 7; ++a) r += w[a] * cc[a];
-        sX[e] = r;
+        for (int a = 0; a < 7; ++a) r += wt[a] * cc[a];
+        sX[w][f] = r;
       }
       __syncthreads();
       // along y: Y[p][q][c] = sum_b W[s0y + q][b] X[p][b][c]
-      for (int e = t; e < nx * ny * 7; e += NF) {
-        const int p = e % nx, q = (e / nx) % ny, c = e / (nx * ny);
-        const double* w = W[I.s0[1] + q];
+      for (int e = t; e < VB * nY; e += NF) {
+        const int w = e / nY, f = e % nY;
+        const int p = f % nx, q = (f / nx) % ny, c = f / (nx * ny);
+        const double* wt = W[I.s0[1] + q];
         double r = 0.0;
 #pragma unroll
-        for (int b = 0; b < 7; ++b) r += w[b] * sX[p + nx * (b + 7 * c)];
-        sY[e] = r;
+        for (int b = 0; b < 7; ++b) r += wt[b] * sX[w][p + nx * (b + 7 * c)];
+        sY[w][f] = r;
       }
       __syncthreads();
       // along z, into the fine block's padding
-      gdouble* dst = (gdouble*)u.v[v] + I.dst;
-      for (int e = t; e < nx * ny * nz; e += NF) {
-        const int p = e % nx, q = (e / nx) % ny, r3 = e / (nx * ny);
-        const double* w = W[I.s0[2] + r3];
+      for (int e = t; e < VB * nZ; e += NF) {
+        const int w = e / nZ, f = e % nZ;
+        const int p = f % nx, q = (f / nx) % ny, r3 = f / (nx * ny);
+        const double* wt = W[I.s0[2] + r3];
         double r = 0.0;
 #pragma unroll
-        for (int c = 0; c < 7; ++c) r += w[c] * sY[p + nx * (q + ny * c)];
+        for (int c = 0; c < 7; ++c) r += wt[c] * sY[w][p + nx * (q + ny * c)];
+        gdouble* dst = (gdouble*)u.v[v0 + w] + I.dst;
         dst[p + (uint64_t)I.dsz[0] * (q + (uint64_t)I.dsz[1] * r3)] = r;
       }
     }
@@ -268,7 +277,7 @@ int dgr_internal::launch_level_fill(dgr_plan_t plan, const OutPtrs& u, hipStream
                        u);
   }
   if (plan->nfill_items) {
-    const uint32_t g = std::min<uint32_t>(plan->nfill_items, 8192);
+    const uint32_t g = std::min<uint32_t>(plan->nfill_items, 16384);
     hipLaunchKernelGGL(k_fill_interp, dim3(g), dim3(NF), 0, st, plan->d_fill_items, plan->nfill_items, u);
   }
   DGR_HIP(hipGetLastError());
