@@ -275,6 +275,9 @@ def main():
                     help="cap on resident interior points (0: memory budget only)")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-autotune", action="store_true", help="library default schedule instead of the autotuner's")
+    ap.add_argument("--calibration-copy-gib", type=float, default=0.0,
+                    help="first copy a float64 tensor of this many GiB three times: a plain copy of known size "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box here has one GPU); "
@@ -285,6 +288,14 @@ def main():
     from pydgr import RK4, Plan, data, layout, make_params
 
     dev = torch.device("cuda:0")
+    if a.calibration_copy_gib > 0:
+        src = torch.ones(int(a.calibration_copy_gib * 2 ** 27), dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        torch.cuda.empty_cache()
     dx = 1.0 / 64
     blocks, dof = layout.uniform_grid(a.nb, ele_order=6, dx=dx, outer_bflags=True)
     pts = layout.interior_points(blocks)
