@@ -219,12 +219,30 @@ __device__ __forceinline__ uint32_t gather_codes(const TileGeom& T, int t) {
   return code;
 }
 
-__device__ __forceinline__ void stage(const double (&r)[NPF], double* su, const TileGeom& T, int t) {
+// zmask (boundary tiles): bit m set when slot m lies beyond a flagged face.
+// Those padding values are read only by interior formulas that the face
+// closures then replace; staging them as 0 keeps every value the closure
+// sums touch finite, whatever the caller left in the padding.
+__device__ __forceinline__ void stage(const double (&r)[NPF], double* su, const TileGeom& T, int t, uint32_t zmask) {
 #pragma unroll
   for (int m = 0; m < NPF; ++m) {
     const int e = t + m * NT;
-    if (e < NSU) su[e] = r[m];
+    if (e < NSU) su[e] = ((zmask >> m) & 1u) ? 0.0 : r[m];
   }
+}
+
+// Slots of this thread beyond a flagged face (boundary tiles).
+__device__ __forceinline__ uint32_t beyond_face_mask(const TileGeom& T, unsigned bflag, int t) {
+  uint32_t z = 0;
+#pragma unroll
+  for (int m = 0; m < NPF; ++m) {
+    const int e = t + m * NT;
+    const int i = T.i0 - 3 + e % SX, j = T.j0 - 3 + (e / SX) % SY, k = T.k0 - 3 + e / (SX * SY);
+    const bool out = ((bflag & 1u) && i < 3) || ((bflag & 2u) && i >= T.sz0 - 3) || ((bflag & 4u) && j < 3) ||
+                     ((bflag & 8u) && j >= T.sz1 - 3) || ((bflag & 16u) && k < 3) || ((bflag & 32u) && k >= T.sz2 - 3);
+    z |= (uint32_t)out << m;
+  }
+  return z;
 }
 
 // Per-thread context of one tile.
@@ -315,7 +333,26 @@ __device__ __forceinline__ void axis_ops(const double* A, int o, int s, int q, c
           interior = f == F_D1 || f == F_D2 || f == F_ADV_P || f == F_ADV_N || f == F_KO;
         }
         if (interior) continue;
-        const double r = eval_form(f, A, c, s, x);
+        // the closure rows stay within the register window (+-4 for the
+        // 6th-order rows); staged values are finite, so no masking
+        // (q-4 is read only on the high side and q+4 only on the low side,
+        // where they lie inside the block's interior)
+        double w[dgr_forms::W];
+        w[0] = (ORD == 64 && q >= x.e - 3) ? A[c - 4 * s] : 0.0;
+        w[1] = m3; w[2] = m2; w[3] = m1; w[4] = z0; w[5] = p1; w[6] = p2; w[7] = p3;
+        w[8] = (ORD == 64 && q <= 5) ? A[c + 4 * s] : 0.0;
+        const Form& F = s_forms[f];
+        double sum;
+        {
+#pragma clang fp contract(off)
+          double up = 0.0, dn = 0.0;
+#pragma unroll
+          for (int m = 0; m < dgr_forms::W; ++m) up = up + F.c[m] * w[m];
+#pragma unroll
+          for (int m = dgr_forms::W - 1; m >= 0; --m) dn = dn + F.c[m] * w[m];
+          sum = F.desc ? dn : up;
+        }
+        const double r = form_scale(sum, F.sel, &x.ih2);
         if (op == 0) d1 = r;
         else if (op == 1) d2 = r;
         else if (op == 2) ad = r;
@@ -468,11 +505,11 @@ __device__ __forceinline__ void plane_y(const double* su, double* sgy, const Til
 template <int I, bool BND, int ORD, bool GATHER>
 __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], double* su, double* sgx,
                                          double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io,
-                                         int t, const long long* sd, uint32_t gcode) {
+                                         int t, const long long* sd, uint32_t gcode, uint32_t zmask) {
   if constexpr (I < 24) {
     constexpr int V = h_order[I];
     using G = Planes<ORD>;
-    stage(pf, su, T, t);
+    stage(pf, su, T, t, zmask);
     __syncthreads();
     Mixed M;
     if constexpr (has_d2(V) && G::SHARED) {
@@ -508,7 +545,7 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
     }
     process_var<V, BND, ORD>(s, C, su, sgx, sgy, M, T, io);
     __syncthreads();
-    run_vars<I + 1, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, io, t, sd, gcode);
+    run_vars<I + 1, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, io, t, sd, gcode, zmask);
   }
 }
 
@@ -627,7 +664,8 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
       }
       s.begin();
     }
-    run_vars<0, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, s_delta, gcode);
+    const uint32_t zmask = BND ? beyond_face_mask(T, B.bflag, t) : 0u;
+    run_vars<0, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, s_delta, gcode, zmask);
   }
 }
 
