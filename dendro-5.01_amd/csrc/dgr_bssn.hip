@@ -49,7 +49,10 @@ using namespace dgr_internal;
 
 constexpr int TX = 7, TY = 7, TZ = 7;         // interior tile
 constexpr int NPT = TX * TY * TZ;              // 343 points
-constexpr int NT = 384;                        // 6 waves
+// 8 waves: threads [0, 343) own a point each; all threads share the
+// staging, prefetch and derivative-plane loops (8 waves per CU fit the
+// 256-VGPR budget and hide more latency than 6)
+constexpr int NT = 512;
 constexpr int SX = TX + 6, SY = TY + 6, SZ = TZ + 6;  // staged region
 constexpr int NSU = SX * SY * SZ;              // 2197
 constexpr int NPF = (NSU + NT - 1) / NT;       // prefetch registers per thread
@@ -135,8 +138,14 @@ __device__ __forceinline__ double d1_op(const double* A, int o, int s, int q, co
   const double m3 = ORD == 64 ? A[c - 3 * s] : 0.0, p3 = ORD == 64 ? A[c + 3 * s] : 0.0;
   double r = d1_int<ORD>(m3, m2, m1, p1, p2, p3, x);
   if (BND && near_face(q, x)) {
-    const int f = ORD == 64 ? dgr_forms::form64_d1(q, x.e, x.lf, x.hf) : dgr_forms::form_d1(q, x.e, x.lf, x.hf);
-    if (f != (ORD == 64 ? (int)dgr_forms::G_D1 : (int)dgr_forms::F_D1)) r = eval_form(f, A, c, s, x);
+    if constexpr (ORD == 64) {
+      const int f = dgr_forms::form64_d1(q, x.e, x.lf, x.hf);
+      if (f != (int)dgr_forms::G_D1) r = eval_form(f, A, c, s, x);
+    } else {
+      // the closure rows read q-2 .. q+2 on the interior side of the face
+      const int f = dgr_forms::form_d1(q, x.e, x.lf, x.hf);
+      if (f != (int)dgr_forms::F_D1) r = dgr_forms::closure42_d1(f, m2, m1, A[c], p1, p2, &x.ih2);
+    }
   }
   return r;
 }
@@ -310,29 +319,37 @@ __device__ __forceinline__ void axis_ops(const double* A, int o, int s, int q, c
     ad = (-m3 + 6.0 * m2 - 18.0 * m1 + 10.0 * z0 + 3.0 * p1) * x.ih12;
   }
   ko = x.pre * (-m3 + 6.0 * m2 - 15.0 * m1 + 20.0 * z0 - 15.0 * p1 + 6.0 * p2 - p3);
-  if constexpr (BND) {
-    // closures: one generic row evaluation per operator whose form differs
-    // from the interior one at this point
+  if constexpr (BND && ORD == 42) {
+    // closures: the written-out deriv42 rows over the register window, for
+    // each operator whose row differs from the interior one at this point
+    if (near_face(q, x)) {
+      using namespace dgr_forms;
+      const double* sc = &x.ih2;
+      if constexpr (D1) {
+        const int f = form_d1(q, x.e, x.lf, x.hf);
+        if (f != F_D1) d1 = closure42_d1(f, m2, m1, z0, p1, p2, sc);
+      }
+      if constexpr (D2) {
+        const int f = form_d2(q, x.e, x.lf, x.hf);
+        if (f != F_D2) d2 = closure42_d2(f, m3, m2, m1, z0, p1, p2, p3, sc);
+      }
+      const int fa = form_adv(q, x.e, x.lf, x.hf, beta > 0.0);
+      if (fa != F_ADV_P && fa != F_ADV_N) ad = closure42_adv(fa, m3, m2, m1, z0, p1, p2, p3, sc);
+      const int fk = form_ko(q, x.e, x.lf, x.hf);
+      if (fk != F_KO) ko = closure42_ko(fk, m3, m2, m1, z0, p1, p2, p3, sc);
+    }
+  } else if constexpr (BND) {
+    // 6th-order closures: one generic row evaluation per operator whose form
+    // differs from the interior one at this point
     if (near_face(q, x)) {
       using namespace dgr_forms;
 #pragma unroll 1
       for (int op = D1 ? 0 : (D2 ? 1 : 2); op < 4; ++op) {
         if (op == 1 && !D2) continue;
-        int f;
-        bool interior;
-        if constexpr (ORD == 64) {
-          f = op == 0 || op == 2 ? form64_d1(q, x.e, x.lf, x.hf)
-            : op == 1            ? form64_d2(q, x.e, x.lf, x.hf)
-                                 : form64_ko(q, x.e, x.lf, x.hf);
-          interior = f == G_D1 || f == G_D2 || f == G_KO;
-        } else {
-          f = op == 0 ? form_d1(q, x.e, x.lf, x.hf)
-            : op == 1 ? form_d2(q, x.e, x.lf, x.hf)
-            : op == 2 ? form_adv(q, x.e, x.lf, x.hf, beta > 0.0)
-                      : form_ko(q, x.e, x.lf, x.hf);
-          interior = f == F_D1 || f == F_D2 || f == F_ADV_P || f == F_ADV_N || f == F_KO;
-        }
-        if (interior) continue;
+        const int f = op == 0 || op == 2 ? form64_d1(q, x.e, x.lf, x.hf)
+                    : op == 1            ? form64_d2(q, x.e, x.lf, x.hf)
+                                         : form64_ko(q, x.e, x.lf, x.hf);
+        if (f == G_D1 || f == G_D2 || f == G_KO) continue;
         // the closure rows stay within the register window (+-4 for the
         // 6th-order rows); staged values are finite, so no masking
         // (q-4 is read only on the high side and q+4 only on the low side,

@@ -261,6 +261,67 @@ DGR_HD inline double form_scale(double sum, int sel, const double* scales) {
   return (sel >= S_SMR1 && sel <= S_SMR3) ? sum / s : sum * s;
 }
 
+// The deriv42 rows of DGR_FORMS_TABLE written out over the 7-point window
+// m3 .. p3 (points q-3 .. q+3; every deriv42 row lies inside it), term by
+// term in the row's summation order (descending for the low-face KO rows),
+// then scaled as form_scale does.  The zero terms of a row are left out:
+// the running sum starts at +0.0, so adding +0.0 never changes it (except
+// for the sign of an all-zero result).  One function per operator family;
+// f outside the family's closure rows gives the family's interior row.
+DGR_HD inline double closure42_d1(int f, double m2, double m1, double z0, double p1, double p2,
+                                  const double* sc) {
+#pragma clang fp contract(off)
+  switch (f) {
+    case F_D1_L3: return ((-3.0 * z0 + 4.0 * p1) - p2) * sc[S_IH2];
+    case F_D1_L4:
+    case F_D1_H2: return (-m1 + p1) * sc[S_IH2];
+    case F_D1_H1: return ((m2 - 4.0 * m1) + 3.0 * z0) * sc[S_IH2];
+    default: return (((m2 - 8.0 * m1) + 8.0 * p1) - p2) * sc[S_IH12];
+  }
+}
+DGR_HD inline double closure42_d2(int f, double m3, double m2, double m1, double z0, double p1, double p2,
+                                  double p3, const double* sc) {
+#pragma clang fp contract(off)
+  switch (f) {
+    case F_D2_L3: return (((2.0 * z0 - 5.0 * p1) + 4.0 * p2) - p3) * sc[S_ISQ];
+    case F_D2_L4:
+    case F_D2_H2: return ((m1 - 2.0 * z0) + p1) * sc[S_ISQ];
+    case F_D2_H1: return (((-m3 + 4.0 * m2) - 5.0 * m1) + 2.0 * z0) * sc[S_ISQ];
+    default: return ((((-m2 + 16.0 * m1) - 30.0 * z0) + 16.0 * p1) - p2) * sc[S_ISQ12];
+  }
+}
+DGR_HD inline double closure42_adv(int f, double m3, double m2, double m1, double z0, double p1, double p2,
+                                   double p3, const double* sc) {
+#pragma clang fp contract(off)
+  switch (f) {
+    case F_ADV_L3:
+    case F_ADV_L4P:
+    case F_ADV_H3P: return ((-3.0 * z0 + 4.0 * p1) - p2) * sc[S_IH2];
+    case F_ADV_L4N:
+    case F_ADV_H2P: return (-m1 + p1) * sc[S_IH2];
+    case F_ADV_L5N:
+    case F_ADV_H1:
+    case F_ADV_H2N: return ((m2 - 4.0 * m1) + 3.0 * z0) * sc[S_IH2];
+    case F_ADV_N:
+    case F_ADV_H3N: return ((((-m3 + 6.0 * m2) - 18.0 * m1) + 10.0 * z0) + 3.0 * p1) * sc[S_IH12];
+    default: return ((((-3.0 * m1 - 10.0 * z0) + 18.0 * p1) - 6.0 * p2) + p3) * sc[S_IH12];  // P, L5P
+  }
+}
+DGR_HD inline double closure42_ko(int f, double m3, double m2, double m1, double z0, double p1, double p2,
+                                  double p3, const double* sc) {
+#pragma clang fp contract(off)
+  switch (f) {
+    case F_KO_L3: return (((p3 - 3.0 * p2) + 3.0 * p1) - z0) / sc[S_SMR3];
+    case F_KO_L4: return ((((p3 - 6.0 * p2) + 12.0 * p1) - 10.0 * z0) + 3.0 * m1) / sc[S_SMR2];
+    case F_KO_L5: return (((((p3 - 6.0 * p2) + 15.0 * p1) - 19.0 * z0) + 12.0 * m1) - 3.0 * m2) / sc[S_SMR1];
+    case F_KO_H3: return (((((m3 - 6.0 * m2) + 15.0 * m1) - 19.0 * z0) + 12.0 * p1) - 3.0 * p2) / sc[S_SMR1];
+    case F_KO_H2: return ((((m3 - 6.0 * m2) + 12.0 * m1) - 10.0 * z0) + 3.0 * p1) / sc[S_SMR2];
+    case F_KO_H1: return (((m3 - 3.0 * m2) + 3.0 * m1) - z0) / sc[S_SMR3];
+    default:
+      return ((((((-m3 + 6.0 * m2) - 15.0 * m1) + 20.0 * z0) - 15.0 * p1) + 6.0 * p2) - p3) * sc[S_PRE];
+  }
+}
+
 }  // namespace dgr_forms
 
 #endif

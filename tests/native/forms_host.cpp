@@ -2,7 +2,9 @@
 // padded block, for comparison with the oracle's operators in
 // tests/test_stencil_forms.py.  Writes interior points along `axis` only
 // and leaves every other point untouched.  kind: 0..3 deriv42 first,
-// second, advective, KO; 4..6 deriv64 first, second, advective (= first).
+// second, advective, KO; 4..6 deriv64 first, second, advective (= first);
+// 7..10 the deriv42 operators of kinds 0..3 through the written-out rows
+// (closure42_*) that the boundary kernels evaluate.
 #include <cstdint>
 
 #include "../../dendro-5.01_amd/csrc/stencil_forms.h"
@@ -32,6 +34,17 @@ extern "C" void forms_deriv(int kind, int axis, double* Du, const double* u, dou
         for (int m = 0; m < W; ++m) {
           const int p = q + m - 4;
           w[m] = (p >= 0 && p < n[axis]) ? u[c + (m - 4) * st[axis]] : 0.0;
+        }
+        if (kind >= 7) {
+          const bool pos = beta[c] > 0.0;
+          double r;
+          if (kind == 7) r = closure42_d1(form_d1(q, e, lf, hf), w[2], w[3], w[4], w[5], w[6], scales);
+          else if (kind == 8) r = closure42_d2(form_d2(q, e, lf, hf), w[1], w[2], w[3], w[4], w[5], w[6], w[7], scales);
+          else if (kind == 9)
+            r = closure42_adv(form_adv(q, e, lf, hf, pos), w[1], w[2], w[3], w[4], w[5], w[6], w[7], scales);
+          else r = closure42_ko(form_ko(q, e, lf, hf), w[1], w[2], w[3], w[4], w[5], w[6], w[7], scales);
+          Du[c] = r;
+          continue;
         }
         const Form* F;
         if (kind == 0) F = &kForms[form_d1(q, e, lf, hf)];
