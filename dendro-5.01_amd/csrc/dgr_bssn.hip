@@ -259,11 +259,24 @@ struct Ctx {
   bool act, on_face;
   unsigned bflag;
   int ti, tj, tk, gi, gj, gk;
-  double x, y, z, inv_r, eta, sigma;
+  double eta, sigma;
   uint64_t o;  // global index of the point
-  const AxisC* ax;  // per-axis constants of the block (device memory)
+  const AxisC* ax;  // per-axis constants of the block (LDS)
+  const double* geo;  // pmin[3], h[3] of the block (LDS)
   dgr::Params P;
 };
+
+// Coordinates of block point (gi, gj, gk) and its distance from the origin
+// (rhs.cpp:208-213: x = pmin + i hx).  Recomputed where needed rather than
+// kept live across the variable loop.
+__device__ __forceinline__ double point_coords(const double* geo, int gi, int gj, int gk, double& x, double& y,
+                                               double& z) {
+#pragma clang fp contract(off)
+  x = geo[0] + gi * geo[3];
+  y = geo[1] + gj * geo[4];
+  z = geo[2] + gk * geo[5];
+  return sqrt(x * x + y * y + z * z);
+}
 
 // rhs = eq + (KO), or (Sommerfeld + KO) on flagged faces (rhs.cpp:265-373)
 __device__ __forceinline__ void emit(const IOPtrs* __restrict__ io, const Ctx& C, int v, double eq, double ex) {
@@ -443,7 +456,9 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
     const double ko = C.sigma * (k0 + k1 + k2);
     if (BND && C.on_face) {
       const double f = su[c];
-      const double bc = -C.inv_r * (C.x * d1[0] + C.y * d1[1] + C.z * d1[2] + c_falloff[V] * (f - c_asym[V]));
+      double x, y, z;
+      const double inv_r = 1.0 / point_coords(C.geo, C.gi, C.gj, C.gk, x, y, z);
+      const double bc = -inv_r * (x * d1[0] + y * d1[1] + z * d1[2] + c_falloff[V] * (f - c_asym[V]));
       ex = bc + ko;
     } else {
       ex = ko;
@@ -578,6 +593,7 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
   __shared__ double sacc[NSLOT * NPT];
   __shared__ IOPtrs s_io;           // array pointers of this launch
   __shared__ AxisC s_ax[3];         // stencil constants of the current block
+  __shared__ double s_geo[6];       // pmin[3], h[3] of the current block
   __shared__ long long s_delta[GATHER ? NNBR : 1];  // gather mode: source distance per ghost direction
 
   const int t = threadIdx.x;
@@ -623,7 +639,10 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
     if (t < 24) s_io.u.v[t] = io->u.v[t];
     else if (t < 48) s_io.r.v[t - 24] = io->r.v[t - 24];
     else if (t < 51) s_ax[t - 48] = B.ax[t - 48];
+    else if (t < 54) s_geo[t - 51] = B.pmin[t - 51];
+    else if (t < 57) s_geo[t - 51] = B.h[t - 54];
     C.ax = s_ax;
+    C.geo = s_geo;
 
     if constexpr (GATHER) {
       // source distance of each ghost direction: the same point of the
@@ -658,12 +677,10 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
     C.eta = P.eta_const;
     if (C.act) {
 #pragma clang fp contract(off)
-      C.x = B.pmin[0] + C.gi * B.h[0];
-      C.y = B.pmin[1] + C.gj * B.h[1];
-      C.z = B.pmin[2] + C.gk * B.h[2];
-      const double rr = sqrt(C.x * C.x + C.y * C.y + C.z * C.z);
+      double x, y, z;
+      const double geo[6] = {B.pmin[0], B.pmin[1], B.pmin[2], B.h[0], B.h[1], B.h[2]};
+      const double rr = point_coords(geo, C.gi, C.gj, C.gk, x, y, z);
       if (rr >= P.eta_r0) C.eta *= pow(P.eta_r0 / rr, P.eta_damping_exp);
-      C.inv_r = 1.0 / rr;
       const unsigned bf = C.bflag;
       C.on_face = ((bf & 1u) && C.gi == 3) || ((bf & 2u) && C.gi == T.sz0 - 4) || ((bf & 4u) && C.gj == 3) ||
                   ((bf & 8u) && C.gj == T.sz1 - 4) || ((bf & 16u) && C.gk == 3) ||
