@@ -32,7 +32,7 @@ Two emitters exist:
                   pieces as named intermediates, advection pre-contracted
                   with beta) for the GPU kernels; fewer live values.
 
-Run ``python bssn_sym.py --out-oracle ... --out-device ...`` to regenerate.
+Run ``python bssn_sym.py --out-oracle ... --out-io ... --out-physcon-io ...`` to regenerate.
 """
 from __future__ import annotations
 
@@ -354,13 +354,60 @@ def emit_io_header(fname):
         f.write("\n".join(lines) + "\n")
 
 
+# Point inputs of the constraint equations (BSSN_GR/src/physconeqs.cpp,
+# derivatives listed in BSSN_GR/scripts/constraint_derivs.h): state values,
+# the point coordinates, first derivatives of 17 variables and second
+# derivatives of chi and the conformal metric.
+PHYSCON_STATE = ["chi", "K", "Gt0", "Gt1", "Gt2", "gt0", "gt1", "gt2", "gt3", "gt4", "gt5",
+                 "At0", "At1", "At2", "At3", "At4", "At5"]
+PHYSCON_GRAD2 = ["chi", "gt0", "gt1", "gt2", "gt3", "gt4", "gt5"]
+PHYSCON_OUT = ["ham", "mom0", "mom1", "mom2", "psi4_real", "psi4_img"]
+
+
+def physcon_input_names():
+    names = list(PHYSCON_STATE) + ["px", "py", "pz"]
+    for v in PHYSCON_STATE:
+        for i in E_I:
+            names.append("grad_%d_%s" % (i, v))
+    for v in PHYSCON_GRAD2:
+        for (i, j) in SYM_PAIRS:
+            names.append("grad2_%d_%d_%s" % (i, j, v))
+    return names
+
+
+def emit_physcon_io(fname):
+    """X-macro list of the constraint point inputs / outputs (oracle and
+    reference harness)."""
+    names = physcon_input_names()
+    lines = ["/* generated by dendro-5.01_amd/codegen/bssn_sym.py (emit_physcon_io); do not edit.",
+             " * Point inputs of the constraint equations in a fixed order: 17 state",
+             " * values, the coordinates px, py, pz, 51 first derivatives, 42 second",
+             " * derivatives.  Outputs in VAR_CONSTRAINT order (grDef.h:34). */",
+             "#ifndef DGR_PHYSCON_POINT_IO_H", "#define DGR_PHYSCON_POINT_IO_H",
+             "#define DGR_PHYSCON_NIN %d" % len(names),
+             "#define DGR_PHYSCON_INPUTS(X) \\"]
+    for k, n in enumerate(names):
+        lines.append("  X(%s, %d) \\" % (n, k))
+    lines.append("")
+    lines.append("#define DGR_PHYSCON_OUTPUTS(X) \\")
+    for k, n in enumerate(PHYSCON_OUT):
+        lines.append("  X(%s, %d) \\" % (n, k))
+    lines.append("")
+    lines.append("#endif")
+    with open(fname, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out-oracle", help="CSE point body for the CPU oracle")
     ap.add_argument("--out-io", help="X-macro header of point inputs/outputs")
+    ap.add_argument("--out-physcon-io", help="X-macro header of the constraint point inputs/outputs")
     args = ap.parse_args(argv)
     if args.out_io:
         emit_io_header(args.out_io)
+    if args.out_physcon_io:
+        emit_physcon_io(args.out_physcon_io)
     if not args.out_oracle:
         return
     s = BSSNSystem().build()

@@ -703,6 +703,387 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Constraint monitor: physical_constraints (BSSN_GR/src/physcon.cpp:12-96)
+// ---------------------------------------------------------------------------
+// Hamiltonian and momentum constraints and the Weyl scalar psi4 on the
+// interior points of every block.  Derivatives are deriv42 whatever the
+// evolution's stencil order (BSSN_GR/include/physcon.h:8-14): first
+// derivatives of 17 variables and second derivatives of chi and the
+// conformal metric, mixed ones by composition (BSSN_GR/scripts/
+// constraint_derivs.h), with the face closures on flagged faces.  The
+// equations are those of GR/rhs_scripts/bssn/constraints.py.  The Ricci
+// tensor and the Christoffels come from the staged point machinery of the
+// right-hand side (bssn_staged.h); the momentum and psi4 terms that are
+// linear in the derivatives of At are summed as each At component is
+// processed, the rest is evaluated at the end.
+
+// Processing order: metric (Christoffels), chi (closes them), Gt (Ricci), K, At.
+constexpr int con_order[17] = {dgr::V_SG0, dgr::V_SG1, dgr::V_SG2, dgr::V_SG3, dgr::V_SG4, dgr::V_SG5,
+                               dgr::V_CHI, dgr::V_GT0, dgr::V_GT1, dgr::V_GT2, dgr::V_K,
+                               dgr::V_SA0, dgr::V_SA1, dgr::V_SA2, dgr::V_SA3, dgr::V_SA4, dgr::V_SA5};
+__host__ __device__ constexpr bool con_d2(int v) {
+  return v == dgr::V_CHI || (v >= dgr::V_SG0 && v <= dgr::V_SG5);
+}
+
+// Per-point constraint sums kept in registers.
+struct ConAcc {
+  double r[3], mr[3], mi[3];  // tetrad: unit radial vector, m = (mr + i mi) (constraints.py)
+  double mom[3];              // sum_jk g^jk d_k At_ij
+  double p3r, p3i;            // derivative parts of psi4_3 (before the 1/chi)
+  double Gt[3], dK[3];
+  bool axis;                  // within 1e-7 of the z axis: psi4 = 0 (physconeqs.cpp:1039-1044)
+};
+
+// Tetrad by Gram-Schmidt in the physical metric gt/chi (constraints.py).
+__device__ __forceinline__ void con_tetrad(const KPoint& s, double x, double y, double z, ConAcc& A) {
+  const double ichi = 1.0 / s.chi;
+  auto dot = [&](const double* a, const double* b) {
+    double v = 0.0;
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) v += s.gt[dgr::sp(i, j)] * ichi * a[i] * b[j];
+    return v;
+  };
+  double r[3] = {x, y, z};
+  double th[3] = {x * z, y * z, -(x * x + y * y)};
+  double ph[3] = {-y, x, 0.0};
+  const double nr = sqrt(dot(r, r));
+  for (int i = 0; i < 3; ++i) r[i] /= nr;
+  {
+    const double p1 = dot(th, th), p2 = dot(r, th);
+    for (int i = 0; i < 3; ++i) th[i] -= p2 * r[i];
+    const double n = sqrt(p1 - p2 * p2);
+    for (int i = 0; i < 3; ++i) th[i] /= n;
+  }
+  {
+    const double p1 = dot(ph, ph), p2 = dot(r, ph), p3 = dot(th, ph);
+    for (int i = 0; i < 3; ++i) ph[i] -= p2 * r[i] + p3 * th[i];
+    const double n = sqrt(p1 - p2 * p2 - p3 * p3);
+    for (int i = 0; i < 3; ++i) ph[i] /= n;
+  }
+  const double is2 = 0.7071067811865475244;
+  for (int i = 0; i < 3; ++i) {
+    A.r[i] = r[i];
+    A.mr[i] = th[i] * is2;
+    A.mi[i] = ph[i] * is2;
+  }
+}
+
+template <int V, bool BND>
+__device__ __forceinline__ void process_con(KPoint& s, ConAcc& A, const Ctx& C, const double* su,
+                                            const double* sgx, const double* sgy, const TileGeom& T) {
+  using namespace dgr;
+  if (!C.act) return;
+  const AxisC* ax = C.ax;
+  const int ti = C.ti, tj = C.tj, tk = C.tk, gi = C.gi, gj = C.gj, gk = C.gk;
+  const int c = (ti + 3) + SX * ((tj + 3) + SY * (tk + 3));
+  const int ox = c - gi, oy = c - gj * SX, oz = c - gk * (SX * SY);
+  constexpr bool D2 = con_d2(V);
+  double d1[3] = {0.0, 0.0, 0.0};
+  double d2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double av, kv;  // unused operators of axis_ops
+  axis_ops<BND, !D2, D2, 42>(su, ox, 1, gi, ax[0], 0.0, d1[0], d2[0], av, kv);
+  axis_ops<BND, !D2, D2, 42>(su, oy, SX, gj, ax[1], 0.0, d1[1], d2[3], av, kv);
+  axis_ops<BND, true, D2, 42>(su, oz, SX * SY, gk, ax[2], 0.0, d1[2], d2[5], av, kv);
+  if constexpr (D2) {
+    constexpr int GXY = Planes<42>::GXY;
+    d1[0] = sgx[ti + TX * ((tj + 2) + GXY * (tk + 2))];
+    d1[1] = sgy[ti + TX * (tj + TY * (tk + 2))];
+    d2[1] = d1_op<BND, 42>(sgx, ti + TX * ((2 - T.j0) + GXY * (tk + 2)), TX, gj, ax[1]);
+    d2[2] = d1_op<BND, 42>(sgx, ti + TX * ((tj + 2) + GXY * (2 - T.k0)), TX * GXY, gk, ax[2]);
+    d2[4] = d1_op<BND, 42>(sgy, ti + TX * (tj + TY * (2 - T.k0)), TX * TY, gk, ax[2]);
+  }
+  if constexpr (V >= V_SG0 && V <= V_SG5) s.add_gt(V - V_SG0, d1, d2, 0.0);
+  if constexpr (V == V_CHI) s.add_chi(d1, d2, 0.0);
+  if constexpr (V >= V_GT0 && V <= V_GT2) {
+    s.add_Gt(C.P, V - V_GT0, d1, 0.0);
+    A.Gt[V - V_GT0] = su[c];
+  }
+  if constexpr (V == V_K) {
+    for (int i = 0; i < 3; ++i) A.dK[i] = d1[i];
+  }
+  if constexpr (V >= V_SA0 && V <= V_SA5) {
+    constexpr int p = V - V_SA0, a = spi(p), b = spj(p);
+    // sum_jk g^jk d_k At_ij for (i, j) = (a, b) and (b, a)
+    A.mom[a] += s.igt[sp(b, 0)] * d1[0] + s.igt[sp(b, 1)] * d1[1] + s.igt[sp(b, 2)] * d1[2];
+    if constexpr (a != b) A.mom[b] += s.igt[sp(a, 0)] * d1[0] + s.igt[sp(a, 1)] * d1[1] + s.igt[sp(a, 2)] * d1[2];
+    // psi4_3: U_ij, V_ij hold m^k d_j At_ki, for (k, i) = (a, b) and (b, a)
+#pragma unroll
+    for (int h = 0; h < (a != b ? 2 : 1); ++h) {
+      const int k = h ? b : a, i = h ? a : b;
+      for (int j = 0; j < 3; ++j) {
+        const double MRij = A.mr[i] * A.r[j] - A.mr[j] * A.r[i];
+        const double NRij = A.mi[i] * A.r[j] - A.mi[j] * A.r[i];
+        A.p3r += (MRij * A.mr[k] - NRij * A.mi[k]) * d1[j];
+        A.p3i += (NRij * A.mr[k] + MRij * A.mi[k]) * d1[j];
+      }
+    }
+  }
+}
+
+template <int I, bool BND>
+__device__ __forceinline__ void con_vars(KPoint& s, ConAcc& A, Ctx& C, double (&pf)[NPF], double* su, double* sgx,
+                                         double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io, int t,
+                                         uint32_t zmask) {
+  if constexpr (I < 17) {
+    constexpr int V = con_order[I];
+    stage(pf, su, T, t, zmask);
+    __syncthreads();
+    if constexpr (con_d2(V)) {
+      plane_x<BND, 42>(su, sgx, T, C.ax, t);
+      plane_y<BND, 42>(su, sgy, T, C.ax, t);
+    }
+    if constexpr (I + 1 < 17) prefetch<false>(pf, io->u.v[con_order[I + 1]], T, t, nullptr, 0u);
+    if constexpr (con_d2(V)) __syncthreads();
+    if constexpr (V == dgr::V_CHI) {
+      if (C.act) s.finish_metric();
+    }
+    process_con<V, BND>(s, A, C, su, sgx, sgy, T);
+    __syncthreads();
+    con_vars<I + 1, BND>(s, A, C, pf, su, sgx, sgy, T, io, t, zmask);
+  }
+}
+
+// Ham, mom, psi4 from the completed sums (constraints.py).
+__device__ __forceinline__ void con_finish(const KPoint& s, const ConAcc& A, double out[6]) {
+  using namespace dgr;
+  const double chi = s.chi, ichi = 1.0 / chi, K = s.K;
+  double Rm[6], trR = 0.0, sqrA = 0.0;
+  for (int p = 0; p < 6; ++p) {
+    Rm[p] = s.R[p];
+    trR += spw(p) * s.igt[p] * Rm[p];
+    sqrA += spw(p) * s.At[p] * s.AUU[p];
+  }
+  out[0] = chi * trR - sqrA + (2.0 / 3.0) * K * K;
+  // Gamma^m_ki from the slots: C2[m*6 + sp(k, i)]
+  double G[3][6];
+  for (int m = 0; m < 3; ++m)
+    for (int p = 0; p < 6; ++p) G[m][p] = s.C2[m * 6 + p];
+  for (int i = 0; i < 3; ++i) {
+    double cc = 0.0, gA = 0.0, dc = 0.0;
+    for (int j = 0; j < 3; ++j) {
+      for (int k = 0; k < 3; ++k) {
+        double g = 0.0;
+        for (int m = 0; m < 3; ++m) g += G[m][sp(k, i)] * s.At[sp(j, m)];
+        cc += s.igt[sp(j, k)] * g;
+        dc += s.igt[sp(j, k)] * s.At[sp(k, i)] * s.dchi[j];
+      }
+      gA += A.Gt[j] * s.At[sp(i, j)];
+    }
+    out[1 + i] = A.mom[i] - cc - gA - 1.5 * dc * ichi - (2.0 / 3.0) * A.dK[i];
+  }
+  if (A.axis) {
+    out[4] = 0.0;
+    out[5] = 0.0;
+    return;
+  }
+  const double* r = A.r;
+  const double* mr = A.mr;
+  const double* mi = A.mi;
+  double Av[3];
+  for (int i = 0; i < 3; ++i) Av[i] = s.At[sp(i, 0)] * r[0] + s.At[sp(i, 1)] * r[1] + s.At[sp(i, 2)] * r[2];
+  double p1r = 0.0, p1i = 0.0, p2r = 0.0, p2i = 0.0, p3r = A.p3r, p3i = A.p3i;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const double MM = mr[i] * mr[j] - mi[i] * mi[j];
+      const double NN = mr[i] * mi[j] + mr[j] * mi[i];
+      p1r += Rm[sp(i, j)] * MM;
+      p1i += Rm[sp(i, j)] * NN;
+      p2r += s.At[sp(i, j)] * MM;
+      p2i += s.At[sp(i, j)] * NN;
+      // non-derivative part of U_ij, V_ij: m^k Gamma^m_ki At_mj
+      double cu = 0.0, cv = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        double g = 0.0;
+        for (int m = 0; m < 3; ++m) g += G[m][sp(k, i)] * s.At[sp(m, j)];
+        cu += mr[k] * g;
+        cv += mi[k] * g;
+      }
+      const double MRij = mr[i] * r[j] - mr[j] * r[i];
+      const double NRij = mi[i] * r[j] - mi[j] * r[i];
+      p3r += MRij * cu - NRij * cv;
+      p3i += NRij * cu + MRij * cv;
+    }
+  double rdchi = 0.0, rA = 0.0, mrdchi = 0.0, midchi = 0.0, mrA = 0.0, miA = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    rdchi += r[i] * s.dchi[i];
+    rA += Av[i] * r[i];
+    mrdchi += mr[i] * s.dchi[i];
+    midchi += mi[i] * s.dchi[i];
+    mrA += mr[i] * Av[i];
+    miA += mi[i] * Av[i];
+  }
+  const double Atmp = ichi * ichi * (rA + K * chi / 3.0 + 0.5 * rdchi);
+  p2r *= Atmp;
+  p2i *= Atmp;
+  p3r *= ichi;
+  p3i *= ichi;
+  const double p4r = ichi * ichi * (mrA * (mrA + 0.5 * mrdchi) - miA * (miA + 0.5 * midchi));
+  const double p4i = ichi * ichi * (mrA * (miA - 0.5 * midchi) + miA * (mrA - 0.5 * mrdchi));
+  out[4] = p1r + p2r - p3r - p4r;
+  out[5] = -(p1i + p2i - p3i - p4i);
+}
+
+template <bool BND>
+__global__ void __launch_bounds__(NT) k_physcon(const DevBlock* __restrict__ blocks,
+                                                const DevTile* __restrict__ tiles, int tile_begin, int ntiles,
+                                                const IOPtrs* __restrict__ io, DevParams P) {
+  __shared__ double su[NSU];
+  __shared__ double splanes[Planes<42>::N];
+  double* const sgx = splanes;
+  double* const sgy = splanes + Planes<42>::NGX;
+  __shared__ double sacc[NSLOT * NPT];
+  __shared__ IOPtrs s_io;      // u: state, r[0..5]: ham, mom0..2, psi4_real, psi4_img
+  __shared__ AxisC s_ax[3];
+  __shared__ double s_geo[6];
+
+  const int t = threadIdx.x;
+  Ctx C;
+  if (t < NPT) {
+    const int m = c_tile_map[t];
+    C.ti = m & 7;
+    C.tj = (m >> 3) & 7;
+    C.tk = m >> 6;
+  } else {
+    C.ti = C.tj = C.tk = TX;
+  }
+  for (int i = 0; i < 4; ++i) C.P.lambda[i] = P.lambda[i];
+  C.P.lambda_f[0] = P.lambda_f[0];
+  C.P.lambda_f[1] = P.lambda_f[1];
+  C.sigma = 0.0;
+  C.eta = 0.0;
+  C.on_face = false;
+
+  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    const DevTile tl = tiles[tile_begin + w];
+    const DevBlock& B = blocks[tl.block];
+    TileGeom T;
+    T.off = B.offset;
+    T.sz0 = B.sz[0]; T.sz1 = B.sz[1]; T.sz2 = B.sz[2];
+    T.i0 = tl.i0; T.j0 = tl.j0; T.k0 = tl.k0;
+    T.nx = tl.nx; T.ny = tl.ny; T.nz = tl.nz;
+    T.rx = T.nx + 6; T.ry = T.ny + 6; T.rz = T.nz + 6;
+    T.nreg = T.rx * T.ry * T.rz;
+    C.act = t < NPT && C.ti < T.nx && C.tj < T.ny && C.tk < T.nz;
+    C.gi = T.i0 + C.ti; C.gj = T.j0 + C.tj; C.gk = T.k0 + C.tk;
+    C.bflag = B.bflag;
+    if (t < 24) s_io.u.v[t] = io->u.v[t];
+    else if (t < 30) s_io.r.v[t - 24] = io->r.v[t - 24];
+    else if (t < 33) s_ax[t - 30] = B.ax[t - 30];
+    else if (t < 36) s_geo[t - 33] = B.pmin[t - 33];
+    else if (t < 39) s_geo[t - 33] = B.h[t - 36];
+    C.ax = s_ax;
+    C.geo = s_geo;
+    double pf[NPF];
+    prefetch<false>(pf, io->u.v[con_order[0]], T, t, nullptr, 0u);
+    __syncthreads();
+
+    KPoint s;
+    s.C2.p = sacc + t;
+    s.R.p = sacc + 18 * NPT + t;
+    s.gt_rhs.p = sacc + 24 * NPT + t;
+    s.At_rhs.p = sacc + 30 * NPT + t;
+    s.acc.p = sacc + 36 * NPT + t;
+    C.o = T.off + (uint64_t)C.gi + (uint64_t)T.sz0 * (C.gj + (uint64_t)T.sz1 * C.gk);
+    ConAcc A;
+    if (C.act) {
+      const uint64_t o = C.o;
+      s.alpha = 1.0;
+      s.chi = gmem(s_io.u.v[dgr::V_CHI])[o];
+      s.K = gmem(s_io.u.v[dgr::V_K])[o];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) s.beta[i] = 0.0;
+#pragma unroll
+      for (int p = 0; p < 6; ++p) {
+        s.gt[p] = gmem(s_io.u.v[dgr::V_SG0 + p])[o];
+        s.At[p] = gmem(s_io.u.v[dgr::V_SA0 + p])[o];
+      }
+      s.begin();
+      double x, y, z;
+      point_coords(C.geo, C.gi, C.gj, C.gk, x, y, z);
+      A.axis = fabs(x) <= 1e-7 && fabs(y) <= 1e-7;
+      if (!A.axis) con_tetrad(s, x, y, z, A);
+      for (int i = 0; i < 3; ++i) A.mom[i] = 0.0;
+      A.p3r = A.p3i = 0.0;
+    }
+    const uint32_t zmask = BND ? beyond_face_mask(T, B.bflag, t) : 0u;
+    con_vars<0, BND>(s, A, C, pf, su, sgx, sgy, T, &s_io, t, zmask);
+    if (C.act) {
+      double out[6];
+      con_finish(s, A, out);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) gmem(s_io.r.v[q])[C.o] = out[q];
+    }
+    __syncthreads();  // s_io, s_ax, s_geo are rewritten by the next tile
+  }
+}
+
+// Masked constraint norms (computeConstraintL2Norm, BSSN_GR/include/
+// grUtils.tcc:40-88): over the block-interior points where the mask is at
+// least the threshold, each node once (a point on a block's high face is
+// left to a same-level neighbour on that side, which holds it on its low
+// face).  One workgroup per tile writes 13 partials: sum of squares and max
+// |.| of the 6 outputs, and the point count.
+struct Ptr6 {
+  const double* v[6];
+};
+constexpr int NORM_T = 256;
+
+__global__ void __launch_bounds__(NORM_T) k_con_norm_partial(const DevBlock* __restrict__ blocks,
+                                                             const DevTile* __restrict__ tiles, int ntiles,
+                                                             const int32_t* __restrict__ nbr, Ptr6 con,
+                                                             const double* __restrict__ mask, double thr,
+                                                             double* __restrict__ part) {
+  __shared__ double red[NORM_T];
+  const int t = threadIdx.x;
+  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    const DevTile tl = tiles[w];
+    const DevBlock& B = blocks[tl.block];
+    const int32_t* nb = nbr + (size_t)tl.block * NNBR;
+    double sq[6] = {0, 0, 0, 0, 0, 0}, mx[6] = {0, 0, 0, 0, 0, 0}, cnt = 0.0;
+    const int np = tl.nx * tl.ny * tl.nz;
+    for (int e = t; e < np; e += NORM_T) {
+      const int i = tl.i0 + e % tl.nx, j = tl.j0 + (e / tl.nx) % tl.ny, k = tl.k0 + e / (tl.nx * tl.ny);
+      if ((i == (int)B.sz[0] - 4 && nb[14] >= 0) || (j == (int)B.sz[1] - 4 && nb[16] >= 0) ||
+          (k == (int)B.sz[2] - 4 && nb[22] >= 0))
+        continue;
+      const uint64_t o = B.offset + (uint64_t)i + (uint64_t)B.sz[0] * (j + (uint64_t)B.sz[1] * k);
+      if (mask[o] < thr) continue;
+      cnt += 1.0;
+      for (int q = 0; q < 6; ++q) {
+        const double v = con.v[q][o];
+        sq[q] += v * v;
+        mx[q] = fmax(mx[q], fabs(v));
+      }
+    }
+    // fixed-order tree reductions (deterministic)
+    for (int q = 0; q < 13; ++q) {
+      red[t] = q < 6 ? sq[q] : (q < 12 ? mx[q - 6] : cnt);
+      __syncthreads();
+      for (int h = NORM_T / 2; h > 0; h >>= 1) {
+        if (t < h) red[t] = q >= 6 && q < 12 ? fmax(red[t], red[t + h]) : red[t] + red[t + h];
+        __syncthreads();
+      }
+      if (t == 0) part[(size_t)w * 13 + q] = red[0];
+      __syncthreads();
+    }
+  }
+}
+
+// Sum of the per-tile partials in tile order; out[13] = L2[6], max[6], count.
+__global__ void __launch_bounds__(64) k_con_norm_final(const double* __restrict__ part, int ntiles,
+                                                       double* __restrict__ out) {
+  const int q = threadIdx.x;
+  if (q >= 13) return;
+  double a = 0.0;
+  for (int w = 0; w < ntiles; ++w) {
+    const double v = part[(size_t)w * 13 + q];
+    a = (q >= 6 && q < 12) ? fmax(a, v) : a + v;
+  }
+  out[q] = a;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -727,6 +1108,8 @@ void free_plan(dgr_plan_s* plan) {
   (void)hipFree(plan->d_nbr);
   (void)hipFree(plan->d_fill_items);
   (void)hipFree(plan->d_fill_copies);
+  (void)hipFree(plan->d_io_con);
+  (void)hipFree(plan->d_norm);
   delete plan;
 }
 
@@ -1094,6 +1477,69 @@ extern "C" int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const
                        plan->params, (const int32_t*)nullptr);
   }
   DGR_HIP(hipGetLastError());
+  return DGR_OK;
+}
+
+extern "C" int dgr_physcon(dgr_plan_t plan, double* const* con, const double* const* uin, void* stream) {
+  if (!plan || !con || !uin) return DGR_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  IOPtrs io{};
+  for (int v = 0; v < 24; ++v) {
+    if (!uin[v]) return DGR_E_INVALID;
+    io.u.v[v] = uin[v];
+  }
+  for (int q = 0; q < DGR_NUM_CONSTRAINTS; ++q) {
+    if (!con[q]) return DGR_E_INVALID;
+    io.r.v[q] = con[q];
+  }
+  if (!plan->d_io_con) DGR_HIP(hipMalloc(&plan->d_io_con, sizeof(IOPtrs)));
+  if (!plan->io_con_valid || std::memcmp(&io, &plan->h_io_con, sizeof(IOPtrs)) != 0) {
+    plan->h_io_con = io;
+    DGR_HIP(hipMemcpyAsync(plan->d_io_con, &plan->h_io_con, sizeof(IOPtrs), hipMemcpyHostToDevice, st));
+    plan->io_con_valid = true;
+  }
+  const int ni = (int)plan->ntiles_interior;
+  const int nbd = (int)plan->ntiles - ni;
+  auto grid = [&](int n) { return plan->grid_mult == 0 ? n : std::min(plan->grid * (int)plan->grid_mult, n); };
+  if (ni > 0)
+    hipLaunchKernelGGL(k_physcon<false>, dim3(grid(ni)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, 0, ni,
+                       plan->d_io_con, plan->params);
+  if (nbd > 0)
+    hipLaunchKernelGGL(k_physcon<true>, dim3(grid(nbd)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, ni, nbd,
+                       plan->d_io_con, plan->params);
+  DGR_HIP(hipGetLastError());
+  return DGR_OK;
+}
+
+extern "C" int dgr_physcon_norms(dgr_plan_t plan, const double* const* con, const double* mask, double threshold,
+                                 double* out, void* stream) {
+  if (!plan || !con || !mask || !out) return DGR_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  Ptr6 c6;
+  for (int q = 0; q < DGR_NUM_CONSTRAINTS; ++q) {
+    if (!con[q]) return DGR_E_INVALID;
+    c6.v[q] = con[q];
+  }
+  const int nt = (int)plan->ntiles;
+  if (nt == 0) {
+    for (int q = 0; q < 13; ++q) out[q] = 0.0;
+    return DGR_OK;
+  }
+  if (!plan->d_norm) DGR_HIP(hipMalloc(&plan->d_norm, sizeof(double) * 13 * ((size_t)nt + 1)));
+  double* res = plan->d_norm + (size_t)13 * nt;
+  hipLaunchKernelGGL(k_con_norm_partial, dim3(std::min(nt, 8192)), dim3(NORM_T), 0, st, plan->d_blocks,
+                     plan->d_tiles_blk, nt, plan->d_nbr, c6, mask, threshold, plan->d_norm);
+  hipLaunchKernelGGL(k_con_norm_final, dim3(1), dim3(64), 0, st, plan->d_norm, nt, res);
+  DGR_HIP(hipGetLastError());
+  double h[13];
+  DGR_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
+  DGR_HIP(hipStreamSynchronize(st));
+  const double n = h[12];
+  for (int q = 0; q < 6; ++q) {
+    out[q] = n > 0 ? std::sqrt(h[q] / n) : 0.0;
+    out[6 + q] = h[6 + q];
+  }
+  out[12] = n;
   return DGR_OK;
 }
 

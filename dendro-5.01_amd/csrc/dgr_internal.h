@@ -120,6 +120,11 @@ struct dgr_plan_s {
   uint32_t nfill_items = 0;
   dgr_internal::FillCopy* d_fill_copies = nullptr;
   uint64_t nfill_copies = 0;
+  // constraint monitor: pointer table, norm partials (allocated on first use)
+  dgr_internal::IOPtrs* d_io_con = nullptr;
+  dgr_internal::IOPtrs h_io_con{};
+  bool io_con_valid = false;
+  double* d_norm = nullptr;  // 13 x ntiles partials + 13 results
 };
 
 namespace dgr_internal {

@@ -124,6 +124,8 @@ def lib():
         L.dgr_ghost_refill.argtypes = [vp, pp, vp]
         L.dgr_plan_level_fills.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]
         L.dgr_enforce_constraints.argtypes = [vp, pp, vp]
+        L.dgr_physcon.argtypes = [vp, pp, pp, vp]
+        L.dgr_physcon_norms.argtypes = [vp, pp, vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double), vp]
         L.dgr_rk4_create.argtypes = [ctypes.POINTER(vp), vp, pp, ctypes.c_int]
         L.dgr_rk4_work_doubles.restype = ctypes.c_uint64
         L.dgr_rk4_work_doubles.argtypes = [vp]
@@ -147,6 +149,10 @@ def check(rc, what=""):
 def ptr_array(ptrs):
     arr = (ctypes.c_void_p * len(ptrs))(*[ctypes.c_void_p(int(p)) for p in ptrs])
     return ctypes.cast(arr, ctypes.POINTER(ctypes.c_void_p)), arr
+
+
+NUM_CONSTRAINTS = 6
+CONSTRAINT_NAMES = ("ham", "mom0", "mom1", "mom2", "psi4_real", "psi4_img")
 
 
 def var_ptrs(t):
@@ -233,6 +239,26 @@ class Plan:
             return False
         check(rc, "dgr_enforce_constraints")
         return True
+
+    def physcon(self, con, u, stream=None):
+        """Constraint monitor (physical_constraints): con is a (6, dof)
+        float64 tensor (ham, mom0..2, psi4_real, psi4_img), written on the
+        interior points of every block."""
+        assert con.dim() == 2 and con.shape[0] == NUM_CONSTRAINTS
+        cp, _c = ptr_array([con[q].data_ptr() for q in range(NUM_CONSTRAINTS)])
+        up, _u = ptr_array(var_ptrs(u))
+        check(lib().dgr_physcon(self.handle, cp, up, _stream_ptr(stream)), "dgr_physcon")
+
+    def physcon_norms(self, con, mask, threshold=0.3, stream=None):
+        """Masked L2 and max norms of the 6 constraint outputs over the
+        nodes where mask >= threshold (alpha and 0.3 by default in the
+        reference).  Returns a dict."""
+        cp, _c = ptr_array([con[q].data_ptr() for q in range(NUM_CONSTRAINTS)])
+        out = (ctypes.c_double * 13)()
+        check(lib().dgr_physcon_norms(self.handle, cp, ctypes.c_void_p(mask.data_ptr()), float(threshold), out,
+                                      _stream_ptr(stream)), "dgr_physcon_norms")
+        return {"l2": dict(zip(CONSTRAINT_NAMES, out[0:6])), "max": dict(zip(CONSTRAINT_NAMES, out[6:12])),
+                "points": int(out[12])}
 
     def close(self):
         if getattr(self, "handle", None):

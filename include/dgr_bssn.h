@@ -151,6 +151,26 @@ int dgr_plan_level_fills(dgr_plan_t plan, uint32_t* boxes, uint64_t* copies);
  * det < 0 are left unchanged.  Synchronous. */
 int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* stream);
 
+/* Constraint monitor.  Outputs in VAR_CONSTRAINT order
+ * (BSSN_GR/include/grDef.h:34): ham, mom0, mom1, mom2, psi4_real, psi4_img. */
+#define DGR_NUM_CONSTRAINTS 6
+/* physical_constraints (BSSN_GR/src/physcon.cpp:12-96, declared in
+ * BSSN_GR/include/physcon.h:17-20) over every block of the plan: con[6]
+ * device arrays of the same unzipped layout as u[24], written on interior
+ * points only.  deriv42 stencils with face closures whatever the plan's
+ * deriv_order (the reference's physcon.h:8-14).  Asynchronous on stream. */
+int dgr_physcon(dgr_plan_t plan, double* const* con, const double* const* u, void* stream);
+/* Masked constraint norms (computeConstraintL2Norm,
+ * BSSN_GR/include/grUtils.tcc:40-88, called per constraint by
+ * extractConstraints, grUtils.tcc:91-130): over the interior points whose
+ * mask value (the reference's EXTRACTION_VAR_ID, alpha by default,
+ * parameters.cpp:157-158) is >= threshold, each node once (a block's high
+ * face is left to a same-level neighbour on that side).  out[13] (host):
+ * sqrt(mean of squares) of the 6 outputs, their max |.|, and the point
+ * count.  Deterministic (fixed reduction order).  Synchronous. */
+int dgr_physcon_norms(dgr_plan_t plan, const double* const* con, const double* mask, double threshold,
+                      double* out, void* stream);
+
 /* Device-resident RK4 evolution of the 24 state arrays u (caller-owned
  * device memory, ghost layers filled).  One step replaces the RK4 branch of
  * RK_BSSN::performSingleIteration (BSSN_GR/src/rkBSSN.cpp:803-1011):

@@ -52,6 +52,9 @@ def load(fast=False):
         L.orc_enforce_constraints.restype = ctypes.c_int
         L.orc_rk4_stage.argtypes = [dp, dp, dp, ctypes.c_double, ctypes.c_long]
         L.orc_rk4_final.argtypes = [dp, dp, ctypes.POINTER(dp), ctypes.c_double, ctypes.c_long]
+        L.orc_physcon_point.argtypes = [ctypes.c_long, dp, dp]
+        L.orc_physcon.argtypes = [ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.c_uint64, dp, dp,
+                                  ctypes.POINTER(ctypes.c_uint), ctypes.c_uint]
         _libs[name] = L
     return _libs[name]
 
@@ -104,6 +107,34 @@ def point_eqs(inp, eta, prm=None):
     out = np.zeros((NUM_VARS, n))
     L.orc_point_eqs(n, _p(inp), _p(np.ascontiguousarray(eta, dtype=np.float64)), ctypes.byref(prm), _p(out))
     return out
+
+
+PHYSCON_NIN = 113
+PHYSCON_NAMES = ["ham", "mom0", "mom1", "mom2", "psi4_real", "psi4_img"]
+
+
+def physcon_point(inp):
+    """Constraint point equations: inp (113, n) in physcon_point_io.h order
+    -> (6, n) ham, mom0..2, psi4_real, psi4_img."""
+    L = load()
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    assert inp.shape[0] == PHYSCON_NIN
+    out = np.zeros((6, inp.shape[1]))
+    L.orc_physcon_point(inp.shape[1], _p(inp), _p(out))
+    return out
+
+
+def physcon_blocks(U, blocks, fill=np.nan):
+    """physical_constraints over all blocks of a (24, dof) array -> (6, dof);
+    non-interior points keep `fill`."""
+    L = load()
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    C = np.full((6, U.shape[1]), fill)
+    for b in blocks:
+        szc = (ctypes.c_uint * 3)(*b["sz"])
+        L.orc_physcon(_rows(C), _rows(U), b["offset"], _p(np.asarray(b["pmin"], dtype=np.float64)),
+                      _p(np.asarray(b["pmax"], dtype=np.float64)), szc, b["bflag"])
+    return C
 
 
 def enforce_constraints(U, chi_floor=0.1):

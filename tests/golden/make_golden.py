@@ -1,13 +1,19 @@
 """Write the golden point-equation vectors from the reference's own
-generated BSSN equations (BSSN_GR/src/bssneqs_eta_const_standard_gauge.cpp,
-compiled by oracle/Makefile `make ref` into oracle/_ref/libref_eqs.so).
+generated equations, compiled by oracle/Makefile `make ref`:
+
+  point_eqs_*.npz  BSSN_GR/src/bssneqs_eta_const_standard_gauge.cpp
+                   (oracle/_ref/libref_eqs.so): the 207 point inputs in the
+                   order of oracle/bssn_point_io.h, the per-point eta,
+                   lambda / lambda_f, and the 24 right-hand sides;
+  physcon_*.npz    BSSN_GR/src/physconeqs.cpp (oracle/_ref/libref_physcon.so):
+                   the 113 point inputs in the order of
+                   oracle/physcon_point_io.h and the 6 constraint outputs
+                   (ham, mom0..2, psi4_real, psi4_img).
 
     python tests/golden/make_golden.py
 
-Each fixture holds the 207 point inputs in the order of
-oracle/bssn_point_io.h, the per-point eta, lambda / lambda_f, and the 24
-right-hand sides the reference computes.  Inputs are seeded and stored, so
-the fixture does not depend on a random-number stream.
+Inputs are seeded and stored, so a fixture does not depend on a
+random-number stream.
 """
 import ctypes
 import hashlib
@@ -19,7 +25,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_eqs.so")
+REF_PHYSCON = os.path.join(ROOT, "oracle", "_ref", "libref_physcon.so")
 NIN = 207
+NIN_PHYSCON = 113
 dp = ctypes.POINTER(ctypes.c_double)
 
 
@@ -52,6 +60,35 @@ SETS = {
 }
 
 
+def ref_physcon_point(inp):
+    L = ctypes.CDLL(REF_PHYSCON)
+    L.ref_physcon_point.argtypes = [ctypes.c_long, dp, dp]
+    n = inp.shape[1]
+    out = np.zeros((6, n))
+    L.ref_physcon_point(n, np.ascontiguousarray(inp, dtype=np.float64).ctypes.data_as(dp), out.ctypes.data_as(dp))
+    return out
+
+
+def physcon_inputs(rng, n, scale, chi_lo, chi_hi, coord, axis_frac=0.0):
+    x = rng.standard_normal((NIN_PHYSCON, n)) * scale
+    x[0] = rng.uniform(chi_lo, chi_hi, n)        # chi
+    for k in (5, 8, 10):                         # gt diagonal
+        x[k] += 1.0
+    x[17:20] = rng.uniform(-coord, coord, (3, n))  # point coordinates
+    m = rng.random(n) < axis_frac                 # points on the z axis: psi4 = 0 there
+    x[17, m] = 0.0
+    x[18, m] = 0.0
+    return x
+
+
+PHYSCON_SETS = {
+    # perturbed flat space at coordinates of a 64-block domain
+    "near_flat": dict(seed=21, n=256, scale=1e-3, chi=(0.999, 1.001), coord=1.0, axis=0.1),
+    # strong field near a puncture
+    "strong": dict(seed=22, n=256, scale=1e-1, chi=(0.05, 1.0), coord=8.0, axis=0.1),
+}
+
+
 def main():
     manifest = {}
     for name, c in SETS.items():
@@ -64,8 +101,18 @@ def main():
                  out=out)
         manifest[os.path.basename(path)] = hashlib.sha256(open(path, "rb").read()).hexdigest()
         print(name, "max|out| per var", np.abs(out).max(1).round(4).tolist())
+    for name, c in PHYSCON_SETS.items():
+        rng = np.random.default_rng(c["seed"])
+        inp = physcon_inputs(rng, c["n"], c["scale"], *c["chi"], c["coord"], c["axis"])
+        out = ref_physcon_point(inp)
+        path = os.path.join(HERE, "physcon_%s.npz" % name)
+        np.savez(path, inp=inp, out=out)
+        manifest[os.path.basename(path)] = hashlib.sha256(open(path, "rb").read()).hexdigest()
+        print("physcon", name, "max|out| per output", np.abs(out).max(1).round(4).tolist())
     with open(os.path.join(HERE, "MANIFEST.json"), "w") as f:
-        json.dump({"source": "BSSN_GR/src/bssneqs_eta_const_standard_gauge.cpp via oracle/_ref/libref_eqs.so",
+        json.dump({"source": {"point_eqs_*": "BSSN_GR/src/bssneqs_eta_const_standard_gauge.cpp via "
+                                             "oracle/_ref/libref_eqs.so",
+                              "physcon_*": "BSSN_GR/src/physconeqs.cpp via oracle/_ref/libref_physcon.so"},
                    "files": manifest}, f, indent=1)
 
 
