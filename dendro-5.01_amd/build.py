@@ -35,6 +35,7 @@ def build_hip():
     run([HIPCC] + HIP_FLAGS + ["-o", tmp, os.path.join(PKG, "csrc", "dgr_bssn.hip"),
                                os.path.join(PKG, "csrc", "dgr_rk4.hip"),
                                os.path.join(PKG, "csrc", "dgr_levels.hip"),
+                               os.path.join(PKG, "csrc", "dgr_init.hip"),
                                os.path.join(PKG, "host", "bssn_rhs_host.cpp")])
     os.replace(tmp, out)
     return out

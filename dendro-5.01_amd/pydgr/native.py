@@ -125,6 +125,7 @@ def lib():
         L.dgr_plan_level_fills.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]
         L.dgr_enforce_constraints.argtypes = [vp, pp, vp]
         L.dgr_physcon.argtypes = [vp, pp, pp, vp]
+        L.dgr_init_puncture.argtypes = [vp, pp, ctypes.POINTER(Puncture), vp]
         L.dgr_physcon_norms.argtypes = [vp, pp, vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double), vp]
         L.dgr_rk4_create.argtypes = [ctypes.POINTER(vp), vp, pp, ctypes.c_int]
         L.dgr_rk4_work_doubles.restype = ctypes.c_uint64
@@ -149,6 +150,27 @@ def check(rc, what=""):
 def ptr_array(ptrs):
     arr = (ctypes.c_void_p * len(ptrs))(*[ctypes.c_void_p(int(p)) for p in ptrs])
     return ctypes.cast(arr, ctypes.POINTER(ctypes.c_void_p)), arr
+
+
+class Puncture(ctypes.Structure):
+    """dgr_puncture (include/dgr_bssn.h)."""
+    _fields_ = [("mass", ctypes.c_double), ("pos", ctypes.c_double * 3), ("p", ctypes.c_double * 3),
+                ("spin", ctypes.c_double), ("spin_theta", ctypes.c_double), ("spin_phi", ctypes.c_double)]
+
+
+def punctures(holes):
+    """Two hole dicts (mass, pos, optional p, spin, spin_theta, spin_phi)
+    -> ctypes array of 2 dgr_puncture."""
+    arr = (Puncture * 2)()
+    for q, h in enumerate(holes):
+        arr[q].mass = h["mass"]
+        for d in range(3):
+            arr[q].pos[d] = h["pos"][d]
+            arr[q].p[d] = h.get("p", (0.0, 0.0, 0.0))[d]
+        arr[q].spin = h.get("spin", 0.0)
+        arr[q].spin_theta = h.get("spin_theta", 0.0)
+        arr[q].spin_phi = h.get("spin_phi", 0.0)
+    return arr
 
 
 NUM_CONSTRAINTS = 6
@@ -239,6 +261,13 @@ class Plan:
             return False
         check(rc, "dgr_enforce_constraints")
         return True
+
+    def init_puncture(self, u, holes, stream=None):
+        """punctureData initial data at every padded point of every block
+        of the plan (holes: two dicts, see punctures())."""
+        up, _u = ptr_array(var_ptrs(u))
+        arr = punctures(holes)
+        check(lib().dgr_init_puncture(self.handle, up, arr, _stream_ptr(stream)), "dgr_init_puncture")
 
     def physcon(self, con, u, stream=None):
         """Constraint monitor (physical_constraints): con is a (6, dof)

@@ -65,7 +65,8 @@ typedef struct dgr_bssn_params {
   double eta_damping_exp; /* ETA_DAMPING_EXP */
   double ko_sigma;        /* KO_DISS_SIGMA */
   double chi_floor;       /* CHI_FLOOR (constraint projection) */
-  uint32_t deriv_order;   /* 42 (reference deriv42) */
+  uint32_t deriv_order;   /* 42: deriv42 (the reference default); 64: deriv64 first / second /
+                             advective with ko_deriv42 (QuadGrav/include/derivs.h:80-96) */
   uint32_t reserved;
 } dgr_bssn_params;
 
@@ -170,6 +171,23 @@ int dgr_physcon(dgr_plan_t plan, double* const* con, const double* const* u, voi
  * count.  Deterministic (fixed reduction order).  Synchronous. */
 int dgr_physcon_norms(dgr_plan_t plan, const double* const* con, const double* mask, double threshold,
                       double* out, void* stream);
+
+/* One puncture of the initial data: the fields of the reference's BH
+ * (BSSN_GR/include/bh.h:58-80) that punctureData reads. */
+typedef struct {
+  double mass;                  /* getBHMass */
+  double pos[3];                /* getBHCoordX/Y/Z */
+  double p[3];                  /* getVx/Vy/Vz: linear momentum (boost) */
+  double spin;                  /* getBHSpin */
+  double spin_theta, spin_phi;  /* getBHSpinTheta/Phi */
+} dgr_puncture;
+/* punctureData (BSSN_GR/src/grUtils.cpp:398-702) at every padded point of
+ * every block of the plan (x = pmin + i h): Brill-Lindquist conformal
+ * factor with the reference's spin / boost corrections, alpha and chi
+ * floored at params.chi_floor (the reference's CHI_FLOOR), Bowen-York At,
+ * gt = delta, all else zero.  holes[2]; masses must be positive.
+ * Asynchronous on stream. */
+int dgr_init_puncture(dgr_plan_t plan, double* const* u, const dgr_puncture* holes, void* stream);
 
 /* Device-resident RK4 evolution of the 24 state arrays u (caller-owned
  * device memory, ghost layers filled).  One step replaces the RK4 branch of

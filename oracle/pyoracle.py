@@ -53,6 +53,7 @@ def load(fast=False):
         L.orc_rk4_stage.argtypes = [dp, dp, dp, ctypes.c_double, ctypes.c_long]
         L.orc_rk4_final.argtypes = [dp, dp, ctypes.POINTER(dp), ctypes.c_double, ctypes.c_long]
         L.orc_physcon_point.argtypes = [ctypes.c_long, dp, dp]
+        L.orc_puncture_points.argtypes = [ctypes.c_long, dp, dp, ctypes.c_double, dp]
         L.orc_physcon.argtypes = [ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.c_uint64, dp, dp,
                                   ctypes.POINTER(ctypes.c_uint), ctypes.c_uint]
         _libs[name] = L
@@ -135,6 +136,26 @@ def physcon_blocks(U, blocks, fill=np.nan):
         L.orc_physcon(_rows(C), _rows(U), b["offset"], _p(np.asarray(b["pmin"], dtype=np.float64)),
                       _p(np.asarray(b["pmax"], dtype=np.float64)), szc, b["bflag"])
     return C
+
+
+def puncture_holes(holes):
+    """Two hole dicts (mass, pos, p, spin, spin_theta, spin_phi) -> the
+    (20,) array orc_puncture_point takes."""
+    a = []
+    for h in holes:
+        a += [h["mass"], *h["pos"], *h.get("p", (0.0, 0.0, 0.0)), h.get("spin", 0.0), h.get("spin_theta", 0.0),
+              h.get("spin_phi", 0.0)]
+    return np.asarray(a, dtype=np.float64)
+
+
+def puncture_points(xyz, holes, chi_floor=0.1):
+    """punctureData at points xyz (3, n) -> (24, n)."""
+    L = load()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    n = xyz.shape[1]
+    out = np.zeros((24, n))
+    L.orc_puncture_points(n, _p(xyz), _p(puncture_holes(holes)), chi_floor, _p(out))
+    return out
 
 
 def enforce_constraints(U, chi_floor=0.1):
