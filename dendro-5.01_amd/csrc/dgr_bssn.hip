@@ -1110,6 +1110,8 @@ void free_plan(dgr_plan_s* plan) {
   (void)hipFree(plan->d_fill_copies);
   (void)hipFree(plan->d_io_con);
   (void)hipFree(plan->d_norm);
+  (void)hipFree(plan->d_tiles_sub);
+  (void)hipFree(plan->d_io_blk);
   delete plan;
 }
 
@@ -1357,6 +1359,7 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
   std::vector<DevTile> hs = ht;
   order_tiles(hs, 0, plan->ntiles_interior, sc.order, blocks);
   order_tiles(hs, plan->ntiles_interior, hs.size(), sc.order, blocks);
+  plan->h_tiles_blk = ht;
   for (uint32_t b = 0; b < nblocks; ++b)
     plan->dof_end = std::max<uint64_t>(plan->dof_end, blocks[b].offset + (uint64_t)blocks[b].sz[0] * blocks[b].sz[1] *
                                                                            blocks[b].sz[2]);
@@ -1476,6 +1479,75 @@ extern "C" int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const
     hipLaunchKernelGGL(k, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, tb, nt, plan->d_io,
                        plan->params, (const int32_t*)nullptr);
   }
+  DGR_HIP(hipGetLastError());
+  return DGR_OK;
+}
+
+extern "C" int dgr_bssn_rhs_blkwise(dgr_plan_t plan, double* const* rhs, const double* const* uin,
+                                    const uint32_t* blk_ids, uint32_t n, void* stream) {
+  if (!plan || (n > 0 && !blk_ids)) return DGR_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<DevTile> sub;
+  uint32_t nin = 0;
+  // interior blocks' tiles, then boundary blocks' tiles (two kernels)
+  for (int pass = 0; pass < 2; ++pass)
+    for (uint32_t q = 0; q < n; ++q) {
+      const uint32_t b = blk_ids[q];
+      if (b >= plan->nblocks) return DGR_E_INVALID;
+      const uint32_t tb = plan->block_tile_begin[b], tc = plan->block_tile_count[b];
+      if ((pass == 0) != (tb < plan->ntiles_interior)) continue;
+      sub.insert(sub.end(), plan->h_tiles_blk.begin() + tb, plan->h_tiles_blk.begin() + tb + tc);
+      if (pass == 0) nin += tc;
+    }
+  if (sub.empty()) return DGR_OK;
+  int rc = upload_ptrs(plan, rhs, uin, st);
+  if (rc) return rc;
+  if (!plan->d_tiles_sub) DGR_HIP(hipMalloc(&plan->d_tiles_sub, sizeof(DevTile) * plan->ntiles));
+  // the subset list is reused by the next call: wait for earlier launches
+  DGR_HIP(hipStreamSynchronize(st));
+  DGR_HIP(hipMemcpy(plan->d_tiles_sub, sub.data(), sizeof(DevTile) * sub.size(), hipMemcpyHostToDevice));
+  const bool six = plan->order == 64;
+  const auto kin = six ? k_bssn_rhs<false, 64, false> : k_bssn_rhs<false, 42, false>;
+  const auto kbd = six ? k_bssn_rhs<true, 64, false> : k_bssn_rhs<true, 42, false>;
+  const int nbd = (int)sub.size() - (int)nin;
+  auto grid = [&](int m) { return std::min(plan->grid * 2, m); };
+  if (nin > 0)
+    hipLaunchKernelGGL(kin, dim3(grid((int)nin)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_sub, 0, (int)nin,
+                       plan->d_io, plan->params, (const int32_t*)nullptr);
+  if (nbd > 0)
+    hipLaunchKernelGGL(kbd, dim3(grid(nbd)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_sub, (int)nin, nbd,
+                       plan->d_io, plan->params, (const int32_t*)nullptr);
+  DGR_HIP(hipGetLastError());
+  return DGR_OK;
+}
+
+extern "C" int dgr_bssn_rhs_blk(dgr_plan_t plan, double* out, const double* in, uint32_t dof, uint32_t blk_id,
+                                void* stream) {
+  if (!plan || !out || !in || dof != 24 || blk_id >= plan->nblocks) return DGR_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  const uint32_t tb = plan->block_tile_begin[blk_id], tc = plan->block_tile_count[blk_id];
+  // block-local arrays: the kernels address u[v] + offset + local index, so
+  // each pointer is shifted back by the block's offset
+  uint64_t off = 0, nn = 0;
+  {
+    DevBlock B;
+    DGR_HIP(hipMemcpy(&B, plan->d_blocks + blk_id, sizeof(DevBlock), hipMemcpyDeviceToHost));
+    off = B.offset;
+    nn = (uint64_t)B.sz[0] * B.sz[1] * B.sz[2];
+  }
+  IOPtrs io;
+  for (int v = 0; v < 24; ++v) {
+    io.u.v[v] = reinterpret_cast<const double*>(reinterpret_cast<uintptr_t>(in + v * nn) - off * sizeof(double));
+    io.r.v[v] = reinterpret_cast<double*>(reinterpret_cast<uintptr_t>(out + v * nn) - off * sizeof(double));
+  }
+  if (!plan->d_io_blk) DGR_HIP(hipMalloc(&plan->d_io_blk, sizeof(IOPtrs)));
+  DGR_HIP(hipStreamSynchronize(st));
+  DGR_HIP(hipMemcpy(plan->d_io_blk, &io, sizeof(IOPtrs), hipMemcpyHostToDevice));
+  const bool six = plan->order == 64;
+  const auto k = tb < plan->ntiles_interior ? (six ? k_bssn_rhs<false, 64, false> : k_bssn_rhs<false, 42, false>)
+                                            : (six ? k_bssn_rhs<true, 64, false> : k_bssn_rhs<true, 42, false>);
+  hipLaunchKernelGGL(k, dim3(tc), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, (int)tb, (int)tc,
+                     plan->d_io_blk, plan->params, (const int32_t*)nullptr);
   DGR_HIP(hipGetLastError());
   return DGR_OK;
 }

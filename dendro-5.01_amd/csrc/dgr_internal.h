@@ -125,6 +125,11 @@ struct dgr_plan_s {
   dgr_internal::IOPtrs h_io_con{};
   bool io_con_valid = false;
   double* d_norm = nullptr;  // 13 x ntiles partials + 13 results
+  // block-wise RHS: host copy of the block-order tiles, a device list of a
+  // block subset's tiles and a pointer table (allocated on first use)
+  std::vector<dgr_internal::DevTile> h_tiles_blk;
+  dgr_internal::DevTile* d_tiles_sub = nullptr;
+  dgr_internal::IOPtrs* d_io_blk = nullptr;
 };
 
 namespace dgr_internal {

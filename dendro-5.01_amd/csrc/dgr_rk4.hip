@@ -550,3 +550,192 @@ extern "C" int dgr_rk4_destroy(dgr_rk4_t r) {
   destroy(r);
   return DGR_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Explicit Runge-Kutta stepper with a Butcher tableau: the ETS time stepper
+// (ODE/include/ets.h:337-445 tableaux, :469-545 evolve) driving the BSSN
+// context (BSSN_GR/src/bssnCtx.cpp:151-181 rhs: constraint projection of the
+// stage input, unzip, bssnRHS).  Per stage s
+//     tmp = u + sum_{p<s} (a_sp dt) k_p     (p in order, on interior points)
+//     enforce_bssn_constraints(tmp); ghost fill of tmp; k_s = RHS(tmp)
+// then u += (b_k dt) k_k for k in order (VecFMA), and u's ghosts are
+// refilled.  u itself is not projected (the reference projects only the
+// stage inputs).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int ETS_MAX_STAGES = 8;
+struct EtsTerms {
+  double c[ETS_MAX_STAGES];
+  int idx[ETS_MAX_STAGES];
+  int n;
+};
+
+// out = base + sum_q c[q] k_{idx[q]} on the interior points of every tile;
+// k_p variable v at kb + (p*24 + v) * dof.
+__global__ void __launch_bounds__(384) k_ets_combine(const DevBlock* __restrict__ blocks,
+                                                    const DevTile* __restrict__ tiles, int ntiles, Ptrs24 out,
+                                                    Ptrs24 base, const double* __restrict__ kb, uint64_t dof,
+                                                    EtsTerms T) {
+  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    const DevTile tl = tiles[w];
+    const DevBlock& B = blocks[tl.block];
+    uint64_t o;
+    if (!tile_point(tl, B, threadIdx.x, o)) continue;
+    for (int v = 0; v < 24; ++v) {
+#pragma clang fp contract(off)
+      double x = base.v[v][o];
+      for (int q = 0; q < T.n; ++q) x = x + T.c[q] * kb[((uint64_t)T.idx[q] * 24 + v) * dof + o];
+      out.v[v][o] = x;
+    }
+  }
+}
+
+}  // namespace
+
+struct dgr_ets_s {
+  dgr_plan_t plan = nullptr;
+  uint64_t dof = 0;
+  int ns = 0;
+  std::vector<double> a, b, c;  // a[s*ns + p], b[k], c[s]
+  double time = 0.0;
+  double* buf = nullptr;  // tmp (24 x dof) then k_0 .. k_{ns-1} (24 x dof each)
+  Ptrs24 state{}, tmp{};
+  IOPtrs* d_io = nullptr;  // RHS tables: stage s reads tmp, writes k_s
+  unsigned* d_flag = nullptr;
+};
+
+namespace {
+void ets_destroy(dgr_ets_s* e) {
+  (void)hipFree(e->buf);
+  (void)hipFree(e->d_io);
+  (void)hipFree(e->d_flag);
+  delete e;
+}
+}  // namespace
+
+extern "C" int dgr_ets_create_tableau(dgr_ets_t* out, dgr_plan_t plan, double* const* u, const double* aij,
+                                      const double* bi, const double* ci, uint32_t stages) {
+  if (!out || !plan || check_ptrs(u) || !aij || !bi || !ci || stages < 1 || stages > ETS_MAX_STAGES)
+    return DGR_E_INVALID;
+  // explicit: a_sp = 0 for p >= s
+  for (uint32_t s = 0; s < stages; ++s)
+    for (uint32_t p = s; p < stages; ++p)
+      if (aij[s * stages + p] != 0.0) return DGR_E_INVALID;
+  dgr_ets_s* e = new dgr_ets_s();
+  e->plan = plan;
+  e->dof = plan->dof_end;
+  e->ns = (int)stages;
+  e->a.assign(aij, aij + stages * stages);
+  e->b.assign(bi, bi + stages);
+  e->c.assign(ci, ci + stages);
+  for (int v = 0; v < 24; ++v) e->state.v[v] = u[v];
+  const size_t nb = sizeof(double) * e->dof;
+  if (hipMalloc(&e->buf, nb * 24 * (1 + stages)) != hipSuccess ||
+      hipMalloc(&e->d_io, sizeof(IOPtrs) * stages) != hipSuccess || hipMalloc(&e->d_flag, 256) != hipSuccess) {
+    ets_destroy(e);
+    return DGR_E_NOMEM;
+  }
+  std::vector<IOPtrs> io(stages);
+  for (int v = 0; v < 24; ++v) e->tmp.v[v] = e->buf + (size_t)v * e->dof;
+  for (uint32_t s = 0; s < stages; ++s)
+    for (int v = 0; v < 24; ++v) {
+      io[s].u.v[v] = e->tmp.v[v];
+      io[s].r.v[v] = e->buf + ((size_t)(1 + s) * 24 + v) * e->dof;
+    }
+  hipError_t er = hipMemcpy(e->d_io, io.data(), sizeof(IOPtrs) * stages, hipMemcpyHostToDevice);
+  if (er == hipSuccess) er = hipMemset(e->d_flag, 0, 256);
+  if (er == hipSuccess) er = hipMemset(e->buf, 0, nb * 24 * (1 + stages));
+  // ghosts no fill reaches keep the state's values in the stage input
+  for (int v = 0; v < 24 && er == hipSuccess; ++v)
+    er = hipMemcpy(e->tmp.v[v], e->state.v[v], nb, hipMemcpyDeviceToDevice);
+  if (er != hipSuccess) {
+    ets_destroy(e);
+    return hip_fail(er, "dgr_ets_create");
+  }
+  *out = e;
+  return DGR_OK;
+}
+
+extern "C" int dgr_ets_create(dgr_ets_t* out, dgr_plan_t plan, double* const* u, int type) {
+  // ETS::set_ets_coefficients(ETSType) (ODE/include/ets.h:388-436); RK5 is
+  // disabled there and here
+  if (type == DGR_ETS_RK3) {
+    const double a[9] = {0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 1.0 / 4.0, 1.0 / 4.0, 0.0};
+    const double b[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
+    const double c[3] = {0.0, 1.0, 1.0 / 2.0};
+    return dgr_ets_create_tableau(out, plan, u, a, b, c, 3);
+  }
+  if (type == DGR_ETS_RK4) {
+    const double a[16] = {0.0, 0.0, 0.0, 0.0, 1.0 / 2.0, 0.0, 0.0, 0.0, 0.0, 1.0 / 2.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0};
+    const double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    const double c[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
+    return dgr_ets_create_tableau(out, plan, u, a, b, c, 4);
+  }
+  return DGR_E_UNSUPPORTED;
+}
+
+extern "C" int dgr_ets_evolve(dgr_ets_t e, double dt, void* stream) {
+  if (!e || !(dt > 0.0)) return DGR_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  dgr_plan_t p = e->plan;
+  const int nt = (int)p->ntiles;
+  const int g = grid_for(p, nt);
+  const double* kb = e->buf + (size_t)24 * e->dof;
+  OutPtrs tmp_o;
+  for (int v = 0; v < 24; ++v) tmp_o.v[v] = e->tmp.v[v];
+  for (int s = 0; s < e->ns; ++s) {
+    EtsTerms T{};
+    for (int q = 0; q < s; ++q) {
+      const double aq = e->a[(size_t)s * e->ns + q];
+      if (aq == 0.0) continue;  // adding (0 dt) k leaves the sum unchanged
+      T.c[T.n] = aq * dt;
+      T.idx[T.n++] = q;
+    }
+    hipLaunchKernelGGL(k_ets_combine, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->tmp, e->state,
+                       kb, e->dof, T);
+    hipLaunchKernelGGL(k_enforce, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->tmp, p->chi_floor,
+                       e->d_flag);
+    int rc = launch_level_fill(p, tmp_o, st);
+    if (rc) return rc;
+    rc = launch_rhs(p, e->d_io + s, st, true);
+    if (rc) return rc;
+  }
+  EtsTerms T{};
+  for (int k = 0; k < e->ns; ++k) {
+    T.c[k] = e->b[k] * dt;
+    T.idx[k] = k;
+  }
+  T.n = e->ns;
+  hipLaunchKernelGGL(k_ets_combine, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->state, e->state,
+                     kb, e->dof, T);
+  hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(p, (int)p->nblocks)), dim3(RK_NT), 0, st, p->d_blocks, p->d_nbr,
+                     (int)p->nblocks, e->state);
+  DGR_HIP(hipGetLastError());
+  OutPtrs so;
+  for (int v = 0; v < 24; ++v) so.v[v] = e->state.v[v];
+  int rc = launch_level_fill(p, so, st);
+  if (rc) return rc;
+  e->time += dt;
+  return DGR_OK;
+}
+
+extern "C" double dgr_ets_time(dgr_ets_t e) { return e ? e->time : 0.0; }
+
+extern "C" int dgr_ets_status(dgr_ets_t e, void* stream) {
+  if (!e) return DGR_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  unsigned flag = 0;
+  DGR_HIP(hipMemcpyAsync(&flag, e->d_flag, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  DGR_HIP(hipStreamSynchronize(st));
+  DGR_HIP(hipMemsetAsync(e->d_flag, 0, sizeof(unsigned), st));
+  DGR_HIP(hipStreamSynchronize(st));
+  return flag ? DGR_E_CONSTRAINT : DGR_OK;
+}
+
+extern "C" int dgr_ets_destroy(dgr_ets_t e) {
+  if (!e) return DGR_E_INVALID;
+  (void)hipDeviceSynchronize();
+  ets_destroy(e);
+  return DGR_OK;
+}

@@ -126,6 +126,17 @@ def lib():
         L.dgr_enforce_constraints.argtypes = [vp, pp, vp]
         L.dgr_physcon.argtypes = [vp, pp, pp, vp]
         L.dgr_init_puncture.argtypes = [vp, pp, ctypes.POINTER(Puncture), vp]
+        L.dgr_bssn_rhs_blkwise.argtypes = [vp, pp, pp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, vp]
+        L.dgr_bssn_rhs_blk.argtypes = [vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp]
+        L.dgr_ets_create.argtypes = [ctypes.POINTER(vp), vp, pp, ctypes.c_int]
+        L.dgr_ets_create_tableau.argtypes = [ctypes.POINTER(vp), vp, pp, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                             ctypes.c_uint32]
+        L.dgr_ets_evolve.argtypes = [vp, ctypes.c_double, vp]
+        L.dgr_ets_time.argtypes = [vp]
+        L.dgr_ets_time.restype = ctypes.c_double
+        L.dgr_ets_status.argtypes = [vp, vp]
+        L.dgr_ets_destroy.argtypes = [vp]
         L.dgr_physcon_norms.argtypes = [vp, pp, vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double), vp]
         L.dgr_rk4_create.argtypes = [ctypes.POINTER(vp), vp, pp, ctypes.c_int]
         L.dgr_rk4_work_doubles.restype = ctypes.c_uint64
@@ -262,6 +273,20 @@ class Plan:
         check(rc, "dgr_enforce_constraints")
         return True
 
+    def rhs_blkwise(self, rhs, u, blk_ids, stream=None):
+        """RHS of the blocks blk_ids only (BSSNCtx::rhs_blkwise)."""
+        rp, _r = ptr_array(var_ptrs(rhs))
+        up, _u = ptr_array(var_ptrs(u))
+        ids = (ctypes.c_uint32 * len(blk_ids))(*[int(b) for b in blk_ids])
+        check(lib().dgr_bssn_rhs_blkwise(self.handle, rp, up, ids, len(blk_ids), _stream_ptr(stream)),
+              "dgr_bssn_rhs_blkwise")
+
+    def rhs_blk(self, out, inp, blk_id, stream=None):
+        """RHS of one block with block-local (24, NN) arrays (BSSNCtx::rhs_blk)."""
+        assert out.is_contiguous() and inp.is_contiguous() and out.shape == inp.shape and out.shape[0] == NUM_VARS
+        check(lib().dgr_bssn_rhs_blk(self.handle, ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(inp.data_ptr()),
+                                     NUM_VARS, int(blk_id), _stream_ptr(stream)), "dgr_bssn_rhs_blk")
+
     def init_puncture(self, u, holes, stream=None):
         """punctureData initial data at every padded point of every block
         of the plan (holes: two dicts, see punctures())."""
@@ -359,3 +384,56 @@ class RK4:
             self.close()
         except Exception:
             pass
+
+
+ETS_RK3, ETS_RK4 = 0, 1
+
+
+class ETS:
+    """Explicit Runge-Kutta stepper with a Butcher tableau on a plan's
+    device-resident state u (dgr_ets_*): ETS type (ETS_RK3 / ETS_RK4) or an
+    explicit tableau (a, b, c)."""
+
+    def __init__(self, plan, u, type=ETS_RK4, tableau=None):
+        L = lib()
+        self.plan = plan
+        self._u = u
+        up, self._ua = ptr_array(var_ptrs(u))
+        h = ctypes.c_void_p()
+        if tableau is None:
+            check(L.dgr_ets_create(ctypes.byref(h), plan.handle, up, int(type)), "dgr_ets_create")
+        else:
+            a, b, c = (list(map(float, x)) for x in tableau)
+            n = len(b)
+            assert len(a) == n * n and len(c) == n
+            arr = lambda x: (ctypes.c_double * len(x))(*x)  # noqa: E731
+            check(L.dgr_ets_create_tableau(ctypes.byref(h), plan.handle, up, arr(a), arr(b), arr(c), n),
+                  "dgr_ets_create_tableau")
+        self.handle = h
+
+    def evolve(self, dt, nsteps=1, stream=None):
+        for _ in range(nsteps):
+            check(lib().dgr_ets_evolve(self.handle, float(dt), _stream_ptr(stream)), "dgr_ets_evolve")
+
+    @property
+    def time(self):
+        return lib().dgr_ets_time(self.handle)
+
+    def status_ok(self, stream=None):
+        rc = lib().dgr_ets_status(self.handle, _stream_ptr(stream))
+        if rc == -5:
+            return False
+        check(rc, "dgr_ets_status")
+        return True
+
+    def close(self):
+        if self.handle:
+            lib().dgr_ets_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

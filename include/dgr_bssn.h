@@ -127,6 +127,19 @@ int dgr_bssn_rhs(dgr_plan_t plan, double* const* rhs, const double* const* u, vo
  * measured against. */
 int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const double* const* u, void* stream);
 
+/* Block-wise RHS of the time-stepper context (BSSNCtx::rhs_blkwise,
+ * BSSN_GR/include/bssnCtx.h:79-91, BSSN_GR/src/bssnCtx.cpp:186-245): the
+ * blocks blk_ids[0..n) of the plan, same arrays and layout as dgr_bssn_rhs,
+ * one batched launch.  Asynchronous. */
+int dgr_bssn_rhs_blkwise(dgr_plan_t plan, double* const* rhs, const double* const* u, const uint32_t* blk_ids,
+                         uint32_t n, void* stream);
+/* Single-block RHS (BSSNCtx::rhs_blk, BSSN_GR/include/bssnCtx.h:93,
+ * BSSN_GR/src/bssnCtx.cpp:247-305): block blk_id of the plan with its own
+ * block-local arrays, variable v of the padded block at in + v*NN and
+ * out + v*NN (NN = sz0*sz1*sz2), dof = 24.  Asynchronous. */
+int dgr_bssn_rhs_blk(dgr_plan_t plan, double* out, const double* in, uint32_t dof, uint32_t blk_id,
+                     void* stream);
+
 /* One-shot form of bssnRHS: builds a temporary plan, runs, synchronises. */
 int dgr_bssn_rhs_blocks(double* const* rhs, const double* const* u, const dgr_block* blocks,
                         uint32_t nblocks, const dgr_bssn_params* params, void* stream);
@@ -217,6 +230,31 @@ int dgr_rk4_steps(dgr_rk4_t rk, double dt, int nsteps, void* stream);
  * since the last call met a condition the reference exits on. */
 int dgr_rk4_status(dgr_rk4_t rk);
 int dgr_rk4_destroy(dgr_rk4_t rk);
+
+/* Explicit Runge-Kutta time stepper with a Butcher tableau: the reference's
+ * ETS (ODE/include/ets.h:337-445 set_ets_coefficients, :469-545 evolve)
+ * driving the BSSN context (BSSN_GR/src/bssnCtx.cpp:151-181 rhs).  Per stage
+ * s: tmp = u + sum_{p<s} (a_sp dt) k_p on interior points, constraint
+ * projection of tmp (as BSSNCtx::rhs does on its input), ghost fill of tmp,
+ * k_s = RHS(tmp); then u += (b_k dt) k_k for k in order, and u's ghost
+ * layers are refilled.  Stage buffers are allocated at creation (24 x
+ * (1 + stages) padded arrays).  Types follow ETSType (ODE/include/ts.h:26);
+ * RK5 is disabled in the reference and here. */
+typedef struct dgr_ets_s* dgr_ets_t;
+enum { DGR_ETS_RK3 = 0, DGR_ETS_RK4 = 1 };
+int dgr_ets_create(dgr_ets_t* ets, dgr_plan_t plan, double* const* u, int type);
+/* set_ets_coefficients(aij, bi, ci, num_stages): aij row-major stages x
+ * stages, strictly lower triangular (explicit), at most 8 stages. */
+int dgr_ets_create_tableau(dgr_ets_t* ets, dgr_plan_t plan, double* const* u, const double* aij,
+                           const double* bi, const double* ci, uint32_t stages);
+/* One step of size dt (ETS::evolve), asynchronous on stream. */
+int dgr_ets_evolve(dgr_ets_t ets, double dt, void* stream);
+/* Time advanced so far (sum of the dt of all evolve calls). */
+double dgr_ets_time(dgr_ets_t ets);
+/* DGR_E_CONSTRAINT if a stage projection met a point where the reference
+ * exits since the last call; synchronises stream. */
+int dgr_ets_status(dgr_ets_t ets, void* stream);
+int dgr_ets_destroy(dgr_ets_t ets);
 
 #ifdef __cplusplus
 }
