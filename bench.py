@@ -259,6 +259,56 @@ def serving(njobs, budget_gb, max_points):
     return stats
 
 
+def extras(plan, blocks, U, stream, steps, warm):
+    """Timings of the entry points around the RHS on the same grid: the
+    constraint monitor (dgr_physcon + masked norms), puncture initial data
+    (dgr_init_puncture) and one RK3 step of the Butcher-tableau stepper."""
+    import torch
+    from pydgr import ETS, ETS_RK3
+
+    pts = layout_interior_points(blocks)
+    padded = sum(b["sz"][0] * b["sz"][1] * b["sz"][2] for b in blocks)
+
+    def timed(fn, n):
+        for _ in range(warm):
+            fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    C = torch.zeros((6, U.shape[1]), dtype=torch.float64, device=U.device)
+    pc_ms = timed(lambda: plan.physcon(C, U, stream), steps)
+    # bytes: 17 padded inputs read once, 6 interior outputs written once
+    pc_bytes = 8 * (17 * padded + 6 * pts)
+    norm_ms = timed(lambda: plan.physcon_norms(C, U[0], 0.3, stream), max(2, steps // 4))
+    holes = [dict(mass=0.5, pos=(0.1013, 0.0021, 0.0017), p=(0.0, 0.1, 0.0), spin=0.1, spin_theta=0.3),
+             dict(mass=0.5, pos=(-0.0987, -0.0021, 0.0017), p=(0.0, -0.1, 0.0), spin=0.1, spin_theta=0.3)]
+    W = torch.empty_like(U)
+    init_ms = timed(lambda: plan.init_puncture(W, holes, stream), steps)
+    W.copy_(U)
+    ets = ETS(plan, W, type=ETS_RK3)
+    rk3_ms = timed(lambda: ets.evolve(0.25 / 64, 1, stream), max(2, steps // 2))
+    ets.close()
+    del C, W
+    torch.cuda.empty_cache()
+    return {"physcon": {"ms": pc_ms, "points_per_s": pts / (pc_ms * 1e-3),
+                        "algorithmic_bytes": pc_bytes, "achieved_gbs": pc_bytes / (pc_ms * 1e-3) / 1e9,
+                        "hbm_frac": pc_bytes / (pc_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "physcon_norms_ms": norm_ms,
+            "init_puncture": {"ms": init_ms, "padded_points_per_s": padded / (init_ms * 1e-3),
+                              "achieved_gbs": 8 * 24 * padded / (init_ms * 1e-3) / 1e9},
+            "ets_rk3_step": {"ms": rk3_ms, "rhs_points_per_s": 3 * pts / (rk3_ms * 1e-3)}}
+
+
+def layout_interior_points(blocks):
+    return sum((b["sz"][0] - 6) * (b["sz"][1] - 6) * (b["sz"][2] - 6) for b in blocks)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,6 +319,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-adaptive", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the constraint-monitor / initial-data / RK3 timings")
     ap.add_argument("--serving-jobs", type=int, default=24, help="0 skips the serving run")
     ap.add_argument("--serving-budget-gb", type=float, default=200.0)
     ap.add_argument("--serving-max-points", type=int, default=3_000_000,
@@ -322,6 +373,11 @@ def main():
     ms, wall = time_rk4(rk, stream, a.steps, a.warmup)
     log("RK4 step (hipGraph) %.4f ms" % ms)
     rk.close()
+    ext = None
+    if not a.no_extras:
+        ext = extras(plan, blocks, U, stream, a.steps, a.warmup)
+        log("constraint monitor %.4f ms, norms %.4f ms, puncture data %.4f ms, ETS RK3 step %.4f ms" %
+            (ext["physcon"]["ms"], ext["physcon_norms_ms"], ext["init_puncture"]["ms"], ext["ets_rk3_step"]["ms"]))
     plan.close()
     del U
     torch.cuda.empty_cache()
@@ -361,6 +417,7 @@ def main():
                                                  "batched_speedup": ms_pb / rhs_ms,
                                                  "rk4_graph_speedup_per_rhs": ms_pb / (ms / 4)},
         "wall_s_timed_region": wall,
+        "extras": ext,
     }
     if not a.no_uneven:
         line["uneven"] = uneven(stream, a.steps, not a.no_autotune)

@@ -127,3 +127,55 @@ void bssnrhs(double** uzipVarsRHS, const double** uZipVars, const unsigned int& 
   }
   bssnRHS(uzipVarsRHS, uZipVars, &b, 1);
 }
+
+void physical_constraints(double** uZipConVars, const double** uZipVars, const unsigned int& offset,
+                          const double* pmin, const double* pmax, const unsigned int* sz, const unsigned int& bflag) {
+  const dgr_bssn_params p = to_params(dgr::bssn_params());
+  dgr_block b{};
+  b.offset = offset;
+  b.bflag = bflag;
+  for (int d = 0; d < 3; ++d) {
+    b.sz[d] = sz[d];
+    b.pmin[d] = pmin[d];
+    b.pmax[d] = pmax[d];
+  }
+  const size_t n = (size_t)sz[0] * sz[1] * sz[2];
+  bool dev = true;
+  for (int v = 0; v < DGR_NUM_VARS; ++v) dev = dev && is_device(uZipVars[v]);
+  for (int q = 0; q < DGR_NUM_CONSTRAINTS; ++q) dev = dev && is_device(uZipConVars[q]);
+  // host arrays: the block's padded range goes through device memory
+  double* buf = nullptr;
+  const double* up[DGR_NUM_VARS];
+  double* cp[DGR_NUM_CONSTRAINTS];
+  if (dev) {
+    for (int v = 0; v < DGR_NUM_VARS; ++v) up[v] = uZipVars[v];
+    for (int q = 0; q < DGR_NUM_CONSTRAINTS; ++q) cp[q] = uZipConVars[q];
+  } else {
+    hipcheck(hipMalloc(&buf, sizeof(double) * n * (DGR_NUM_VARS + DGR_NUM_CONSTRAINTS)), "hipMalloc");
+    b.offset = 0;
+    for (int v = 0; v < DGR_NUM_VARS; ++v) up[v] = buf + (size_t)v * n;
+    for (int q = 0; q < DGR_NUM_CONSTRAINTS; ++q) cp[q] = buf + (size_t)(DGR_NUM_VARS + q) * n;
+  }
+  dgr_plan_t plan = nullptr;
+  try {
+    if (!dev) {
+      for (int v = 0; v < DGR_NUM_VARS; ++v)
+        hipcheck(hipMemcpy((void*)up[v], uZipVars[v] + offset, sizeof(double) * n, hipMemcpyHostToDevice),
+                 "hipMemcpy");
+      for (int q = 0; q < DGR_NUM_CONSTRAINTS; ++q)
+        hipcheck(hipMemcpy(cp[q], uZipConVars[q] + offset, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
+    }
+    check(dgr_plan_create(&plan, &b, 1, &p), "physical_constraints");
+    check(dgr_physcon(plan, cp, up, nullptr), "physical_constraints");
+    hipcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (!dev)
+      for (int q = 0; q < DGR_NUM_CONSTRAINTS; ++q)
+        hipcheck(hipMemcpy(uZipConVars[q] + offset, cp[q], sizeof(double) * n, hipMemcpyDeviceToHost), "hipMemcpy");
+  } catch (...) {
+    if (plan) dgr_plan_destroy(plan);
+    (void)hipFree(buf);
+    throw;
+  }
+  dgr_plan_destroy(plan);
+  (void)hipFree(buf);
+}

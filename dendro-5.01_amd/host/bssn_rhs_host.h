@@ -4,6 +4,8 @@
  *
  *   bssnRHS(rhs, u, blkList, numBlocks)      BSSN_GR/include/rhs.h:45
  *   bssnrhs(rhs, u, offset, ptmin, ptmax, sz, bflag)   BSSN_GR/include/rhs.h:47-50
+ *   physical_constraints(con, u, offset, pmin, pmax, sz, bflag)
+ *                                            BSSN_GR/include/physcon.h:17-20
  *
  * Same names, argument meaning and data layout: 24 unzipped arrays in VAR
  * order, blocks at `offset`, only interior points of the RHS written.  The
@@ -53,5 +55,10 @@ void bssnRHS(double** uzipVarsRHS, const double** uZipVars, const dgr::BlockDesc
 
 void bssnrhs(double** uzipVarsRHS, const double** uZipVars, const unsigned int& offset, const double* ptmin,
              const double* ptmax, const unsigned int* sz, const unsigned int& bflag);
+
+// con: 6 arrays (ham, mom0..2, psi4_real, psi4_img; VAR_CONSTRAINT order,
+// BSSN_GR/include/grDef.h:34) of the same unzipped layout as u.
+void physical_constraints(double** uZipConVars, const double** uZipVars, const unsigned int& offset,
+                          const double* pmin, const double* pmax, const unsigned int* sz, const unsigned int& bflag);
 
 #endif

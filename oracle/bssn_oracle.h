@@ -3,7 +3,9 @@
  *
  * A plain C++ restatement of Dendro-5's BSSN right-hand side on unzipped,
  * ghost-padded blocks (BSSN_GR/src/rhs.cpp:86-392, BSSN_GR/src/derivs.cpp,
- * BSSN_GR/src/bssn_constraints.cpp:12-172, BSSN_GR/src/rkBSSN.cpp:895-1011).
+ * BSSN_GR/src/bssn_constraints.cpp:12-172, BSSN_GR/src/rkBSSN.cpp:895-1011),
+ * of its constraint monitor (BSSN_GR/src/physcon.cpp) and of its puncture
+ * initial data (BSSN_GR/src/grUtils.cpp:398-702).
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it, and only as the checker or the timed CPU baseline; the product
  * path (libdgr_bssn.so) never calls into it.
@@ -75,6 +77,19 @@ void orc_rk4_stage(double* u_out, const double* u_prev, const double* k, double 
 void orc_rk4_final(double* u, const double* u_prev, const double* const* k4, double dt, long n);
 
 int orc_num_point_inputs(void);
+
+/* physical_constraints on one block (physcon_oracle.cpp): con[6] = ham,
+ * mom0..2, psi4_real, psi4_img on interior points. */
+void orc_physcon(double* const* con, const double* const* u, uint64_t offset, const double* pmin,
+                 const double* pmax, const unsigned* sz, unsigned bflag);
+/* Constraint point equations: in[113*n] (physcon_point_io.h order), out[6*n]. */
+void orc_physcon_point(long n, const double* in, double* out);
+int orc_num_physcon_inputs(void);
+
+/* punctureData at one point / n points (puncture_oracle.cpp); holes: 2 x
+ * (mass, pos[3], p[3], spin, spin_theta, spin_phi). */
+void orc_puncture_point(const double* xyz, const double* holes, double chi_floor, double* out);
+void orc_puncture_points(long n, const double* xyz, const double* holes, double chi_floor, double* out);
 
 #ifdef __cplusplus
 }

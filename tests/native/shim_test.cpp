@@ -1,4 +1,5 @@
-// GPU test driver for the host-side bssnRHS / bssnrhs mirror, in the manner
+// GPU test driver for the host-side bssnRHS / bssnrhs / physical_constraints
+// mirror, in the manner
 // of the reference's rhsTestBed (BSSN_GR/src/rhsTestBed.cpp:181-295):
 // fill padded blocks with smooth data, evaluate the RHS with host arrays,
 // with device arrays and block by block, and report the per-variable
@@ -92,7 +93,32 @@ int main() {
   for (int v = 0; v < 24; ++v) rp[v] = r_orc[v].data();
   for (const auto& b : blocks) orc_bssnrhs(rp, up, b.offset, b.ptmin, b.ptmax, b.sz, b.bflag, &op);
 
+  // 4) constraint monitor, host arrays, block by block
+  std::vector<std::vector<double>> c_host(6, std::vector<double>(dof, -1.0)), c_orc(6, std::vector<double>(dof, -1.0));
+  double* cp[6];
+  for (int q = 0; q < 6; ++q) cp[q] = c_host[q].data();
+  for (const auto& b : blocks) physical_constraints(cp, up, b.offset, b.ptmin, b.ptmax, b.sz, b.bflag);
+  for (int q = 0; q < 6; ++q) cp[q] = c_orc[q].data();
+  for (const auto& b : blocks) orc_physcon(cp, up, b.offset, b.ptmin, b.ptmax, b.sz, b.bflag);
+
   int fail = 0;
+  {
+    double worst = 0.0;
+    for (int q = 0; q < 6; ++q) {
+      double dmax = 0.0, smax = 0.0;
+      for (size_t p = 0; p < dof; ++p) {
+        if (c_orc[q][p] == -1.0) {
+          if (c_host[q][p] != -1.0) { printf("physical_constraints: padding written\n"); fail = 1; break; }
+          continue;
+        }
+        dmax = std::fmax(dmax, std::fabs(c_host[q][p] - c_orc[q][p]));
+        smax = std::fmax(smax, std::fabs(c_orc[q][p]));
+      }
+      worst = std::fmax(worst, smax > 0 ? dmax / smax : dmax);
+    }
+    printf("physical_constraints: max rel err per output %.3e\n", worst);
+    if (!(worst <= 1e-11)) fail = 1;
+  }
   const char* names[3] = {"host arrays", "per-block bssnrhs", "device arrays"};
   std::vector<std::vector<double>>* res[3] = {&r_host, &r_blk, &r_dev};
   for (int c = 0; c < 3; ++c) {
