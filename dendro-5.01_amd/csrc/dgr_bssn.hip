@@ -260,7 +260,8 @@ struct Ctx {
   unsigned bflag;
   int ti, tj, tk, gi, gj, gk;
   double eta, sigma;
-  uint64_t o;  // global index of the point
+  uint64_t o;   // global index of the point
+  uint64_t oo;  // index of its right-hand side (= o, or the compact slot in RK4 stages)
   const AxisC* ax;  // per-axis constants of the block (LDS)
   const double* geo;  // pmin[3], h[3] of the block (LDS)
   dgr::Params P;
@@ -282,15 +283,15 @@ __device__ __forceinline__ double point_coords(const double* geo, int gi, int gj
 __device__ __forceinline__ void emit(const IOPtrs* __restrict__ io, const Ctx& C, int v, double eq, double ex) {
 #pragma clang fp contract(off)
   const double r = C.on_face ? ex : (eq + ex);
-  __builtin_nontemporal_store(r, gmem(io->r.v[v]) + C.o);
+  __builtin_nontemporal_store(r, gmem(io->r.v[v]) + C.oo);
 }
 // Outputs that become final several variables later park their dissipation
 // (+ Sommerfeld) term in the output array and pick it up at emission.
 __device__ __forceinline__ void park(const IOPtrs* __restrict__ io, const Ctx& C, int v, double ex) {
-  gmem(io->r.v[v])[C.o] = ex;
+  gmem(io->r.v[v])[C.oo] = ex;
 }
 __device__ __forceinline__ void emit_parked(const IOPtrs* __restrict__ io, const Ctx& C, int v, double eq) {
-  emit(io, C, v, eq, gmem(io->r.v[v])[C.o]);
+  emit(io, C, v, eq, gmem(io->r.v[v])[C.oo]);
 }
 
 // Per-thread LDS slots: element i of a thread's array sits at p[i * NPT]
@@ -585,7 +586,8 @@ template <bool BND, int ORD, bool GATHER>
 __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ blocks,
                                                  const DevTile* __restrict__ tiles, int tile_begin, int ntiles,
                                                  const IOPtrs* __restrict__ io, DevParams P,
-                                                 const int32_t* __restrict__ nbr) {
+                                                 const int32_t* __restrict__ nbr,
+                                                 const uint32_t* __restrict__ slot) {
   __shared__ double su[NSU];
   __shared__ double splanes[Planes<ORD>::N];
   double* const sgx = splanes;
@@ -673,6 +675,9 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
     s.At_rhs.p = sacc + 30 * NPT + t;
     s.acc.p = sacc + 36 * NPT + t;
     C.o = T.off + (uint64_t)C.gi + (uint64_t)T.sz0 * (C.gj + (uint64_t)T.sz1 * C.gk);
+    // RK4 stages (gather) write k in the compact tile-major layout of the
+    // stage kernels: slot of the tile in block order x 343 + local index
+    C.oo = GATHER ? (uint64_t)slot[tile_begin + w] * NPT + (uint64_t)(C.ti + TX * (C.tj + TY * C.tk)) : C.o;
     C.on_face = false;
     C.eta = P.eta_const;
     if (C.act) {
@@ -1104,6 +1109,7 @@ void free_plan(dgr_plan_s* plan) {
   (void)hipFree(plan->d_blocks);
   (void)hipFree(plan->d_tiles);
   (void)hipFree(plan->d_tiles_blk);
+  (void)hipFree(plan->d_tile_slot);
   (void)hipFree(plan->d_io);
   (void)hipFree(plan->d_nbr);
   (void)hipFree(plan->d_fill_items);
@@ -1360,6 +1366,13 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
   order_tiles(hs, 0, plan->ntiles_interior, sc.order, blocks);
   order_tiles(hs, plan->ntiles_interior, hs.size(), sc.order, blocks);
   plan->h_tiles_blk = ht;
+  // block-order index of every scheduled tile (compact k of the RK4 stages)
+  std::vector<uint32_t> hslot(hs.size());
+  {
+    std::map<std::tuple<uint32_t, uint16_t, uint16_t, uint16_t>, uint32_t> pos;
+    for (uint32_t q = 0; q < (uint32_t)ht.size(); ++q) pos[std::make_tuple(ht[q].block, ht[q].i0, ht[q].j0, ht[q].k0)] = q;
+    for (size_t q = 0; q < hs.size(); ++q) hslot[q] = pos[std::make_tuple(hs[q].block, hs[q].i0, hs[q].j0, hs[q].k0)];
+  }
   for (uint32_t b = 0; b < nblocks; ++b)
     plan->dof_end = std::max<uint64_t>(plan->dof_end, blocks[b].offset + (uint64_t)blocks[b].sz[0] * blocks[b].sz[1] *
                                                                            blocks[b].sz[2]);
@@ -1382,6 +1395,7 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
   if (hipMalloc(&plan->d_blocks, sizeof(DevBlock) * nblocks) != hipSuccess ||
       hipMalloc(&plan->d_tiles, sizeof(DevTile) * ht.size()) != hipSuccess ||
       hipMalloc(&plan->d_tiles_blk, sizeof(DevTile) * ht.size()) != hipSuccess ||
+      hipMalloc(&plan->d_tile_slot, sizeof(uint32_t) * std::max<size_t>(1, hslot.size())) != hipSuccess ||
       hipMalloc(&plan->d_io, sizeof(IOPtrs)) != hipSuccess ||
       hipMalloc(&plan->d_nbr, sizeof(int32_t) * plan->nbr.size()) != hipSuccess) {
     free_plan(plan);
@@ -1393,6 +1407,8 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
     e = hipMemcpy(plan->d_tiles_blk, ht.data(), sizeof(DevTile) * ht.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(plan->d_nbr, plan->nbr.data(), sizeof(int32_t) * plan->nbr.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !hslot.empty())
+    e = hipMemcpy(plan->d_tile_slot, hslot.data(), sizeof(uint32_t) * hslot.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     free_plan(plan);
     return hip_fail(e, "hipMemcpy(plan)");
@@ -1445,12 +1461,13 @@ int dgr_internal::launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st
   const auto kbd = gather ? (six ? k_bssn_rhs<true, 64, true> : k_bssn_rhs<true, 42, true>)
                           : (six ? k_bssn_rhs<true, 64, false> : k_bssn_rhs<true, 42, false>);
   const int32_t* nb = gather ? plan->d_nbr : nullptr;
+  const uint32_t* sl = gather ? plan->d_tile_slot : nullptr;
   if (ni > 0)
     hipLaunchKernelGGL(kin, dim3(grid(ni)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, 0, ni, d_io,
-                       plan->params, nb);
+                       plan->params, nb, sl);
   if (nbd > 0)
     hipLaunchKernelGGL(kbd, dim3(grid(nbd)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, ni, nbd, d_io,
-                       plan->params, nb);
+                       plan->params, nb, sl);
   DGR_HIP(hipGetLastError());
   return DGR_OK;
 }
@@ -1477,7 +1494,7 @@ extern "C" int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const
                        ? (six ? k_bssn_rhs<false, 64, false> : k_bssn_rhs<false, 42, false>)
                        : (six ? k_bssn_rhs<true, 64, false> : k_bssn_rhs<true, 42, false>);
     hipLaunchKernelGGL(k, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, tb, nt, plan->d_io,
-                       plan->params, (const int32_t*)nullptr);
+                       plan->params, (const int32_t*)nullptr, (const uint32_t*)nullptr);
   }
   DGR_HIP(hipGetLastError());
   return DGR_OK;
@@ -1513,10 +1530,10 @@ extern "C" int dgr_bssn_rhs_blkwise(dgr_plan_t plan, double* const* rhs, const d
   auto grid = [&](int m) { return std::min(plan->grid * 2, m); };
   if (nin > 0)
     hipLaunchKernelGGL(kin, dim3(grid((int)nin)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_sub, 0, (int)nin,
-                       plan->d_io, plan->params, (const int32_t*)nullptr);
+                       plan->d_io, plan->params, (const int32_t*)nullptr, (const uint32_t*)nullptr);
   if (nbd > 0)
     hipLaunchKernelGGL(kbd, dim3(grid(nbd)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_sub, (int)nin, nbd,
-                       plan->d_io, plan->params, (const int32_t*)nullptr);
+                       plan->d_io, plan->params, (const int32_t*)nullptr, (const uint32_t*)nullptr);
   DGR_HIP(hipGetLastError());
   return DGR_OK;
 }
@@ -1547,7 +1564,7 @@ extern "C" int dgr_bssn_rhs_blk(dgr_plan_t plan, double* out, const double* in, 
   const auto k = tb < plan->ntiles_interior ? (six ? k_bssn_rhs<false, 64, false> : k_bssn_rhs<false, 42, false>)
                                             : (six ? k_bssn_rhs<true, 64, false> : k_bssn_rhs<true, 42, false>);
   hipLaunchKernelGGL(k, dim3(tc), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, (int)tb, (int)tc,
-                     plan->d_io_blk, plan->params, (const int32_t*)nullptr);
+                     plan->d_io_blk, plan->params, (const int32_t*)nullptr, (const uint32_t*)nullptr);
   DGR_HIP(hipGetLastError());
   return DGR_OK;
 }

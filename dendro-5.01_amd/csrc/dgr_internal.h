@@ -100,6 +100,7 @@ struct dgr_plan_s {
   dgr_internal::DevBlock* d_blocks = nullptr;
   dgr_internal::DevTile* d_tiles = nullptr;      // scheduled order (batched launches)
   dgr_internal::DevTile* d_tiles_blk = nullptr;  // block order (per-block launches)
+  uint32_t* d_tile_slot = nullptr;        // block-order index of each scheduled tile
   uint32_t ntiles_interior = 0;           // tiles [0, n) belong to blocks with bflag 0
   std::vector<uint32_t> block_tile_begin;  // first tile of block b in d_tiles_blk
   std::vector<uint32_t> block_tile_count;
@@ -135,7 +136,10 @@ struct dgr_plan_s {
 namespace dgr_internal {
 // Launch the RHS kernels of `plan` with a device pointer table.  With
 // gather, ghost points that have a same-level neighbour are read from that
-// neighbour's interior (the RK4 stages), so no refill is needed before.
+// neighbour's interior (the RK4 stages), so no refill is needed before, and
+// the right-hand side is written in the compact tile-major layout of the
+// stage kernels: variable v of interior point (a, b, c) of block-order tile
+// w at rhs[v][w * 343 + a + 7 (b + 7 c)].
 int launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st, bool gather);
 
 // Find the level-jump ghost fills of a plan's blocks and upload them

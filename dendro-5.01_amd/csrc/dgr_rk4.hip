@@ -53,14 +53,16 @@ struct Ptrs24 {
   double* v[24];
 };
 
-// RK4 work memory: one allocation holding the stage state (stg) and the RHS
-// (k) as 24 padded arrays of dof doubles each, then the running final sum in
-// a compact layout; all addressed arithmetically so that only the caller's
-// state needs a pointer table.
+// RK4 work memory: one allocation holding the stage state (stg) as 24
+// padded arrays of dof doubles each, then the RHS (k) and the running final
+// sum in a compact tile-major layout (343 slots per tile in block order, the
+// layout the RHS kernels write in their gather mode); all addressed
+// arithmetically so that only the caller's state needs a pointer table.
 struct Work {
-  double* base;   // stage state (24 x dof, padded layout) then k (24 x dof)
+  double* base;   // stage state (24 x dof, padded layout)
   uint64_t dof;
-  double* acc;    // running sum, compact: 343 slots per tile, 24 x nacc
+  double* kc;     // k, compact: 24 x nacc
+  double* acc;    // running sum, compact: 24 x nacc
   uint64_t nacc;
 };
 
@@ -71,10 +73,8 @@ struct Work {
 typedef __attribute__((address_space(1))) double gdouble;  // global memory
 struct WorkCursor {
   double* q;        // stage state of variable v at padded point o
-  uint64_t s24;     // 24 * dof
-  __device__ __forceinline__ WorkCursor(const Work& W, uint64_t o) : q(W.base + o), s24(24 * W.dof) {}
+  __device__ __forceinline__ WorkCursor(const Work& W, uint64_t o) : q(W.base + o) {}
   __device__ __forceinline__ gdouble& stg() { return ((gdouble*)q)[0]; }
-  __device__ __forceinline__ gdouble& k() { return ((gdouble*)q)[s24]; }
   __device__ __forceinline__ void next(uint64_t dof) {
     q += dof;
     asm volatile("" : "+v"(q));
@@ -85,7 +85,7 @@ struct WorkCursor {
 // writes instead of the 7-point row fragments of the padded layout.
 struct AccCursor {
   double* q;
-  __device__ __forceinline__ AccCursor(const Work& W, uint64_t co) : q(W.acc + co) {}
+  __device__ __forceinline__ AccCursor(double* base, uint64_t co) : q(base + co) {}
   __device__ __forceinline__ gdouble& acc() { return ((gdouble*)q)[0]; }
   __device__ __forceinline__ void next(uint64_t n) {
     q += n;
@@ -168,14 +168,14 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
     if (KIND != ST_LAST) {
       const double cu = c_rkU[stage] * dt, cc = c_rkC[stage] * dt;
       double acc[24];
-      WorkCursor c(W, o);
-      AccCursor a(W, co);
+      AccCursor c(W.kc, co);
+      AccCursor a(W.acc, co);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
         un[v] = state.v[v][o];  // u_prev: the state at the start of the step
-        kk[v] = c.k();
+        kk[v] = c.acc();
         acc[v] = KIND == ST_FIRST ? un[v] : a.acc();
-        c.next(W.dof);
+        c.next(W.nacc);
         a.next(W.nacc);
       }
 #pragma unroll
@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
         acc[v] = acc[v] + cc * kk[v];
         un[v] = un[v] + cu * kk[v];
       }
-      AccCursor d(W, co);
+      AccCursor d(W.acc, co);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
         d.acc() = acc[v];
@@ -192,13 +192,13 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
       }
     } else {
       const double cc = c_rkC[3] * dt;
-      WorkCursor c(W, o);
-      AccCursor a(W, co);
+      AccCursor c(W.kc, co);
+      AccCursor a(W.acc, co);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
         un[v] = a.acc();
-        kk[v] = c.k();
-        c.next(W.dof);
+        kk[v] = c.acc();
+        c.next(W.nacc);
         a.next(W.nacc);
       }
 #pragma unroll
@@ -284,8 +284,8 @@ int grid_for(dgr_plan_t plan, int n) {
 struct dgr_rk4_s {
   dgr_plan_t plan = nullptr;
   uint64_t dof = 0;
-  double* buf = nullptr;        // stage state, k (24 x dof each), running sum (24 x 343 per tile)
-  Ptrs24 state{}, stg{}, k{};  // the state and the padded work arrays in buf
+  double* buf = nullptr;        // stage state (24 x dof), k and running sum (24 x 343 per tile each)
+  Ptrs24 state{}, stg{}, k{};  // the state and the work arrays in buf
   uint64_t nacc = 0;           // compact running-sum slots per variable (343 per tile)
   IOPtrs* d_io = nullptr;       // RHS pointer tables: [0] (u -> k), [1] (stage -> k)
   double* d_gdt = nullptr;      // time step per block group
@@ -312,7 +312,7 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
     // neighbours' interiors (gather), so no refill runs between stages
     const int grid = std::min(nt, std::max(1, p->grid * 2));
     const int chunk = (nt + grid - 1) / grid;
-    const Work W = {r->buf, r->dof, r->buf + 48 * r->dof, r->nacc};
+    const Work W = {r->buf, r->dof, r->buf + 24 * r->dof, r->buf + 24 * r->dof + 24 * r->nacc, r->nacc};
     const dim3 g((nt + chunk - 1) / chunk), b(384);
     if (s == 0)
       hipLaunchKernelGGL(k_stage<ST_FIRST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, chunk, r->state, W,
@@ -411,9 +411,9 @@ extern "C" int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* 
   return flag ? DGR_E_CONSTRAINT : DGR_OK;
 }
 
-// stage state and k in the padded layout, the running sum compact
+// stage state in the padded layout, k and the running sum compact
 extern "C" uint64_t dgr_rk4_work_doubles(dgr_plan_t plan) {
-  return plan ? 48ull * plan->dof_end + 24ull * 343ull * plan->ntiles : 0;
+  return plan ? 24ull * plan->dof_end + 48ull * 343ull * plan->ntiles : 0;
 }
 
 extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double* const* u, double* work,
@@ -439,7 +439,7 @@ extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double*
   Ptrs24& k = r->k;
   for (int v = 0; v < 24; ++v) {
     stg.v[v] = r->buf + (size_t)v * r->dof;
-    k.v[v] = r->buf + (size_t)(24 + v) * r->dof;
+    k.v[v] = r->buf + (size_t)24 * r->dof + (size_t)v * r->nacc;
   }
   IOPtrs io[2];
   for (int v = 0; v < 24; ++v) {
@@ -572,20 +572,22 @@ struct EtsTerms {
 };
 
 // out = base + sum_q c[q] k_{idx[q]} on the interior points of every tile;
-// k_p variable v at kb + (p*24 + v) * dof.
+// k_p variable v in the compact layout of the RHS in gather mode, at
+// kb + (p*24 + v) * nacc + (tile * 343 + local index).
 __global__ void __launch_bounds__(384) k_ets_combine(const DevBlock* __restrict__ blocks,
                                                     const DevTile* __restrict__ tiles, int ntiles, Ptrs24 out,
-                                                    Ptrs24 base, const double* __restrict__ kb, uint64_t dof,
+                                                    Ptrs24 base, const double* __restrict__ kb, uint64_t nacc,
                                                     EtsTerms T) {
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const DevTile tl = tiles[w];
     const DevBlock& B = blocks[tl.block];
     uint64_t o;
     if (!tile_point(tl, B, threadIdx.x, o)) continue;
+    const uint64_t co = (uint64_t)w * 343 + threadIdx.x;
     for (int v = 0; v < 24; ++v) {
 #pragma clang fp contract(off)
       double x = base.v[v][o];
-      for (int q = 0; q < T.n; ++q) x = x + T.c[q] * kb[((uint64_t)T.idx[q] * 24 + v) * dof + o];
+      for (int q = 0; q < T.n; ++q) x = x + T.c[q] * kb[((uint64_t)T.idx[q] * 24 + v) * nacc + co];
       out.v[v][o] = x;
     }
   }
@@ -599,7 +601,8 @@ struct dgr_ets_s {
   int ns = 0;
   std::vector<double> a, b, c;  // a[s*ns + p], b[k], c[s]
   double time = 0.0;
-  double* buf = nullptr;  // tmp (24 x dof) then k_0 .. k_{ns-1} (24 x dof each)
+  uint64_t nacc = 0;      // compact k slots per variable (343 per tile)
+  double* buf = nullptr;  // tmp (24 x dof) then k_0 .. k_{ns-1} (24 x nacc each, compact)
   Ptrs24 state{}, tmp{};
   IOPtrs* d_io = nullptr;  // RHS tables: stage s reads tmp, writes k_s
   unsigned* d_flag = nullptr;
@@ -631,7 +634,9 @@ extern "C" int dgr_ets_create_tableau(dgr_ets_t* out, dgr_plan_t plan, double* c
   e->c.assign(ci, ci + stages);
   for (int v = 0; v < 24; ++v) e->state.v[v] = u[v];
   const size_t nb = sizeof(double) * e->dof;
-  if (hipMalloc(&e->buf, nb * 24 * (1 + stages)) != hipSuccess ||
+  e->nacc = 343ull * plan->ntiles;
+  const size_t nbuf = sizeof(double) * 24 * (e->dof + (size_t)stages * e->nacc);
+  if (hipMalloc(&e->buf, nbuf) != hipSuccess ||
       hipMalloc(&e->d_io, sizeof(IOPtrs) * stages) != hipSuccess || hipMalloc(&e->d_flag, 256) != hipSuccess) {
     ets_destroy(e);
     return DGR_E_NOMEM;
@@ -641,11 +646,11 @@ extern "C" int dgr_ets_create_tableau(dgr_ets_t* out, dgr_plan_t plan, double* c
   for (uint32_t s = 0; s < stages; ++s)
     for (int v = 0; v < 24; ++v) {
       io[s].u.v[v] = e->tmp.v[v];
-      io[s].r.v[v] = e->buf + ((size_t)(1 + s) * 24 + v) * e->dof;
+      io[s].r.v[v] = e->buf + (size_t)24 * e->dof + ((size_t)s * 24 + v) * e->nacc;
     }
   hipError_t er = hipMemcpy(e->d_io, io.data(), sizeof(IOPtrs) * stages, hipMemcpyHostToDevice);
   if (er == hipSuccess) er = hipMemset(e->d_flag, 0, 256);
-  if (er == hipSuccess) er = hipMemset(e->buf, 0, nb * 24 * (1 + stages));
+  if (er == hipSuccess) er = hipMemset(e->buf, 0, nbuf);
   // ghosts no fill reaches keep the state's values in the stage input
   for (int v = 0; v < 24 && er == hipSuccess; ++v)
     er = hipMemcpy(e->tmp.v[v], e->state.v[v], nb, hipMemcpyDeviceToDevice);
@@ -693,7 +698,7 @@ extern "C" int dgr_ets_evolve(dgr_ets_t e, double dt, void* stream) {
       T.idx[T.n++] = q;
     }
     hipLaunchKernelGGL(k_ets_combine, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->tmp, e->state,
-                       kb, e->dof, T);
+                       kb, e->nacc, T);
     hipLaunchKernelGGL(k_enforce, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->tmp, p->chi_floor,
                        e->d_flag);
     int rc = launch_level_fill(p, tmp_o, st);
@@ -708,7 +713,7 @@ extern "C" int dgr_ets_evolve(dgr_ets_t e, double dt, void* stream) {
   }
   T.n = e->ns;
   hipLaunchKernelGGL(k_ets_combine, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->state, e->state,
-                     kb, e->dof, T);
+                     kb, e->nacc, T);
   hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(p, (int)p->nblocks)), dim3(RK_NT), 0, st, p->d_blocks, p->d_nbr,
                      (int)p->nblocks, e->state);
   DGR_HIP(hipGetLastError());
