@@ -182,50 +182,53 @@ struct TileGeom {
   int nreg;
 };
 
-// Issue the global loads of one variable's staged region into registers.
-// The region is walked in the fixed SX x SY x SZ layout of the LDS buffer
-// (compile-time divisors); elements outside a partial tile's region are
-// skipped and their LDS slots are never read.
-//
-// Gather mode (RK4 stages): a ghost point with a same-level neighbour is
-// read from the neighbour's interior instead of the block's own padding,
-// sd[direction] being the index distance to that source (0 for interior
-// points and for ghosts without a neighbour, which keep their own padding).
-// This is the same-level refill done on the fly, so the RK4 step needs no
-// refill kernel between stages.
+// Source offset (from the block's first padded point) of every staging slot
+// of this thread and a bit per slot that lies inside the tile's region.
+// Computed once per tile; every variable's prefetch then costs one address
+// add per slot.  Gather mode (RK4 stages): a ghost slot with a same-level
+// neighbour reads the neighbour's interior, sd[direction] being the index
+// distance to that source (0 for interior slots and for ghosts without a
+// neighbour, which keep their own padding): the same-level refill done on
+// the fly, so the RK4 step needs no refill kernel between stages.
+struct Slots {
+  int32_t off[NPF];
+  uint32_t valid;
+};
+
 template <bool GATHER>
-__device__ __forceinline__ void prefetch(double (&r)[NPF], const double* __restrict__ g, const TileGeom& T, int t,
-                                         const long long* sd, uint32_t gcode) {
-  const double* base = g + T.off + (T.i0 - 3) + (long)T.sz0 * ((T.j0 - 3) + (long)T.sz1 * (T.k0 - 3));
+__device__ __forceinline__ Slots make_slots(const TileGeom& T, int t, const long long* sd) {
+  Slots S;
+  S.valid = 0;
 #pragma unroll
   for (int m = 0; m < NPF; ++m) {
     const int e = t + m * NT;
     const int a = e % SX;
     const int b = (e / SX) % SY;
     const int c = e / (SX * SY);
-    if (e < NSU && a < T.rx && b < T.ry && c < T.rz) {
-      if constexpr (GATHER)
-        r[m] = __builtin_nontemporal_load(gmem(base) + a + (long)T.sz0 * (b + (long)T.sz1 * c) +
-                                          sd[(gcode >> (5 * m)) & 31u]);
-      else
-        r[m] = __builtin_nontemporal_load(gmem(base) + a + (long)T.sz0 * (b + (long)T.sz1 * c));
+    const int i = T.i0 - 3 + a, j = T.j0 - 3 + b, k = T.k0 - 3 + c;
+    long long o = (long long)i + (long long)T.sz0 * (j + (long long)T.sz1 * k);
+    if constexpr (GATHER) {
+      const int dx = i < 3 ? 0 : (i >= T.sz0 - 3 ? 2 : 1);
+      const int dy = j < 3 ? 0 : (j >= T.sz1 - 3 ? 2 : 1);
+      const int dz = k < 3 ? 0 : (k >= T.sz2 - 3 ? 2 : 1);
+      o += sd[dx + 3 * dy + 9 * dz];
     }
+    S.off[m] = (int32_t)o;
+    if (e < NSU && a < T.rx && b < T.ry && c < T.rz) S.valid |= 1u << m;
   }
+  return S;
 }
 
-// Ghost direction of every prefetch slot of this thread (gather mode).
-__device__ __forceinline__ uint32_t gather_codes(const TileGeom& T, int t) {
-  uint32_t code = 0;
+// Issue the global loads of one variable's staged region into registers.
+// The region is walked in the fixed SX x SY x SZ layout of the LDS buffer;
+// slots outside a partial tile's region are skipped and their LDS slots are
+// never read.
+__device__ __forceinline__ void prefetch(double (&r)[NPF], const double* __restrict__ g, const TileGeom& T,
+                                         const Slots& S) {
+  const double* base = g + T.off;
 #pragma unroll
-  for (int m = 0; m < NPF; ++m) {
-    const int e = t + m * NT;
-    const int i = T.i0 - 3 + e % SX, j = T.j0 - 3 + (e / SX) % SY, k = T.k0 - 3 + e / (SX * SY);
-    const int dx = i < 3 ? 0 : (i >= T.sz0 - 3 ? 2 : 1);
-    const int dy = j < 3 ? 0 : (j >= T.sz1 - 3 ? 2 : 1);
-    const int dz = k < 3 ? 0 : (k >= T.sz2 - 3 ? 2 : 1);
-    code |= (uint32_t)(dx + 3 * dy + 9 * dz) << (5 * m);
-  }
-  return code;
+  for (int m = 0; m < NPF; ++m)
+    if ((S.valid >> m) & 1u) r[m] = __builtin_nontemporal_load(gmem(base) + S.off[m]);
 }
 
 // zmask (boundary tiles): bit m set when slot m lies beyond a flagged face.
@@ -538,7 +541,7 @@ __device__ __forceinline__ void plane_y(const double* su, double* sgy, const Til
 template <int I, bool BND, int ORD, bool GATHER>
 __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], double* su, double* sgx,
                                          double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io,
-                                         int t, const long long* sd, uint32_t gcode, uint32_t zmask) {
+                                         int t, const Slots& sl, uint32_t zmask) {
   if constexpr (I < 24) {
     constexpr int V = h_order[I];
     using G = Planes<ORD>;
@@ -549,7 +552,7 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
       // d_x plane -> d_x u, d_y d_x u, d_z d_x u; then the d_y plane in the
       // same buffer -> d_y u, d_z d_y u
       plane_x<BND, ORD>(su, sgx, T, C.ax, t);
-      if constexpr (I + 1 < 24) prefetch<GATHER>(pf, io->u.v[h_order[I + 1]], T, t, sd, gcode);
+      if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
       __syncthreads();
       if (C.act) {
         const int ti = C.ti, tj = C.tj, tk = C.tk;
@@ -570,7 +573,7 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
         plane_x<BND, ORD>(su, sgx, T, C.ax, t);
         plane_y<BND, ORD>(su, sgy, T, C.ax, t);
       }
-      if constexpr (I + 1 < 24) prefetch<GATHER>(pf, io->u.v[h_order[I + 1]], T, t, sd, gcode);
+      if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
       if constexpr (has_d2(V)) __syncthreads();
     }
     if constexpr (V == dgr::V_CHI) {
@@ -578,7 +581,7 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
     }
     process_var<V, BND, ORD>(s, C, su, sgx, sgy, M, T, io);
     __syncthreads();
-    run_vars<I + 1, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, io, t, sd, gcode, zmask);
+    run_vars<I + 1, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, io, t, sl, zmask);
   }
 }
 
@@ -661,9 +664,9 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
       }
       __syncthreads();
     }
-    const uint32_t gcode = GATHER ? gather_codes(T, t) : 0u;
+    const Slots sl = make_slots<GATHER>(T, t, s_delta);
     double pf[NPF];
-    prefetch<GATHER>(pf, io->u.v[h_order[0]], T, t, s_delta, gcode);
+    prefetch(pf, io->u.v[h_order[0]], T, sl);
     __syncthreads();
 
     // point coordinates, eta (rhs.cpp:214-223) and the Sommerfeld face test
@@ -704,7 +707,7 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
       s.begin();
     }
     const uint32_t zmask = BND ? beyond_face_mask(T, B.bflag, t) : 0u;
-    run_vars<0, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, s_delta, gcode, zmask);
+    run_vars<0, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, zmask);
   }
 }
 
@@ -830,7 +833,7 @@ __device__ __forceinline__ void process_con(KPoint& s, ConAcc& A, const Ctx& C, 
 template <int I, bool BND>
 __device__ __forceinline__ void con_vars(KPoint& s, ConAcc& A, Ctx& C, double (&pf)[NPF], double* su, double* sgx,
                                          double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io, int t,
-                                         uint32_t zmask) {
+                                         const Slots& sl, uint32_t zmask) {
   if constexpr (I < 17) {
     constexpr int V = con_order[I];
     stage(pf, su, T, t, zmask);
@@ -839,14 +842,14 @@ __device__ __forceinline__ void con_vars(KPoint& s, ConAcc& A, Ctx& C, double (&
       plane_x<BND, 42>(su, sgx, T, C.ax, t);
       plane_y<BND, 42>(su, sgy, T, C.ax, t);
     }
-    if constexpr (I + 1 < 17) prefetch<false>(pf, io->u.v[con_order[I + 1]], T, t, nullptr, 0u);
+    if constexpr (I + 1 < 17) prefetch(pf, io->u.v[con_order[I + 1]], T, sl);
     if constexpr (con_d2(V)) __syncthreads();
     if constexpr (V == dgr::V_CHI) {
       if (C.act) s.finish_metric();
     }
     process_con<V, BND>(s, A, C, su, sgx, sgy, T);
     __syncthreads();
-    con_vars<I + 1, BND>(s, A, C, pf, su, sgx, sgy, T, io, t, zmask);
+    con_vars<I + 1, BND>(s, A, C, pf, su, sgx, sgy, T, io, t, sl, zmask);
   }
 }
 
@@ -980,8 +983,9 @@ __global__ void __launch_bounds__(NT) k_physcon(const DevBlock* __restrict__ blo
     else if (t < 39) s_geo[t - 33] = B.h[t - 36];
     C.ax = s_ax;
     C.geo = s_geo;
+    const Slots sl = make_slots<false>(T, t, nullptr);
     double pf[NPF];
-    prefetch<false>(pf, io->u.v[con_order[0]], T, t, nullptr, 0u);
+    prefetch(pf, io->u.v[con_order[0]], T, sl);
     __syncthreads();
 
     KPoint s;
@@ -1013,7 +1017,7 @@ __global__ void __launch_bounds__(NT) k_physcon(const DevBlock* __restrict__ blo
       A.p3r = A.p3i = 0.0;
     }
     const uint32_t zmask = BND ? beyond_face_mask(T, B.bflag, t) : 0u;
-    con_vars<0, BND>(s, A, C, pf, su, sgx, sgy, T, &s_io, t, zmask);
+    con_vars<0, BND>(s, A, C, pf, su, sgx, sgy, T, &s_io, t, sl, zmask);
     if (C.act) {
       double out[6];
       con_finish(s, A, out);
