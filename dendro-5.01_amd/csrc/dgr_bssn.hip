@@ -507,41 +507,94 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
 
 // Widened first-derivative planes for the mixed derivatives, walked in
 // their fixed LDS shapes (compile-time divisors) and masked to the tile.
+// Widened first-derivative planes for the mixed derivatives.  Thread t
+// computes plane entries t, t + NT, ... of each plane; in interior tiles
+// their staging offsets and axis coordinates, which depend only on the
+// tile, are computed once per tile (PlaneSlots), so each variable's planes
+// cost the stencils alone.
 // sgx[a + TX*(b + GXY*cc)]: d_x u at block coords (i0+a, j0-GW+b, k0-GW+cc),
 // su coords (a+3, b+3-GW, cc+3-GW).
-template <bool BND, int ORD>
-__device__ __forceinline__ void plane_x(const double* su, double* sgx, const TileGeom& T, const AxisC* ax, int t) {
-  using G = Planes<ORD>;
-  for (int e = t; e < G::NGX; e += NT) {
-    const int a = e % TX;
-    const int b = (e / TX) % G::GXY;
-    const int cc = e / (TX * G::GXY);
-    if (a < T.nx && b < T.ny + 2 * G::GW && cc < T.nz + 2 * G::GW) {
-      const int o = (a + 3) + SX * ((b + 3 - G::GW) + SY * (cc + 3 - G::GW)) - (T.i0 + a);
-      sgx[e] = d1_op<BND, ORD>(su, o, 1, T.i0 + a, ax[0]);
-    }
-  }
-}
 // sgy[a + TX*(b + TY*cc)]: d_y u at (i0+a, j0+b, k0-GW+cc), su coords
 // (a+3, b+3, cc+3-GW).
-template <bool BND, int ORD>
-__device__ __forceinline__ void plane_y(const double* su, double* sgy, const TileGeom& T, const AxisC* ax, int t) {
+template <int ORD>
+struct PlaneSlots {
+  static constexpr int NPL =
+      ((Planes<ORD>::NGX > Planes<ORD>::NGY ? Planes<ORD>::NGX : Planes<ORD>::NGY) + NT - 1) / NT;
+  int ox[NPL], qx[NPL], oy[NPL], qy[NPL];
+  uint32_t valid;  // bit k: x entry k, bit NPL + k: y entry k
+};
+
+template <int ORD>
+__device__ __forceinline__ PlaneSlots<ORD> make_plane_slots(const TileGeom& T, int t) {
   using G = Planes<ORD>;
-  for (int e = t; e < G::NGY; e += NT) {
-    const int a = e % TX;
-    const int b = (e / TX) % TY;
-    const int cc = e / (TX * TY);
-    if (a < T.nx && b < T.ny && cc < T.nz + 2 * G::GW) {
-      const int o = (a + 3) + SX * ((b + 3) + SY * (cc + 3 - G::GW)) - (T.j0 + b) * SX;
-      sgy[e] = d1_op<BND, ORD>(su, o, SX, T.j0 + b, ax[1]);
+  constexpr int NPL = PlaneSlots<ORD>::NPL;
+  PlaneSlots<ORD> P;
+  P.valid = 0;
+#pragma unroll
+  for (int k = 0; k < NPL; ++k) {
+    int e = t + k * NT;
+    int a = e % TX, b = (e / TX) % G::GXY, cc = e / (TX * G::GXY);
+    P.ox[k] = (a + 3) + SX * ((b + 3 - G::GW) + SY * (cc + 3 - G::GW)) - (T.i0 + a);
+    P.qx[k] = T.i0 + a;
+    if (e < G::NGX && a < T.nx && b < T.ny + 2 * G::GW && cc < T.nz + 2 * G::GW) P.valid |= 1u << k;
+    b = (e / TX) % TY;
+    cc = e / (TX * TY);
+    P.oy[k] = (a + 3) + SX * ((b + 3) + SY * (cc + 3 - G::GW)) - (T.j0 + b) * SX;
+    P.qy[k] = T.j0 + b;
+    if (e < G::NGY && a < T.nx && b < T.ny && cc < T.nz + 2 * G::GW) P.valid |= 1u << (NPL + k);
+  }
+  return P;
+}
+
+// Boundary tiles (whose closure code leaves fewer registers) recompute the
+// entry coordinates in the loop instead of holding them.
+template <bool BND, int ORD>
+__device__ __forceinline__ void plane_x(const double* su, double* sgx, const PlaneSlots<ORD>& P, const TileGeom& T,
+                                        const AxisC* ax, int t) {
+  using G = Planes<ORD>;
+  if constexpr (BND) {
+    for (int e = t; e < G::NGX; e += NT) {
+      const int a = e % TX;
+      const int b = (e / TX) % G::GXY;
+      const int cc = e / (TX * G::GXY);
+      if (a < T.nx && b < T.ny + 2 * G::GW && cc < T.nz + 2 * G::GW) {
+        const int o = (a + 3) + SX * ((b + 3 - G::GW) + SY * (cc + 3 - G::GW)) - (T.i0 + a);
+        sgx[e] = d1_op<BND, ORD>(su, o, 1, T.i0 + a, ax[0]);
+      }
     }
+  } else {
+    constexpr int NPL = PlaneSlots<ORD>::NPL;
+#pragma unroll
+    for (int k = 0; k < NPL; ++k)
+      if ((P.valid >> k) & 1u) sgx[t + k * NT] = d1_op<BND, ORD>(su, P.ox[k], 1, P.qx[k], ax[0]);
+  }
+}
+template <bool BND, int ORD>
+__device__ __forceinline__ void plane_y(const double* su, double* sgy, const PlaneSlots<ORD>& P, const TileGeom& T,
+                                        const AxisC* ax, int t) {
+  using G = Planes<ORD>;
+  if constexpr (BND) {
+    for (int e = t; e < G::NGY; e += NT) {
+      const int a = e % TX;
+      const int b = (e / TX) % TY;
+      const int cc = e / (TX * TY);
+      if (a < T.nx && b < T.ny && cc < T.nz + 2 * G::GW) {
+        const int o = (a + 3) + SX * ((b + 3) + SY * (cc + 3 - G::GW)) - (T.j0 + b) * SX;
+        sgy[e] = d1_op<BND, ORD>(su, o, SX, T.j0 + b, ax[1]);
+      }
+    }
+  } else {
+    constexpr int NPL = PlaneSlots<ORD>::NPL;
+#pragma unroll
+    for (int k = 0; k < NPL; ++k)
+      if ((P.valid >> (NPL + k)) & 1u) sgy[t + k * NT] = d1_op<BND, ORD>(su, P.oy[k], SX, P.qy[k], ax[1]);
   }
 }
 
 template <int I, bool BND, int ORD, bool GATHER>
 __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], double* su, double* sgx,
                                          double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io,
-                                         int t, const Slots& sl, uint32_t zmask) {
+                                         int t, const Slots& sl, const PlaneSlots<ORD>& ps, uint32_t zmask) {
   if constexpr (I < 24) {
     constexpr int V = h_order[I];
     using G = Planes<ORD>;
@@ -551,7 +604,7 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
     if constexpr (has_d2(V) && G::SHARED) {
       // d_x plane -> d_x u, d_y d_x u, d_z d_x u; then the d_y plane in the
       // same buffer -> d_y u, d_z d_y u
-      plane_x<BND, ORD>(su, sgx, T, C.ax, t);
+      plane_x<BND, ORD>(su, sgx, ps, T, C.ax, t);
       if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
       __syncthreads();
       if (C.act) {
@@ -561,7 +614,7 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
         M.dxz = d1_op<BND, ORD>(sgx, ti + TX * ((tj + G::GW) + G::GXY * (G::GW - T.k0)), TX * G::GXY, C.gk, C.ax[2]);
       }
       __syncthreads();
-      plane_y<BND, ORD>(su, sgx, T, C.ax, t);
+      plane_y<BND, ORD>(su, sgx, ps, T, C.ax, t);
       __syncthreads();
       if (C.act) {
         const int ti = C.ti, tj = C.tj, tk = C.tk;
@@ -570,8 +623,8 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
       }
     } else {
       if constexpr (has_d2(V)) {
-        plane_x<BND, ORD>(su, sgx, T, C.ax, t);
-        plane_y<BND, ORD>(su, sgy, T, C.ax, t);
+        plane_x<BND, ORD>(su, sgx, ps, T, C.ax, t);
+        plane_y<BND, ORD>(su, sgy, ps, T, C.ax, t);
       }
       if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
       if constexpr (has_d2(V)) __syncthreads();
@@ -581,7 +634,7 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
     }
     process_var<V, BND, ORD>(s, C, su, sgx, sgy, M, T, io);
     __syncthreads();
-    run_vars<I + 1, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, io, t, sl, zmask);
+    run_vars<I + 1, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, io, t, sl, ps, zmask);
   }
 }
 
@@ -707,7 +760,9 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
       s.begin();
     }
     const uint32_t zmask = BND ? beyond_face_mask(T, B.bflag, t) : 0u;
-    run_vars<0, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, zmask);
+    PlaneSlots<ORD> ps;
+    if constexpr (!BND) ps = make_plane_slots<ORD>(T, t);
+    run_vars<0, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, ps, zmask);
   }
 }
 
@@ -833,14 +888,14 @@ __device__ __forceinline__ void process_con(KPoint& s, ConAcc& A, const Ctx& C, 
 template <int I, bool BND>
 __device__ __forceinline__ void con_vars(KPoint& s, ConAcc& A, Ctx& C, double (&pf)[NPF], double* su, double* sgx,
                                          double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io, int t,
-                                         const Slots& sl, uint32_t zmask) {
+                                         const Slots& sl, const PlaneSlots<42>& ps, uint32_t zmask) {
   if constexpr (I < 17) {
     constexpr int V = con_order[I];
     stage(pf, su, T, t, zmask);
     __syncthreads();
     if constexpr (con_d2(V)) {
-      plane_x<BND, 42>(su, sgx, T, C.ax, t);
-      plane_y<BND, 42>(su, sgy, T, C.ax, t);
+      plane_x<BND, 42>(su, sgx, ps, T, C.ax, t);
+      plane_y<BND, 42>(su, sgy, ps, T, C.ax, t);
     }
     if constexpr (I + 1 < 17) prefetch(pf, io->u.v[con_order[I + 1]], T, sl);
     if constexpr (con_d2(V)) __syncthreads();
@@ -849,7 +904,7 @@ __device__ __forceinline__ void con_vars(KPoint& s, ConAcc& A, Ctx& C, double (&
     }
     process_con<V, BND>(s, A, C, su, sgx, sgy, T);
     __syncthreads();
-    con_vars<I + 1, BND>(s, A, C, pf, su, sgx, sgy, T, io, t, sl, zmask);
+    con_vars<I + 1, BND>(s, A, C, pf, su, sgx, sgy, T, io, t, sl, ps, zmask);
   }
 }
 
@@ -1017,7 +1072,9 @@ __global__ void __launch_bounds__(NT) k_physcon(const DevBlock* __restrict__ blo
       A.p3r = A.p3i = 0.0;
     }
     const uint32_t zmask = BND ? beyond_face_mask(T, B.bflag, t) : 0u;
-    con_vars<0, BND>(s, A, C, pf, su, sgx, sgy, T, &s_io, t, sl, zmask);
+    PlaneSlots<42> ps;
+    if constexpr (!BND) ps = make_plane_slots<42>(T, t);
+    con_vars<0, BND>(s, A, C, pf, su, sgx, sgy, T, &s_io, t, sl, ps, zmask);
     if (C.act) {
       double out[6];
       con_finish(s, A, out);
