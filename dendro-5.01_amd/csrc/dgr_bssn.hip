@@ -1137,17 +1137,26 @@ __global__ void __launch_bounds__(NORM_T) k_con_norm_partial(const DevBlock* __r
   }
 }
 
-// Sum of the per-tile partials in tile order; out[13] = L2[6], max[6], count.
-__global__ void __launch_bounds__(64) k_con_norm_final(const double* __restrict__ part, int ntiles,
-                                                       double* __restrict__ out) {
-  const int q = threadIdx.x;
-  if (q >= 13) return;
+// Sum of the per-tile partials; out[13] = L2[6], max[6], count.  One
+// workgroup per output value: thread t takes tiles t, t + NORM_T, ... in
+// order, then a fixed tree combines the threads (deterministic).
+__global__ void __launch_bounds__(NORM_T) k_con_norm_final(const double* __restrict__ part, int ntiles,
+                                                           double* __restrict__ out) {
+  __shared__ double red[NORM_T];
+  const int q = blockIdx.x, t = threadIdx.x;
+  const bool mx = q >= 6 && q < 12;
   double a = 0.0;
-  for (int w = 0; w < ntiles; ++w) {
+  for (int w = t; w < ntiles; w += NORM_T) {
     const double v = part[(size_t)w * 13 + q];
-    a = (q >= 6 && q < 12) ? fmax(a, v) : a + v;
+    a = mx ? fmax(a, v) : a + v;
   }
-  out[q] = a;
+  red[t] = a;
+  __syncthreads();
+  for (int h = NORM_T / 2; h > 0; h >>= 1) {
+    if (t < h) red[t] = mx ? fmax(red[t], red[t + h]) : red[t] + red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) out[q] = red[0];
 }
 
 }  // namespace
@@ -1679,7 +1688,7 @@ extern "C" int dgr_physcon_norms(dgr_plan_t plan, const double* const* con, cons
   double* res = plan->d_norm + (size_t)13 * nt;
   hipLaunchKernelGGL(k_con_norm_partial, dim3(std::min(nt, 8192)), dim3(NORM_T), 0, st, plan->d_blocks,
                      plan->d_tiles_blk, nt, plan->d_nbr, c6, mask, threshold, plan->d_norm);
-  hipLaunchKernelGGL(k_con_norm_final, dim3(1), dim3(64), 0, st, plan->d_norm, nt, res);
+  hipLaunchKernelGGL(k_con_norm_final, dim3(13), dim3(NORM_T), 0, st, plan->d_norm, nt, res);
   DGR_HIP(hipGetLastError());
   double h[13];
   DGR_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
