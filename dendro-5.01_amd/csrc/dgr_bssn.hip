@@ -402,7 +402,7 @@ struct Mixed {
   double d1x, d1y, dxy, dxz, dyz;
 };
 
-template <int V, bool BND, int ORD>
+template <int V, int FM, int ORD>
 __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su, const double* sgx,
                                             const double* sgy, const Mixed& M, const TileGeom& T,
                                             const IOPtrs* __restrict__ io) {
@@ -415,17 +415,20 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
   const int c = (ti + 3) + SX * ((tj + 3) + SY * (tk + 3));
   const int ox = c - gi, oy = c - gj * SX, oz = c - gk * (SX * SY);
   constexpr bool D2 = has_d2(V);
+  // FM: flagged-face axes of the tile's block (bit 0 x, 1 y, 2 z); the
+  // closure rows are compiled in along those axes only
+  constexpr bool BND = FM != 0, CX = (FM & 1) != 0, CY = (FM & 2) != 0, CZ = (FM & 4) != 0;
   // first derivatives along x and y come from the LDS planes for D2 vars
   constexpr bool D1xy = !D2 && (has_d1(V) || BND);
   constexpr bool D1z = has_d1(V) || BND;
   double d1[3] = {0.0, 0.0, 0.0};
   double d2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double av[3], kv[3];
-  axis_ops<BND, D1xy, D2, ORD>(su, ox, 1, gi, ax[0], s.beta[0], d1[0], d2[0], av[0], kv[0]);
+  axis_ops<CX, D1xy, D2, ORD>(su, ox, 1, gi, ax[0], s.beta[0], d1[0], d2[0], av[0], kv[0]);
   __builtin_amdgcn_sched_barrier(0);
-  axis_ops<BND, D1xy, D2, ORD>(su, oy, SX, gj, ax[1], s.beta[1], d1[1], d2[3], av[1], kv[1]);
+  axis_ops<CY, D1xy, D2, ORD>(su, oy, SX, gj, ax[1], s.beta[1], d1[1], d2[3], av[1], kv[1]);
   __builtin_amdgcn_sched_barrier(0);
-  axis_ops<BND, D1z, D2, ORD>(su, oz, SX * SY, gk, ax[2], s.beta[2], d1[2], d2[5], av[2], kv[2]);
+  axis_ops<CZ, D1z, D2, ORD>(su, oz, SX * SY, gk, ax[2], s.beta[2], d1[2], d2[5], av[2], kv[2]);
   __builtin_amdgcn_sched_barrier(0);
   if constexpr (D2 && ORD == 64) {
     d1[0] = M.d1x;
@@ -442,11 +445,11 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
     // sgx[a + TX*(b + GXY*cc)] holds d_x u at block coords (i0+a, j0-2+b, k0-2+cc);
     // sgy[a + TX*(b + TY*cc)] holds d_y u at (i0+a, j0+b, k0-2+cc).
     const int oxy = ti + TX * ((2 - T.j0) + GXY * (tk + 2));
-    d2[1] = d1_op<BND, ORD>(sgx, oxy, TX, gj, ax[1]);
+    d2[1] = d1_op<CY, ORD>(sgx, oxy, TX, gj, ax[1]);
     const int oxz = ti + TX * ((tj + 2) + GXY * (2 - T.k0));
-    d2[2] = d1_op<BND, ORD>(sgx, oxz, TX * GXY, gk, ax[2]);
+    d2[2] = d1_op<CZ, ORD>(sgx, oxz, TX * GXY, gk, ax[2]);
     const int oyz = ti + TX * (tj + TY * (2 - T.k0));
-    d2[4] = d1_op<BND, ORD>(sgy, oyz, TX * TY, gk, ax[2]);
+    d2[4] = d1_op<CZ, ORD>(sgy, oyz, TX * TY, gk, ax[2]);
     __builtin_amdgcn_sched_barrier(0);
   }
   // upwinded derivatives contracted with beta, summed in the order of the
@@ -546,85 +549,87 @@ __device__ __forceinline__ PlaneSlots<ORD> make_plane_slots(const TileGeom& T, i
   return P;
 }
 
-// Boundary tiles (whose closure code leaves fewer registers) recompute the
-// entry coordinates in the loop instead of holding them.
-template <bool BND, int ORD>
+// Boundary tiles (LOOP; their closure code leaves fewer registers)
+// recompute the entry coordinates in the loop instead of holding them.  CL:
+// closure rows along the plane's derivative axis.
+template <bool LOOP, bool CL, int ORD>
 __device__ __forceinline__ void plane_x(const double* su, double* sgx, const PlaneSlots<ORD>& P, const TileGeom& T,
                                         const AxisC* ax, int t) {
   using G = Planes<ORD>;
-  if constexpr (BND) {
+  if constexpr (LOOP) {
     for (int e = t; e < G::NGX; e += NT) {
       const int a = e % TX;
       const int b = (e / TX) % G::GXY;
       const int cc = e / (TX * G::GXY);
       if (a < T.nx && b < T.ny + 2 * G::GW && cc < T.nz + 2 * G::GW) {
         const int o = (a + 3) + SX * ((b + 3 - G::GW) + SY * (cc + 3 - G::GW)) - (T.i0 + a);
-        sgx[e] = d1_op<BND, ORD>(su, o, 1, T.i0 + a, ax[0]);
+        sgx[e] = d1_op<CL, ORD>(su, o, 1, T.i0 + a, ax[0]);
       }
     }
   } else {
     constexpr int NPL = PlaneSlots<ORD>::NPL;
 #pragma unroll
     for (int k = 0; k < NPL; ++k)
-      if ((P.valid >> k) & 1u) sgx[t + k * NT] = d1_op<BND, ORD>(su, P.ox[k], 1, P.qx[k], ax[0]);
+      if ((P.valid >> k) & 1u) sgx[t + k * NT] = d1_op<CL, ORD>(su, P.ox[k], 1, P.qx[k], ax[0]);
   }
 }
-template <bool BND, int ORD>
+template <bool LOOP, bool CL, int ORD>
 __device__ __forceinline__ void plane_y(const double* su, double* sgy, const PlaneSlots<ORD>& P, const TileGeom& T,
                                         const AxisC* ax, int t) {
   using G = Planes<ORD>;
-  if constexpr (BND) {
+  if constexpr (LOOP) {
     for (int e = t; e < G::NGY; e += NT) {
       const int a = e % TX;
       const int b = (e / TX) % TY;
       const int cc = e / (TX * TY);
       if (a < T.nx && b < T.ny && cc < T.nz + 2 * G::GW) {
         const int o = (a + 3) + SX * ((b + 3) + SY * (cc + 3 - G::GW)) - (T.j0 + b) * SX;
-        sgy[e] = d1_op<BND, ORD>(su, o, SX, T.j0 + b, ax[1]);
+        sgy[e] = d1_op<CL, ORD>(su, o, SX, T.j0 + b, ax[1]);
       }
     }
   } else {
     constexpr int NPL = PlaneSlots<ORD>::NPL;
 #pragma unroll
     for (int k = 0; k < NPL; ++k)
-      if ((P.valid >> (NPL + k)) & 1u) sgy[t + k * NT] = d1_op<BND, ORD>(su, P.oy[k], SX, P.qy[k], ax[1]);
+      if ((P.valid >> (NPL + k)) & 1u) sgy[t + k * NT] = d1_op<CL, ORD>(su, P.oy[k], SX, P.qy[k], ax[1]);
   }
 }
 
-template <int I, bool BND, int ORD, bool GATHER>
+template <int I, int FM, int ORD, bool GATHER>
 __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], double* su, double* sgx,
                                          double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io,
                                          int t, const Slots& sl, const PlaneSlots<ORD>& ps, uint32_t zmask) {
   if constexpr (I < 24) {
     constexpr int V = h_order[I];
     using G = Planes<ORD>;
+    constexpr bool BND = FM != 0, CX = (FM & 1) != 0, CY = (FM & 2) != 0, CZ = (FM & 4) != 0;
     stage(pf, su, T, t, zmask);
     __syncthreads();
     Mixed M;
     if constexpr (has_d2(V) && G::SHARED) {
       // d_x plane -> d_x u, d_y d_x u, d_z d_x u; then the d_y plane in the
       // same buffer -> d_y u, d_z d_y u
-      plane_x<BND, ORD>(su, sgx, ps, T, C.ax, t);
+      plane_x<BND, CX, ORD>(su, sgx, ps, T, C.ax, t);
       if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
       __syncthreads();
       if (C.act) {
         const int ti = C.ti, tj = C.tj, tk = C.tk;
         M.d1x = sgx[ti + TX * ((tj + G::GW) + G::GXY * (tk + G::GW))];
-        M.dxy = d1_op<BND, ORD>(sgx, ti + TX * ((G::GW - T.j0) + G::GXY * (tk + G::GW)), TX, C.gj, C.ax[1]);
-        M.dxz = d1_op<BND, ORD>(sgx, ti + TX * ((tj + G::GW) + G::GXY * (G::GW - T.k0)), TX * G::GXY, C.gk, C.ax[2]);
+        M.dxy = d1_op<CY, ORD>(sgx, ti + TX * ((G::GW - T.j0) + G::GXY * (tk + G::GW)), TX, C.gj, C.ax[1]);
+        M.dxz = d1_op<CZ, ORD>(sgx, ti + TX * ((tj + G::GW) + G::GXY * (G::GW - T.k0)), TX * G::GXY, C.gk, C.ax[2]);
       }
       __syncthreads();
-      plane_y<BND, ORD>(su, sgx, ps, T, C.ax, t);
+      plane_y<BND, CY, ORD>(su, sgx, ps, T, C.ax, t);
       __syncthreads();
       if (C.act) {
         const int ti = C.ti, tj = C.tj, tk = C.tk;
         M.d1y = sgx[ti + TX * (tj + TY * (tk + G::GW))];
-        M.dyz = d1_op<BND, ORD>(sgx, ti + TX * (tj + TY * (G::GW - T.k0)), TX * TY, C.gk, C.ax[2]);
+        M.dyz = d1_op<CZ, ORD>(sgx, ti + TX * (tj + TY * (G::GW - T.k0)), TX * TY, C.gk, C.ax[2]);
       }
     } else {
       if constexpr (has_d2(V)) {
-        plane_x<BND, ORD>(su, sgx, ps, T, C.ax, t);
-        plane_y<BND, ORD>(su, sgy, ps, T, C.ax, t);
+        plane_x<BND, CX, ORD>(su, sgx, ps, T, C.ax, t);
+        plane_y<BND, CY, ORD>(su, sgy, ps, T, C.ax, t);
       }
       if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
       if constexpr (has_d2(V)) __syncthreads();
@@ -632,13 +637,19 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
     if constexpr (V == dgr::V_CHI) {
       if (C.act) s.finish_metric();
     }
-    process_var<V, BND, ORD>(s, C, su, sgx, sgy, M, T, io);
+    process_var<V, FM, ORD>(s, C, su, sgx, sgy, M, T, io);
     __syncthreads();
-    run_vars<I + 1, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, io, t, sl, ps, zmask);
+    run_vars<I + 1, FM, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, io, t, sl, ps, zmask);
   }
 }
 
-template <bool BND, int ORD, bool GATHER>
+// FM: 0 = tiles of blocks without flagged faces; 7 = boundary tiles with
+// the closure tests along all three axes; FM_FACES = boundary tiles, each
+// run through the body compiled for its block's flagged-face axes (x, y or z
+// alone, else all three).
+constexpr int FM_FACES = 8;
+
+template <int FM, int ORD, bool GATHER>
 __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ blocks,
                                                  const DevTile* __restrict__ tiles, int tile_begin, int ntiles,
                                                  const IOPtrs* __restrict__ io, DevParams P,
@@ -654,6 +665,7 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
   __shared__ double s_geo[6];       // pmin[3], h[3] of the current block
   __shared__ long long s_delta[GATHER ? NNBR : 1];  // gather mode: source distance per ghost direction
 
+  constexpr bool BND = FM != 0;
   const int t = threadIdx.x;
   if constexpr (BND) {
     // closure rows for the boundary stencils (visible after the first
@@ -762,7 +774,18 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
     const uint32_t zmask = BND ? beyond_face_mask(T, B.bflag, t) : 0u;
     PlaneSlots<ORD> ps;
     if constexpr (!BND) ps = make_plane_slots<ORD>(T, t);
-    run_vars<0, BND, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, ps, zmask);
+    if constexpr (FM == FM_FACES) {
+      // most boundary blocks have flagged faces along one axis only: their
+      // tiles skip the closure tests along the other two
+      const unsigned bf = C.bflag;
+      const int fm = __builtin_amdgcn_readfirstlane(((bf & 3u) ? 1 : 0) | ((bf & 12u) ? 2 : 0) | ((bf & 48u) ? 4 : 0));
+      if (fm == 1) run_vars<0, 1, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, ps, zmask);
+      else if (fm == 2) run_vars<0, 2, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, ps, zmask);
+      else if (fm == 4) run_vars<0, 4, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, ps, zmask);
+      else run_vars<0, 7, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, ps, zmask);
+    } else {
+      run_vars<0, FM, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, ps, zmask);
+    }
   }
 }
 
@@ -894,8 +917,8 @@ __device__ __forceinline__ void con_vars(KPoint& s, ConAcc& A, Ctx& C, double (&
     stage(pf, su, T, t, zmask);
     __syncthreads();
     if constexpr (con_d2(V)) {
-      plane_x<BND, 42>(su, sgx, ps, T, C.ax, t);
-      plane_y<BND, 42>(su, sgy, ps, T, C.ax, t);
+      plane_x<BND, BND, 42>(su, sgx, ps, T, C.ax, t);
+      plane_y<BND, BND, 42>(su, sgy, ps, T, C.ax, t);
     }
     if constexpr (I + 1 < 17) prefetch(pf, io->u.v[con_order[I + 1]], T, sl);
     if constexpr (con_d2(V)) __syncthreads();
@@ -1521,15 +1544,27 @@ int upload_ptrs(dgr_plan_t plan, double* const* rhs, const double* const* uin, h
 }
 }  // namespace
 
+namespace {
+using RhsKernel = decltype(&k_bssn_rhs<0, 42, false>);
+// Kernel of one tile range: interior or boundary tiles, stencil order,
+// gather mode.  deriv42 boundary tiles run the per-face-axis bodies; the
+// 6th-order boundary kernel keeps the one all-axes body.
+RhsKernel rhs_kernel(bool bnd, int order, bool gather) {
+  if (order == 64) {
+    if (gather) return bnd ? k_bssn_rhs<7, 64, true> : k_bssn_rhs<0, 64, true>;
+    return bnd ? k_bssn_rhs<7, 64, false> : k_bssn_rhs<0, 64, false>;
+  }
+  if (gather) return bnd ? k_bssn_rhs<FM_FACES, 42, true> : k_bssn_rhs<0, 42, true>;
+  return bnd ? k_bssn_rhs<FM_FACES, 42, false> : k_bssn_rhs<0, 42, false>;
+}
+}  // namespace
+
 int dgr_internal::launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st, bool gather) {
   const int ni = (int)plan->ntiles_interior;
   const int nbd = (int)plan->ntiles - ni;
   auto grid = [&](int n) { return plan->grid_mult == 0 ? n : std::min(plan->grid * (int)plan->grid_mult, n); };
-  const bool six = plan->order == 64;
-  const auto kin = gather ? (six ? k_bssn_rhs<false, 64, true> : k_bssn_rhs<false, 42, true>)
-                          : (six ? k_bssn_rhs<false, 64, false> : k_bssn_rhs<false, 42, false>);
-  const auto kbd = gather ? (six ? k_bssn_rhs<true, 64, true> : k_bssn_rhs<true, 42, true>)
-                          : (six ? k_bssn_rhs<true, 64, false> : k_bssn_rhs<true, 42, false>);
+  const auto kin = rhs_kernel(false, plan->order, gather);
+  const auto kbd = rhs_kernel(true, plan->order, gather);
   const int32_t* nb = gather ? plan->d_nbr : nullptr;
   const uint32_t* sl = gather ? plan->d_tile_slot : nullptr;
   if (ni > 0)
@@ -1559,10 +1594,7 @@ extern "C" int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const
   for (uint32_t b = 0; b < plan->nblocks; ++b) {
     const int tb = (int)plan->block_tile_begin[b];
     const int nt = (int)plan->block_tile_count[b];
-    const bool six = plan->order == 64;
-    const auto k = tb < (int)plan->ntiles_interior
-                       ? (six ? k_bssn_rhs<false, 64, false> : k_bssn_rhs<false, 42, false>)
-                       : (six ? k_bssn_rhs<true, 64, false> : k_bssn_rhs<true, 42, false>);
+    const auto k = rhs_kernel(tb >= (int)plan->ntiles_interior, plan->order, false);
     hipLaunchKernelGGL(k, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, tb, nt, plan->d_io,
                        plan->params, (const int32_t*)nullptr, (const uint32_t*)nullptr);
   }
@@ -1593,9 +1625,8 @@ extern "C" int dgr_bssn_rhs_blkwise(dgr_plan_t plan, double* const* rhs, const d
   // the subset list is reused by the next call: wait for earlier launches
   DGR_HIP(hipStreamSynchronize(st));
   DGR_HIP(hipMemcpy(plan->d_tiles_sub, sub.data(), sizeof(DevTile) * sub.size(), hipMemcpyHostToDevice));
-  const bool six = plan->order == 64;
-  const auto kin = six ? k_bssn_rhs<false, 64, false> : k_bssn_rhs<false, 42, false>;
-  const auto kbd = six ? k_bssn_rhs<true, 64, false> : k_bssn_rhs<true, 42, false>;
+  const auto kin = rhs_kernel(false, plan->order, false);
+  const auto kbd = rhs_kernel(true, plan->order, false);
   const int nbd = (int)sub.size() - (int)nin;
   auto grid = [&](int m) { return std::min(plan->grid * 2, m); };
   if (nin > 0)
@@ -1630,9 +1661,7 @@ extern "C" int dgr_bssn_rhs_blk(dgr_plan_t plan, double* out, const double* in, 
   if (!plan->d_io_blk) DGR_HIP(hipMalloc(&plan->d_io_blk, sizeof(IOPtrs)));
   DGR_HIP(hipStreamSynchronize(st));
   DGR_HIP(hipMemcpy(plan->d_io_blk, &io, sizeof(IOPtrs), hipMemcpyHostToDevice));
-  const bool six = plan->order == 64;
-  const auto k = tb < plan->ntiles_interior ? (six ? k_bssn_rhs<false, 64, false> : k_bssn_rhs<false, 42, false>)
-                                            : (six ? k_bssn_rhs<true, 64, false> : k_bssn_rhs<true, 42, false>);
+  const auto k = rhs_kernel(tb >= plan->ntiles_interior, plan->order, false);
   hipLaunchKernelGGL(k, dim3(tc), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, (int)tb, (int)tc,
                      plan->d_io_blk, plan->params, (const int32_t*)nullptr, (const uint32_t*)nullptr);
   DGR_HIP(hipGetLastError());
