@@ -1211,6 +1211,9 @@ void free_plan(dgr_plan_s* plan) {
   (void)hipFree(plan->d_norm);
   (void)hipFree(plan->d_tiles_sub);
   (void)hipFree(plan->d_io_blk);
+  if (plan->ev_fork) (void)hipEventDestroy(plan->ev_fork);
+  if (plan->ev_join) (void)hipEventDestroy(plan->ev_join);
+  if (plan->aux) (void)hipStreamDestroy(plan->aux);
   delete plan;
 }
 
@@ -1506,6 +1509,13 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
     free_plan(plan);
     return hip_fail(e, "hipMemcpy(plan)");
   }
+  e = hipStreamCreateWithFlags(&plan->aux, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&plan->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&plan->ev_join, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    free_plan(plan);
+    return hip_fail(e, "hipStreamCreate(plan)");
+  }
   const int rc = build_level_fill(plan, blocks, hb);
   if (rc != DGR_OK) {
     free_plan(plan);
@@ -1567,13 +1577,26 @@ int dgr_internal::launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st
   const auto kbd = rhs_kernel(true, plan->order, gather);
   const int32_t* nb = gather ? plan->d_nbr : nullptr;
   const uint32_t* sl = gather ? plan->d_tile_slot : nullptr;
+  // both ranges present: the boundary kernel (the longer tiles, launched
+  // first) runs on the plan's second stream beside the interior kernel
+  const bool fork = ni > 0 && nbd > 0;
+  hipStream_t sb = st;
+  if (fork) {
+    DGR_HIP(hipEventRecord(plan->ev_fork, st));
+    DGR_HIP(hipStreamWaitEvent(plan->aux, plan->ev_fork, 0));
+    sb = plan->aux;
+  }
+  if (nbd > 0)
+    hipLaunchKernelGGL(kbd, dim3(grid(nbd)), dim3(NT), 0, sb, plan->d_blocks, plan->d_tiles, ni, nbd, d_io,
+                       plan->params, nb, sl);
   if (ni > 0)
     hipLaunchKernelGGL(kin, dim3(grid(ni)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, 0, ni, d_io,
                        plan->params, nb, sl);
-  if (nbd > 0)
-    hipLaunchKernelGGL(kbd, dim3(grid(nbd)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, ni, nbd, d_io,
-                       plan->params, nb, sl);
   DGR_HIP(hipGetLastError());
+  if (fork) {
+    DGR_HIP(hipEventRecord(plan->ev_join, plan->aux));
+    DGR_HIP(hipStreamWaitEvent(st, plan->ev_join, 0));
+  }
   return DGR_OK;
 }
 

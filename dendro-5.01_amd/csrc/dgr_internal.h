@@ -131,6 +131,11 @@ struct dgr_plan_s {
   std::vector<dgr_internal::DevTile> h_tiles_blk;
   dgr_internal::DevTile* d_tiles_sub = nullptr;
   dgr_internal::IOPtrs* d_io_blk = nullptr;
+  // batched RHS: the boundary-tile kernel runs on `aux`, forked from and
+  // joined back to the caller's stream, so its workgroups fill the CUs that
+  // the interior kernel's tail leaves idle (default priority)
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace dgr_internal {
