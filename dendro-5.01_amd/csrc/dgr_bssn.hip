@@ -703,6 +703,16 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
     T.rx = T.nx + 6; T.ry = T.ny + 6; T.rz = T.nz + 6;
     T.nreg = T.rx * T.ry * T.rz;
 
+    if constexpr (FM == FM_FACES) {
+      // a tile touching one flagged face (tl.pad = 1 + 2 axis + side) takes
+      // that face's thread map: its closure points fill the first waves
+      if (t < NPT) {
+        const int m = tl.pad ? c_face_map[tl.pad - 1][t] : c_tile_map[t];
+        C.ti = m & 7;
+        C.tj = (m >> 3) & 7;
+        C.tk = m >> 6;
+      }
+    }
     C.act = t < NPT && C.ti < T.nx && C.tj < T.ny && C.tk < T.nz;
     C.gi = T.i0 + C.ti; C.gj = T.j0 + C.tj; C.gk = T.k0 + C.tk;
     C.bflag = B.bflag;
@@ -1217,6 +1227,27 @@ void free_plan(dgr_plan_s* plan) {
   delete plan;
 }
 
+// Thread map of a boundary tile (DevTile::pad): 1 + 2 axis + side when the
+// tile spans 7 points along that axis, begins (side 0) or ends (side 1) at
+// the block's interior edge on a flagged face, and no other flagged face has
+// closure points (3 or fewer points from it) in the tile; else 0.
+uint8_t face_map_id(const dgr_block& bl, const DevTile& t) {
+  const int q0[3] = {t.i0, t.j0, t.k0}, n[3] = {t.nx, t.ny, t.nz};
+  int cnt = 0, id = 0;
+  for (int a = 0; a < 3; ++a) {
+    const int sz = (int)bl.sz[a];
+    if (((bl.bflag >> (2 * a)) & 1u) && q0[a] <= 5) {
+      ++cnt;
+      if (n[a] == 7 && q0[a] == 3) id = 1 + 2 * a;
+    }
+    if (((bl.bflag >> (2 * a + 1)) & 1u) && q0[a] + n[a] - 1 >= sz - 6) {
+      ++cnt;
+      if (n[a] == 7 && q0[a] + 6 == sz - 4) id = 2 + 2 * a;
+    }
+  }
+  return cnt == 1 ? (uint8_t)id : 0;
+}
+
 // Stencil constants of one axis, with the operations of derivs.cpp
 // (idx = 1/dx; idx_by_2 = 0.5*idx; idx_by_12 = idx/12; idx_sqrd = 1/(dx*dx);
 // pre_factor_6_dx = -1/64/dx; smr3 = 59/48*64*dx, ...).
@@ -1446,7 +1477,7 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
             t.block = b;
             t.i0 = (uint16_t)cx.first; t.j0 = (uint16_t)cy.first; t.k0 = (uint16_t)cz.first;
             t.nx = (uint8_t)cx.second; t.ny = (uint8_t)cy.second; t.nz = (uint8_t)cz.second;
-            t.pad = 0;
+            t.pad = face_map_id(bl, t);
             ht.push_back(t);
           }
       plan->block_tile_count[b] = (uint32_t)ht.size() - plan->block_tile_begin[b];
