@@ -142,26 +142,26 @@ __device__ __forceinline__ bool tile_point(const DevTile& tl, const DevBlock& B,
 __constant__ double c_rkU[3] = {1.0 / 2.0, 1.0 / 2.0, 1.0};
 __constant__ double c_rkC[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
 
-// Stage s of the RK4 step.  gdt[g] is the time step of block group g; the
-// coefficients are formed as the reference forms them (RK4_U[s+1] * dt,
-// RK4_C[s] * dt) and then multiply k.  u_prev is always the state at the
-// start of the step; KIND fixes whether the running sum starts from it
-// (stage 0) and where the new values go (the stage buffer, or the state
-// after the last stage).  Every operand of a point is loaded before the
-// first store.
+// Stage s of the RK4 step, one workgroup per tile.  gdt[g] is the time step
+// of block group g; the coefficients are formed as the reference forms them
+// (RK4_U[s+1] * dt, RK4_C[s] * dt) and then multiply k.  u_prev is always
+// the state at the start of the step; KIND fixes whether the running sum
+// starts from it (stage 0) and where the new values go (the stage buffer,
+// or the state after the last stage).  Every operand of a point is loaded
+// before the first store.  The stores are non-temporal: no kernel reads
+// them back before the whole stage is written, and they would only evict
+// the inputs still to be read.
 template <int KIND>
 __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ blocks, const DevTile* __restrict__ tiles,
-                                              int ntiles, int chunk, Ptrs24 state, Work W,
-                                              const double* __restrict__ gdt, double chi_floor, int stage,
-                                              unsigned* __restrict__ flag) {
-  // contiguous runs of tiles per workgroup keep each workgroup on the same
-  // pages of the arrays it touches
-  const int wend = min(ntiles, (blockIdx.x + 1) * chunk);
-  for (int w = blockIdx.x * chunk; w < wend; ++w) {
+                                              int ntiles, Ptrs24 state, Work W, const double* __restrict__ gdt,
+                                              double chi_floor, int stage, unsigned* __restrict__ flag) {
+  {
+    const int w = blockIdx.x;
+    if (w >= ntiles) return;
     const DevTile tl = tiles[w];
     const DevBlock& B = blocks[tl.block];
     uint64_t o;
-    if (!tile_point(tl, B, threadIdx.x, o)) continue;
+    if (!tile_point(tl, B, threadIdx.x, o)) return;
     const double dt = gdt[B.group];
     const uint64_t co = (uint64_t)w * 343 + threadIdx.x;
     double un[24], kk[24];
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
       AccCursor d(W.acc, co);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
-        d.acc() = acc[v];
+        __builtin_nontemporal_store(acc[v], &d.acc());
         d.next(W.nacc);
       }
     } else {
@@ -211,12 +211,12 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
     if (bad) atomicOr(flag, (unsigned)bad);
     if (KIND == ST_LAST) {
 #pragma unroll
-      for (int v = 0; v < 24; ++v) state.v[v][o] = un[v];
+      for (int v = 0; v < 24; ++v) __builtin_nontemporal_store(un[v], (gdouble*)&state.v[v][o]);
     } else {
       WorkCursor d(W, o);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
-        d.stg() = un[v];
+        __builtin_nontemporal_store(un[v], &d.stg());
         d.next(W.dof);
       }
     }
@@ -310,19 +310,17 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
     if (rc) return rc;
     // stage combine + projection; the next RHS reads ghosts from the
     // neighbours' interiors (gather), so no refill runs between stages
-    const int grid = std::min(nt, std::max(1, p->grid * 2));
-    const int chunk = (nt + grid - 1) / grid;
     const Work W = {r->buf, r->dof, r->buf + 24 * r->dof, r->buf + 24 * r->dof + 24 * r->nacc, r->nacc};
-    const dim3 g((nt + chunk - 1) / chunk), b(384);
+    const dim3 g(std::max(nt, 1)), b(384);
     if (s == 0)
-      hipLaunchKernelGGL(k_stage<ST_FIRST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, chunk, r->state, W,
-                         r->d_gdt, p->chi_floor, s, r->d_flag);
+      hipLaunchKernelGGL(k_stage<ST_FIRST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
+                         p->chi_floor, s, r->d_flag);
     else if (s < 3)
-      hipLaunchKernelGGL(k_stage<ST_MID>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, chunk, r->state, W,
-                         r->d_gdt, p->chi_floor, s, r->d_flag);
+      hipLaunchKernelGGL(k_stage<ST_MID>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
+                         p->chi_floor, s, r->d_flag);
     else
-      hipLaunchKernelGGL(k_stage<ST_LAST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, chunk, r->state, W,
-                         r->d_gdt, p->chi_floor, s, r->d_flag);
+      hipLaunchKernelGGL(k_stage<ST_LAST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
+                         p->chi_floor, s, r->d_flag);
     // ghosts across level jumps (adaptive layouts) of the new stage state
     OutPtrs o;
     for (int v = 0; v < 24; ++v) o.v[v] = s < 3 ? r->stg.v[v] : r->state.v[v];
@@ -573,22 +571,41 @@ struct EtsTerms {
 
 // out = base + sum_q c[q] k_{idx[q]} on the interior points of every tile;
 // k_p variable v in the compact layout of the RHS in gather mode, at
-// kb + (p*24 + v) * nacc + (tile * 343 + local index).
+// kb + (p*24 + v) * nacc + (tile * 343 + local index).  PROJECT: the stage
+// input, projected (enforce_bssn_constraints) before it is stored; a point
+// the projection rejects (det gt < 0) keeps the combined values, as the
+// projection leaves such a node untouched.
+template <bool PROJECT>
 __global__ void __launch_bounds__(384) k_ets_combine(const DevBlock* __restrict__ blocks,
                                                     const DevTile* __restrict__ tiles, int ntiles, Ptrs24 out,
                                                     Ptrs24 base, const double* __restrict__ kb, uint64_t nacc,
-                                                    EtsTerms T) {
+                                                    EtsTerms T, double chi_floor, unsigned* __restrict__ flag) {
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const DevTile tl = tiles[w];
     const DevBlock& B = blocks[tl.block];
     uint64_t o;
     if (!tile_point(tl, B, threadIdx.x, o)) continue;
     const uint64_t co = (uint64_t)w * 343 + threadIdx.x;
-    for (int v = 0; v < 24; ++v) {
+    if constexpr (PROJECT) {
+      double u[24];
+#pragma unroll
+      for (int v = 0; v < 24; ++v) {
 #pragma clang fp contract(off)
-      double x = base.v[v][o];
-      for (int q = 0; q < T.n; ++q) x = x + T.c[q] * kb[((uint64_t)T.idx[q] * 24 + v) * nacc + co];
-      out.v[v][o] = x;
+        double x = base.v[v][o];
+        for (int q = 0; q < T.n; ++q) x = x + T.c[q] * kb[((uint64_t)T.idx[q] * 24 + v) * nacc + co];
+        u[v] = x;
+      }
+      const int bad = enforce_point(u, chi_floor);
+      if (bad) atomicOr(flag, (unsigned)bad);
+#pragma unroll
+      for (int v = 0; v < 24; ++v) out.v[v][o] = u[v];
+    } else {
+      for (int v = 0; v < 24; ++v) {
+#pragma clang fp contract(off)
+        double x = base.v[v][o];
+        for (int q = 0; q < T.n; ++q) x = x + T.c[q] * kb[((uint64_t)T.idx[q] * 24 + v) * nacc + co];
+        out.v[v][o] = x;
+      }
     }
   }
 }
@@ -697,10 +714,8 @@ extern "C" int dgr_ets_evolve(dgr_ets_t e, double dt, void* stream) {
       T.c[T.n] = aq * dt;
       T.idx[T.n++] = q;
     }
-    hipLaunchKernelGGL(k_ets_combine, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->tmp, e->state,
-                       kb, e->nacc, T);
-    hipLaunchKernelGGL(k_enforce, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->tmp, p->chi_floor,
-                       e->d_flag);
+    hipLaunchKernelGGL(k_ets_combine<true>, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->tmp,
+                       e->state, kb, e->nacc, T, p->chi_floor, e->d_flag);
     int rc = launch_level_fill(p, tmp_o, st);
     if (rc) return rc;
     rc = launch_rhs(p, e->d_io + s, st, true);
@@ -712,8 +727,8 @@ extern "C" int dgr_ets_evolve(dgr_ets_t e, double dt, void* stream) {
     T.idx[k] = k;
   }
   T.n = e->ns;
-  hipLaunchKernelGGL(k_ets_combine, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->state, e->state,
-                     kb, e->nacc, T);
+  hipLaunchKernelGGL(k_ets_combine<false>, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->state,
+                     e->state, kb, e->nacc, T, p->chi_floor, e->d_flag);
   hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(p, (int)p->nblocks)), dim3(RK_NT), 0, st, p->d_blocks, p->d_nbr,
                      (int)p->nblocks, e->state);
   DGR_HIP(hipGetLastError());
