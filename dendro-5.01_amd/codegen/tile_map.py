@@ -7,12 +7,14 @@ by 32 lanes is conflict-free when the 32 indices are distinct mod 32.  The
 points are grouped by index mod 32 and dealt out round-robin, so each run of
 32 consecutive threads holds 32 distinct residues.
 
-Face maps (one per axis and side) serve tiles of full extent that touch one
-flagged face: the points of the three slabs nearest the face, where the
-closure stencils apply, come first (nearest slab first), so they fill the
-first three waves and the other waves take no closure branch.  Each run of
-32 threads is filled greedily in that order with points of distinct
-residues, which keeps the runs conflict-free.
+Face maps serve tiles of full extent that touch flagged faces: one map per
+set of faces with at most one side per axis (26 sets: 6 faces, 12 edges, 8
+corners).  The points within three of a face of the set, where the closure
+stencils apply, come first (nearest first), so they fill the first waves
+and the other waves take no closure branch.  Each run of 32 threads is
+filled greedily in that order with points of distinct residues, which
+keeps the runs conflict-free.  Map index: sx + 3 sy + 9 sz - 1 with s = 0
+(no face), 1 (low face), 2 (high face) per axis.
 
     python tile_map.py > ../csrc/tile_map.h
 """
@@ -44,13 +46,34 @@ def lds_index(p):
     return (a + 3) + S * ((b + 3) + S * (c + 3))
 
 
-def face_order(axis, side):
-    """Points by slab distance from the face (axis, side 0 low / 1 high),
-    dealt into runs of 32 with distinct residues."""
-    near = (0, 1, 2) if side == 0 else (T - 1, T - 2, T - 3)
+def face_sets():
+    """(sx, sy, sz) per map index: 0 no face, 1 low face, 2 high face."""
+    out = []
+    for sz in range(3):
+        for sy in range(3):
+            for sx in range(3):
+                if sx or sy or sz:
+                    out.append((sx, sy, sz))
+    return out
+
+
+def face_distance(p, sides):
+    """Distance (in points) of p from the nearest face of the set."""
+    d = T
+    for a, sd in enumerate(sides):
+        if sd == 1:
+            d = min(d, p[a])
+        elif sd == 2:
+            d = min(d, T - 1 - p[a])
+    return d
+
+
+def face_order(sides):
+    """Points by distance from the nearest face of the set (closure points:
+    distance <= 2), dealt into runs of 32 with distinct residues."""
 
     def key(p):
-        return (near.index(p[axis]) if p[axis] in near else len(near), lds_index(p))
+        return (min(face_distance(p, sides), 3), lds_index(p))
 
     rem = sorted(((a, b, c) for c in range(T) for b in range(T) for a in range(T)), key=key)
     out = []
@@ -86,16 +109,15 @@ def main():
     for i in range(0, len(vals), 16):
         print("    " + ", ".join(str(v) for v in vals[i:i + 16]) + ",")
     print("};")
-    print("// Face maps, index 2 * axis + side (side 0 = low face): the same form, the")
-    print("// points of the three slabs nearest that face first.")
-    print("__constant__ unsigned short c_face_map[6][%d] = {" % len(vals))
-    for axis in range(3):
-        for side in range(2):
-            fv = check(face_order(axis, side))
-            print("  {")
-            for i in range(0, len(fv), 16):
-                print("    " + ", ".join(str(v) for v in fv[i:i + 16]) + ",")
-            print("  },")
+    print("// Face maps, index sx + 3 sy + 9 sz - 1 (s = 0 no face, 1 low, 2 high per axis):")
+    print("// the same form, the points within three of a face of the set first.")
+    print("__constant__ unsigned short c_face_map[26][%d] = {" % len(vals))
+    for sides in face_sets():
+        fv = check(face_order(sides))
+        print("  {  // faces %d %d %d" % sides)
+        for i in range(0, len(fv), 16):
+            print("    " + ", ".join(str(v) for v in fv[i:i + 16]) + ",")
+        print("  },")
     print("};")
 
 

@@ -627,11 +627,14 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
         M.dyz = d1_op<CZ, ORD>(sgx, ti + TX * (tj + TY * (G::GW - T.k0)), TX * TY, C.gk, C.ax[2]);
       }
     } else {
+      // gathering stages issue the next variable's loads before the planes
+      // (a longer overlap; measured -1% per RK4 step, +0.7% on the plain RHS)
+      if constexpr (GATHER && I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
       if constexpr (has_d2(V)) {
         plane_x<BND, CX, ORD>(su, sgx, ps, T, C.ax, t);
         plane_y<BND, CY, ORD>(su, sgy, ps, T, C.ax, t);
       }
-      if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
+      if constexpr (!GATHER && I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
       if constexpr (has_d2(V)) __syncthreads();
     }
     if constexpr (V == dgr::V_CHI) {
@@ -704,8 +707,8 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
     T.nreg = T.rx * T.ry * T.rz;
 
     if constexpr (FM == FM_FACES) {
-      // a tile touching one flagged face (tl.pad = 1 + 2 axis + side) takes
-      // that face's thread map: its closure points fill the first waves
+      // a tile touching flagged faces (tl.pad != 0) takes the thread map of
+      // those faces: its closure points fill the first waves
       if (t < NPT) {
         const int m = tl.pad ? c_face_map[tl.pad - 1][t] : c_tile_map[t];
         C.ti = m & 7;
@@ -1227,25 +1230,30 @@ void free_plan(dgr_plan_s* plan) {
   delete plan;
 }
 
-// Thread map of a boundary tile (DevTile::pad): 1 + 2 axis + side when the
-// tile spans 7 points along that axis, begins (side 0) or ends (side 1) at
-// the block's interior edge on a flagged face, and no other flagged face has
-// closure points (3 or fewer points from it) in the tile; else 0.
+// Thread map of a boundary tile (DevTile::pad): 0 = the default map, else
+// 1 + index of the face map of the flagged faces that have closure points
+// (3 or fewer points from the face) in the tile, sx + 3 sy + 9 sz - 1 with
+// s = 0 (none), 1 (low face), 2 (high face) per axis.  A face map applies
+// when, along every axis with such a face, the tile spans 7 points and
+// begins (low face) or ends (high face) at the block's interior edge, and no
+// axis has closure points of both faces.
 uint8_t face_map_id(const dgr_block& bl, const DevTile& t) {
   const int q0[3] = {t.i0, t.j0, t.k0}, n[3] = {t.nx, t.ny, t.nz};
-  int cnt = 0, id = 0;
-  for (int a = 0; a < 3; ++a) {
+  int idx = 0, mul = 1;
+  for (int a = 0; a < 3; ++a, mul *= 3) {
     const int sz = (int)bl.sz[a];
-    if (((bl.bflag >> (2 * a)) & 1u) && q0[a] <= 5) {
-      ++cnt;
-      if (n[a] == 7 && q0[a] == 3) id = 1 + 2 * a;
-    }
-    if (((bl.bflag >> (2 * a + 1)) & 1u) && q0[a] + n[a] - 1 >= sz - 6) {
-      ++cnt;
-      if (n[a] == 7 && q0[a] + 6 == sz - 4) id = 2 + 2 * a;
+    const bool lo = ((bl.bflag >> (2 * a)) & 1u) && q0[a] <= 5;
+    const bool hi = ((bl.bflag >> (2 * a + 1)) & 1u) && q0[a] + n[a] - 1 >= sz - 6;
+    if (lo && hi) return 0;
+    if (lo) {
+      if (n[a] != 7 || q0[a] != 3) return 0;
+      idx += mul;
+    } else if (hi) {
+      if (n[a] != 7 || q0[a] + 6 != sz - 4) return 0;
+      idx += 2 * mul;
     }
   }
-  return cnt == 1 ? (uint8_t)id : 0;
+  return (uint8_t)idx;
 }
 
 // Stencil constants of one axis, with the operations of derivs.cpp

@@ -36,7 +36,7 @@ struct DevTile {
   uint32_t block;
   uint16_t i0, j0, k0;  // first interior point of the tile (block coords)
   uint8_t nx, ny, nz;
-  uint8_t pad;  // boundary tiles: thread map (0 default, 1 + 2 axis + side: face map)
+  uint8_t pad;  // boundary tiles: thread map (0 default, else 1 + face map index)
 };
 
 struct VarPtrs {

@@ -18,7 +18,7 @@ T, S = 7, 13
 
 
 def read_maps():
-    text = open(HDR).read()
+    text = re.sub(r"//[^\n]*", "", open(HDR).read())
     head, _, faces = text.partition("c_face_map")
     default = [int(v) for v in re.findall(r"\b\d+\b", head.split("{", 1)[1].split("}", 1)[0])]
     blocks = re.findall(r"\{([^{}]*)\}", faces.split("=", 1)[1])
@@ -42,7 +42,7 @@ def test_header_matches_generator():
 
 def test_maps_are_conflict_free_permutations():
     default, face = read_maps()
-    assert len(face) == 6
+    assert len(face) == 26
     for m in [default] + face:
         pts = [point(v) for v in m]
         assert len(pts) == T ** 3 and len(set(pts)) == T ** 3
@@ -52,14 +52,36 @@ def test_maps_are_conflict_free_permutations():
             assert len(rs) == len(set(rs)), h
 
 
-@pytest.mark.parametrize("axis", [0, 1, 2])
-@pytest.mark.parametrize("side", [0, 1])
-def test_face_map_groups_closure_points(axis, side):
+FACE_SETS = [(sx, sy, sz) for sz in range(3) for sy in range(3) for sx in range(3) if sx or sy or sz]
+
+
+def face_distance(p, sides):
+    d = T
+    for a, sd in enumerate(sides):
+        if sd == 1:
+            d = min(d, p[a])
+        elif sd == 2:
+            d = min(d, T - 1 - p[a])
+    return d
+
+
+def test_face_map_count_and_index():
     _, face = read_maps()
-    pts = [point(v) for v in face[2 * axis + side]]
-    near = (0, 1, 2) if side == 0 else (6, 5, 4)
-    pos = [t for t, p in enumerate(pts) if p[axis] in near]
-    assert len(pos) == 3 * T * T
-    assert max(pos) < 3 * 64  # waves 0-2
-    # the nearest slab (the one-sided rows) fills wave 0 first
-    assert all(pts[t][axis] in near[:2] for t in range(64))
+    assert len(face) == 26
+    # index sx + 3 sy + 9 sz - 1
+    assert [sx + 3 * sy + 9 * sz - 1 for sx, sy, sz in FACE_SETS] == list(range(26))
+
+
+@pytest.mark.parametrize("sides", FACE_SETS)
+def test_face_map_groups_closure_points(sides):
+    _, face = read_maps()
+    pts = [point(v) for v in face[sides[0] + 3 * sides[1] + 9 * sides[2] - 1]]
+    pos = [t for t, p in enumerate(pts) if face_distance(p, sides) <= 2]
+    nfaces = sum(1 for x in sides if x)
+    assert len(pos) == T ** 3 - (T - 3) ** nfaces * T ** (3 - nfaces)
+    waves = -(-len(pos) // 64)
+    # closure points first: at most one extra wave where residues force interleaving,
+    # none for a single face (the three slabs fill waves 0-2)
+    assert max(pos) < 64 * (waves + (0 if nfaces == 1 else 1))
+    # the nearest points (one-sided rows) fill wave 0
+    assert all(face_distance(pts[t], sides) <= 1 for t in range(64))
