@@ -262,9 +262,10 @@ def serving(njobs, budget_gb, max_points):
 def extras(plan, blocks, U, stream, steps, warm):
     """Timings of the entry points around the RHS on the same grid: the
     constraint monitor (dgr_physcon + masked norms), puncture initial data
-    (dgr_init_puncture) and one RK3 step of the Butcher-tableau stepper."""
+    (dgr_init_puncture), one RK3 step of the Butcher-tableau stepper and the
+    RHS with the 6th-order stencils (deriv64, default schedule)."""
     import torch
-    from pydgr import ETS, ETS_RK3
+    from pydgr import ETS, ETS_RK3, Plan, make_params
 
     pts = layout_interior_points(blocks)
     padded = sum(b["sz"][0] * b["sz"][1] * b["sz"][2] for b in blocks)
@@ -294,6 +295,10 @@ def extras(plan, blocks, U, stream, steps, warm):
     ets = ETS(plan, W, type=ETS_RK3)
     rk3_ms = timed(lambda: ets.evolve(0.25 / 64, 1, stream), max(2, steps // 2))
     ets.close()
+    p64 = Plan(blocks, make_params(ko_sigma=0.4, eta_const=2.0, deriv_order=64))
+    W.zero_()
+    d64_ms = timed(lambda: p64.rhs(W, U, stream), steps)
+    p64.close()
     del C, W
     torch.cuda.empty_cache()
     return {"physcon": {"ms": pc_ms, "points_per_s": pts / (pc_ms * 1e-3),
@@ -302,7 +307,9 @@ def extras(plan, blocks, U, stream, steps, warm):
             "physcon_norms_ms": norm_ms,
             "init_puncture": {"ms": init_ms, "padded_points_per_s": padded / (init_ms * 1e-3),
                               "achieved_gbs": 8 * 24 * padded / (init_ms * 1e-3) / 1e9},
-            "ets_rk3_step": {"ms": rk3_ms, "rhs_points_per_s": 3 * pts / (rk3_ms * 1e-3)}}
+            "ets_rk3_step": {"ms": rk3_ms, "rhs_points_per_s": 3 * pts / (rk3_ms * 1e-3)},
+            "rhs_deriv64": {"ms": d64_ms, "points_per_s": pts / (d64_ms * 1e-3),
+                            "hbm_frac": algorithmic_bytes(blocks) / (d64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
 
 
 def layout_interior_points(blocks):
@@ -376,8 +383,9 @@ def main():
     ext = None
     if not a.no_extras:
         ext = extras(plan, blocks, U, stream, a.steps, a.warmup)
-        log("constraint monitor %.4f ms, norms %.4f ms, puncture data %.4f ms, ETS RK3 step %.4f ms" %
-            (ext["physcon"]["ms"], ext["physcon_norms_ms"], ext["init_puncture"]["ms"], ext["ets_rk3_step"]["ms"]))
+        log("constraint monitor %.4f ms, norms %.4f ms, puncture data %.4f ms, ETS RK3 step %.4f ms, "
+            "deriv64 RHS %.4f ms" % (ext["physcon"]["ms"], ext["physcon_norms_ms"], ext["init_puncture"]["ms"],
+                                     ext["ets_rk3_step"]["ms"], ext["rhs_deriv64"]["ms"]))
     plan.close()
     del U
     torch.cuda.empty_cache()
