@@ -1596,12 +1596,11 @@ int upload_ptrs(dgr_plan_t plan, double* const* rhs, const double* const* uin, h
 namespace {
 using RhsKernel = decltype(&k_bssn_rhs<0, 42, false>);
 // Kernel of one tile range: interior or boundary tiles, stencil order,
-// gather mode.  deriv42 boundary tiles run the per-face-axis bodies; the
-// 6th-order boundary kernel keeps the one all-axes body.
+// gather mode.  Boundary tiles run the per-face-axis bodies.
 RhsKernel rhs_kernel(bool bnd, int order, bool gather) {
   if (order == 64) {
-    if (gather) return bnd ? k_bssn_rhs<7, 64, true> : k_bssn_rhs<0, 64, true>;
-    return bnd ? k_bssn_rhs<7, 64, false> : k_bssn_rhs<0, 64, false>;
+    if (gather) return bnd ? k_bssn_rhs<FM_FACES, 64, true> : k_bssn_rhs<0, 64, true>;
+    return bnd ? k_bssn_rhs<FM_FACES, 64, false> : k_bssn_rhs<0, 64, false>;
   }
   if (gather) return bnd ? k_bssn_rhs<FM_FACES, 42, true> : k_bssn_rhs<0, 42, true>;
   return bnd ? k_bssn_rhs<FM_FACES, 42, false> : k_bssn_rhs<0, 42, false>;
