@@ -1219,6 +1219,7 @@ void free_plan(dgr_plan_s* plan) {
   (void)hipFree(plan->d_io);
   (void)hipFree(plan->d_nbr);
   (void)hipFree(plan->d_fill_items);
+  (void)hipFree(plan->d_fill_groups);
   (void)hipFree(plan->d_fill_copies);
   (void)hipFree(plan->d_io_con);
   (void)hipFree(plan->d_norm);

@@ -118,8 +118,10 @@ struct dgr_plan_s {
   uint32_t ngroups = 1;                  // 1 + largest block group
   int order = 42;                        // stencil set: 42 (deriv42) or 64 (deriv64 + ko_deriv42)
   // level jumps (adaptive layouts): interpolation boxes and copies
-  dgr_internal::FillItem* d_fill_items = nullptr;
+  dgr_internal::FillItem* d_fill_items = nullptr;  // sorted by coarse element
   uint32_t nfill_items = 0;
+  uint32_t* d_fill_groups = nullptr;  // first item of each coarse element, + nfill_items
+  uint32_t nfill_groups = 0;
   dgr_internal::FillCopy* d_fill_copies = nullptr;
   uint64_t nfill_copies = 0;
   // constraint monitor: pointer table, norm partials (allocated on first use)

@@ -15,7 +15,8 @@
  *                spacing, so each axis needs the 7 Lagrange weights of the
  *                coarse nodes at one of the 13 positions 0, 1/2, ..., 6.  The
  *                box is interpolated separably, x then y then z, in LDS
- *                (k_fill_interp, one workgroup per box);
+ *                (k_fill_interp, one workgroup per coarse element and all
+ *                the boxes interpolated from it);
  *   coarse side  every ghost point coincides with a node of a finer block
  *                (coarse spacing = 2 fine spacings) and copies it
  *                (k_fill_copy).
@@ -63,57 +64,67 @@ __device__ __forceinline__ void lagrange_table(double (*W)[7], int t) {
 // Four variables per pass: four times the work between barriers.
 constexpr int VB = 4;
 
-__global__ void __launch_bounds__(NF) k_fill_interp(const FillItem* __restrict__ items, uint32_t nitems,
+// One workgroup per coarse element (a group of boxes, items[groups[g]] ..
+// items[groups[g+1]-1], all interpolated from that element): each pass
+// stages VB variables of the element once and interpolates every box of
+// the group from it.
+__global__ void __launch_bounds__(NF) k_fill_interp(const FillItem* __restrict__ items,
+                                                     const uint32_t* __restrict__ groups, uint32_t ngroups,
                                                      OutPtrs u) {
   __shared__ double W[13][7];
   __shared__ double sC[VB][343], sX[VB][343], sY[VB][343];
   const int t = threadIdx.x;
   lagrange_table(W, t);
-  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const FillItem I = items[it];
-    const int nx = I.n[0], ny = I.n[1], nz = I.n[2];
-    const int nX = nx * 49, nY = nx * ny * 7, nZ = nx * ny * nz;
+  for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const uint32_t i0 = groups[g], i1 = groups[g + 1];
+    const FillItem E = items[i0];  // src and ssz are the group's
     for (int v0 = 0; v0 < 24; v0 += VB) {
-      __syncthreads();  // previous pass (or item) done with sC, sY; W visible
+      __syncthreads();  // previous pass (or group) done with sC, sX, sY; W visible
       for (int e = t; e < VB * 343; e += NF) {
         const int w = e / 343, m = e % 343;
-        const gdouble* src = (const gdouble*)u.v[v0 + w] + I.src;
-        sC[w][m] = src[(m % 7) + (uint64_t)I.ssz[0] * ((m / 7) % 7 + (uint64_t)I.ssz[1] * (m / 49))];
+        const gdouble* src = (const gdouble*)u.v[v0 + w] + E.src;
+        sC[w][m] = src[(m % 7) + (uint64_t)E.ssz[0] * ((m / 7) % 7 + (uint64_t)E.ssz[1] * (m / 49))];
       }
       __syncthreads();
-      // along x: X[p][b][c] = sum_a W[s0x + p][a] C[a][b][c]
-      for (int e = t; e < VB * nX; e += NF) {
-        const int w = e / nX, f = e % nX;
-        const int p = f % nx, b = (f / nx) % 7, c = f / (nx * 7);
-        const double* wt = W[I.s0[0] + p];
-        const double* cc = sC[w] + 7 * b + 49 * c;
-        double r = 0.0;
+      for (uint32_t it = i0; it < i1; ++it) {
+        const FillItem I = items[it];
+        const int nx = I.n[0], ny = I.n[1], nz = I.n[2];
+        const int nX = nx * 49, nY = nx * ny * 7, nZ = nx * ny * nz;
+        // along x: X[p][b][c] = sum_a W[s0x + p][a] C[a][b][c]
+        // (the previous box's y sweep, the last reader of sX, ended at a barrier)
+        for (int e = t; e < VB * nX; e += NF) {
+          const int w = e / nX, f = e % nX;
+          const int p = f % nx, b = (f / nx) % 7, c = f / (nx * 7);
+          const double* wt = W[I.s0[0] + p];
+          const double* cc = sC[w] + 7 * b + 49 * c;
+          double r = 0.0;
 #pragma unroll
-        for (int a = 0; a < 7; ++a) r += wt[a] * cc[a];
-        sX[w][f] = r;
-      }
-      __syncthreads();
-      // along y: Y[p][q][c] = sum_b W[s0y + q][b] X[p][b][c]
-      for (int e = t; e < VB * nY; e += NF) {
-        const int w = e / nY, f = e % nY;
-        const int p = f % nx, q = (f / nx) % ny, c = f / (nx * ny);
-        const double* wt = W[I.s0[1] + q];
-        double r = 0.0;
+          for (int a = 0; a < 7; ++a) r += wt[a] * cc[a];
+          sX[w][f] = r;
+        }
+        __syncthreads();
+        // along y: Y[p][q][c] = sum_b W[s0y + q][b] X[p][b][c]
+        for (int e = t; e < VB * nY; e += NF) {
+          const int w = e / nY, f = e % nY;
+          const int p = f % nx, q = (f / nx) % ny, c = f / (nx * ny);
+          const double* wt = W[I.s0[1] + q];
+          double r = 0.0;
 #pragma unroll
-        for (int b = 0; b < 7; ++b) r += wt[b] * sX[w][p + nx * (b + 7 * c)];
-        sY[w][f] = r;
-      }
-      __syncthreads();
-      // along z, into the fine block's padding
-      for (int e = t; e < VB * nZ; e += NF) {
-        const int w = e / nZ, f = e % nZ;
-        const int p = f % nx, q = (f / nx) % ny, r3 = f / (nx * ny);
-        const double* wt = W[I.s0[2] + r3];
-        double r = 0.0;
+          for (int b = 0; b < 7; ++b) r += wt[b] * sX[w][p + nx * (b + 7 * c)];
+          sY[w][f] = r;
+        }
+        __syncthreads();
+        // along z, into the fine block's padding
+        for (int e = t; e < VB * nZ; e += NF) {
+          const int w = e / nZ, f = e % nZ;
+          const int p = f % nx, q = (f / nx) % ny, r3 = f / (nx * ny);
+          const double* wt = W[I.s0[2] + r3];
+          double r = 0.0;
 #pragma unroll
-        for (int c = 0; c < 7; ++c) r += wt[c] * sY[w][p + nx * (q + ny * c)];
-        gdouble* dst = (gdouble*)u.v[v0 + w] + I.dst;
-        dst[p + (uint64_t)I.dsz[0] * (q + (uint64_t)I.dsz[1] * r3)] = r;
+          for (int c = 0; c < 7; ++c) r += wt[c] * sY[w][p + nx * (q + ny * c)];
+          gdouble* dst = (gdouble*)u.v[v0 + w] + I.dst;
+          dst[p + (uint64_t)I.dsz[0] * (q + (uint64_t)I.dsz[1] * r3)] = r;
+        }
       }
     }
   }
@@ -257,11 +268,22 @@ int dgr_internal::build_level_fill(dgr_plan_t plan, const dgr_block* blocks, con
           }
     }
   }
+  // boxes interpolated from one coarse element form one group (one
+  // workgroup stages the element once per pass for all of them)
+  std::stable_sort(items.begin(), items.end(),
+                   [](const FillItem& a, const FillItem& b) { return a.src < b.src; });
+  std::vector<uint32_t> groups;
+  for (uint32_t i = 0; i < (uint32_t)items.size(); ++i)
+    if (i == 0 || items[i].src != items[i - 1].src) groups.push_back(i);
+  groups.push_back((uint32_t)items.size());
   plan->nfill_items = (uint32_t)items.size();
+  plan->nfill_groups = (uint32_t)groups.size() - 1;
   plan->nfill_copies = copies.size();
   if (!items.empty()) {
     DGR_HIP(hipMalloc(&plan->d_fill_items, sizeof(FillItem) * items.size()));
     DGR_HIP(hipMemcpy(plan->d_fill_items, items.data(), sizeof(FillItem) * items.size(), hipMemcpyHostToDevice));
+    DGR_HIP(hipMalloc(&plan->d_fill_groups, sizeof(uint32_t) * groups.size()));
+    DGR_HIP(hipMemcpy(plan->d_fill_groups, groups.data(), sizeof(uint32_t) * groups.size(), hipMemcpyHostToDevice));
   }
   if (!copies.empty()) {
     DGR_HIP(hipMalloc(&plan->d_fill_copies, sizeof(FillCopy) * copies.size()));
@@ -277,8 +299,9 @@ int dgr_internal::launch_level_fill(dgr_plan_t plan, const OutPtrs& u, hipStream
                        u);
   }
   if (plan->nfill_items) {
-    const uint32_t g = std::min<uint32_t>(plan->nfill_items, 16384);
-    hipLaunchKernelGGL(k_fill_interp, dim3(g), dim3(NF), 0, st, plan->d_fill_items, plan->nfill_items, u);
+    const uint32_t g = std::min<uint32_t>(plan->nfill_groups, 16384);
+    hipLaunchKernelGGL(k_fill_interp, dim3(g), dim3(NF), 0, st, plan->d_fill_items, plan->d_fill_groups,
+                       plan->nfill_groups, u);
   }
   DGR_HIP(hipGetLastError());
   return DGR_OK;
