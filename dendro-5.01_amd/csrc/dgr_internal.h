@@ -120,7 +120,7 @@ struct dgr_plan_s {
   // level jumps (adaptive layouts): interpolation boxes and copies
   dgr_internal::FillItem* d_fill_items = nullptr;  // sorted by coarse element
   uint32_t nfill_items = 0;
-  uint32_t* d_fill_groups = nullptr;  // first item of each coarse element, + nfill_items
+  uint32_t* d_fill_groups = nullptr;  // first item of each group (one coarse element), + nfill_items
   uint32_t nfill_groups = 0;
   dgr_internal::FillCopy* d_fill_copies = nullptr;
   uint64_t nfill_copies = 0;
