@@ -114,6 +114,27 @@ def test_repeat_calls_deterministic(cuda):
     assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
 
 
+def test_rhs_ordered_on_the_callers_stream(cuda):
+    """The boundary-tile kernel runs on the plan's own stream; the call must
+    still complete in order on the caller's stream: work queued after it on
+    that stream (a copy) sees every tile, with no device-wide sync."""
+    blocks, dof = layout.uniform_grid(4, dx=1.0 / 64, outer_bflags=True)
+    U = data.flat_bumps(blocks, dof, seed=10).to(cuda)
+    plan = Plan(blocks, make_params())
+    ref = torch.full_like(U, SENTINEL)
+    plan.rhs(ref, U)
+    torch.cuda.synchronize()
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        for _ in range(3):
+            R = torch.full_like(U, SENTINEL)
+            plan.rhs(R, U, st)
+            C = R.clone()
+            st.synchronize()
+            assert torch.equal(C, ref)
+    plan.close()
+
+
 # 6th-order option: deriv64 first / second / advective derivatives with
 # ko_deriv42 (QuadGrav/include/derivs.h:80-96), same bar.
 
