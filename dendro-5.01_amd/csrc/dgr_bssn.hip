@@ -646,10 +646,10 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
   }
 }
 
-// FM: 0 = tiles of blocks without flagged faces; 7 = boundary tiles with
-// the closure tests along all three axes; FM_FACES = boundary tiles, each
-// run through the body compiled for its block's flagged-face axes (x, y or z
-// alone, else all three).
+// FM: 0 = tiles of blocks without flagged faces; FM_FACES = boundary tiles,
+// each run through the body compiled for its block's flagged-face axes: bit
+// 0/1/2 = closure tests along x/y/z, so 1, 2 or 4 for one axis, 7 for the
+// edge and corner blocks (the bodies are run_vars<0, 1|2|4|7, ...>).
 constexpr int FM_FACES = 8;
 
 template <int FM, int ORD, bool GATHER>
