@@ -627,14 +627,15 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
         M.dyz = d1_op<CZ, ORD>(sgx, ti + TX * (tj + TY * (G::GW - T.k0)), TX * TY, C.gk, C.ax[2]);
       }
     } else {
-      // gathering stages issue the next variable's loads before the planes
-      // (a longer overlap; measured -1% per RK4 step, +0.7% on the plain RHS)
-      if constexpr (GATHER && I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
+      // gathering interior tiles issue the next variable's loads before the
+      // planes (a longer overlap; measured -1.4% per RK4 step; +0.7% on the
+      // plain RHS, and the boundary tiles' spills leave no room for it)
+      if constexpr (GATHER && FM == 0 && I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
       if constexpr (has_d2(V)) {
         plane_x<BND, CX, ORD>(su, sgx, ps, T, C.ax, t);
         plane_y<BND, CY, ORD>(su, sgy, ps, T, C.ax, t);
       }
-      if constexpr (!GATHER && I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
+      if constexpr ((!GATHER || FM != 0) && I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
       if constexpr (has_d2(V)) __syncthreads();
     }
     if constexpr (V == dgr::V_CHI) {
