@@ -54,16 +54,19 @@ struct Ptrs24 {
 };
 
 // RK4 work memory: one allocation holding the stage state (stg) as 24
-// padded arrays of dof doubles each, then the RHS (k) and the running final
-// sum in a compact tile-major layout (343 slots per tile in block order, the
-// layout the RHS kernels write in their gather mode); all addressed
-// arithmetically so that only the caller's state needs a pointer table.
+// padded arrays of dof doubles each, then the RHS (k of stages 1-3), the
+// running final sum and k of stage 0 in a compact tile-major layout (343
+// slots per tile in block order, the layout the RHS kernels write in their
+// gather mode); all addressed arithmetically so that only the caller's state
+// needs a pointer table.  k_0 has its own array so that stage 0 need not
+// store the running sum: stage 1 forms it from k_0 (same operations).
 struct Work {
   double* base;   // stage state (24 x dof, padded layout)
   uint64_t dof;
-  double* kc;     // k, compact: 24 x nacc
+  double* kc;     // k of this stage, compact: 24 x nacc
   double* acc;    // running sum, compact: 24 x nacc
   uint64_t nacc;
+  double* k0;     // k of stage 0, compact: 24 x nacc
 };
 
 // Per-thread cursor over the 24 variables of one point in the work memory.
@@ -93,7 +96,7 @@ struct AccCursor {
   }
 };
 
-enum StageKind { ST_FIRST = 0, ST_MID = 1, ST_LAST = 2 };
+enum StageKind { ST_FIRST = 0, ST_SECOND = 1, ST_MID = 2, ST_LAST = 3 };
 
 // enforce_bssn_constraints on one point (bssn_constraints.cpp:12-172):
 // rescale gt to unit determinant, remove the trace of At, floor chi and
@@ -165,18 +168,41 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
     const double dt = gdt[B.group];
     const uint64_t co = (uint64_t)w * 343 + threadIdx.x;
     double un[24], kk[24];
-    if (KIND != ST_LAST) {
-      const double cu = c_rkU[stage] * dt, cc = c_rkC[stage] * dt;
-      double acc[24];
+    if (KIND == ST_FIRST) {
+      // the running sum u_prev + (RK4_C[0] dt) k_0 is formed in stage 1
+      const double cu = c_rkU[stage] * dt;
       AccCursor c(W.kc, co);
-      AccCursor a(W.acc, co);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
         un[v] = state.v[v][o];  // u_prev: the state at the start of the step
         kk[v] = c.acc();
-        acc[v] = KIND == ST_FIRST ? un[v] : a.acc();
+        c.next(W.nacc);
+      }
+#pragma unroll
+      for (int v = 0; v < 24; ++v) {
+#pragma clang fp contract(off)
+        un[v] = un[v] + cu * kk[v];
+      }
+    } else if (KIND != ST_LAST) {
+      const double cu = c_rkU[stage] * dt, cc = c_rkC[stage] * dt;
+      double acc[24];
+      AccCursor c(W.kc, co);
+      AccCursor a(KIND == ST_SECOND ? W.k0 : W.acc, co);
+#pragma unroll
+      for (int v = 0; v < 24; ++v) {
+        un[v] = state.v[v][o];  // u_prev: the state at the start of the step
+        kk[v] = c.acc();
+        acc[v] = a.acc();       // stage 1: k_0; later: the running sum
         c.next(W.nacc);
         a.next(W.nacc);
+      }
+      if (KIND == ST_SECOND) {
+        const double c0 = c_rkC[0] * dt;
+#pragma unroll
+        for (int v = 0; v < 24; ++v) {
+#pragma clang fp contract(off)
+          acc[v] = un[v] + c0 * acc[v];
+        }
       }
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
@@ -284,7 +310,7 @@ int grid_for(dgr_plan_t plan, int n) {
 struct dgr_rk4_s {
   dgr_plan_t plan = nullptr;
   uint64_t dof = 0;
-  double* buf = nullptr;        // stage state (24 x dof), k and running sum (24 x 343 per tile each)
+  double* buf = nullptr;        // stage state (24 x dof), k, running sum, k_0 (24 x 343 per tile each)
   Ptrs24 state{}, stg{}, k{};  // the state and the work arrays in buf
   uint64_t nacc = 0;           // compact running-sum slots per variable (343 per tile)
   IOPtrs* d_io = nullptr;       // RHS pointer tables: [0] (u -> k), [1] (stage -> k)
@@ -305,15 +331,21 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
   dgr_plan_t p = r->plan;
   const int nt = (int)p->ntiles;
   for (int s = 0; s < 4; ++s) {
-    // k_s = RHS(u_s): stage 0 reads the state, later stages the stage buffer
+    // k_s = RHS(u_s): stage 0 reads the state and writes k_0, later stages
+    // read the stage buffer and write k
     int rc = launch_rhs(p, r->d_io + (s == 0 ? 0 : 1), st, true);
     if (rc) return rc;
     // stage combine + projection; the next RHS reads ghosts from the
     // neighbours' interiors (gather), so no refill runs between stages
-    const Work W = {r->buf, r->dof, r->buf + 24 * r->dof, r->buf + 24 * r->dof + 24 * r->nacc, r->nacc};
+    double* const kbuf = r->buf + 24 * r->dof;
+    double* const k0 = r->buf + 24 * r->dof + 48 * r->nacc;
+    const Work W = {r->buf, r->dof, s == 0 ? k0 : kbuf, r->buf + 24 * r->dof + 24 * r->nacc, r->nacc, k0};
     const dim3 g(std::max(nt, 1)), b(384);
     if (s == 0)
       hipLaunchKernelGGL(k_stage<ST_FIRST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
+                         p->chi_floor, s, r->d_flag);
+    else if (s == 1)
+      hipLaunchKernelGGL(k_stage<ST_SECOND>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
                          p->chi_floor, s, r->d_flag);
     else if (s < 3)
       hipLaunchKernelGGL(k_stage<ST_MID>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
@@ -409,9 +441,9 @@ extern "C" int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* 
   return flag ? DGR_E_CONSTRAINT : DGR_OK;
 }
 
-// stage state in the padded layout, k and the running sum compact
+// stage state in the padded layout; k, the running sum and k_0 compact
 extern "C" uint64_t dgr_rk4_work_doubles(dgr_plan_t plan) {
-  return plan ? 24ull * plan->dof_end + 48ull * 343ull * plan->ntiles : 0;
+  return plan ? 24ull * plan->dof_end + 72ull * 343ull * plan->ntiles : 0;
 }
 
 extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double* const* u, double* work,
@@ -442,7 +474,7 @@ extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double*
   IOPtrs io[2];
   for (int v = 0; v < 24; ++v) {
     io[0].u.v[v] = r->state.v[v];
-    io[0].r.v[v] = k.v[v];
+    io[0].r.v[v] = r->buf + (size_t)24 * r->dof + (size_t)(48 + v) * r->nacc;  // k_0
     io[1].u.v[v] = stg.v[v];
     io[1].r.v[v] = k.v[v];
   }
