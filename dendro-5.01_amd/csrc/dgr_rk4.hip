@@ -54,19 +54,20 @@ struct Ptrs24 {
 };
 
 // RK4 work memory: one allocation holding the stage state (stg) as 24
-// padded arrays of dof doubles each, then the RHS (k of stages 1-3), the
-// running final sum and k of stage 0 in a compact tile-major layout (343
-// slots per tile in block order, the layout the RHS kernels write in their
-// gather mode); all addressed arithmetically so that only the caller's state
-// needs a pointer table.  k_0 has its own array so that stage 0 need not
-// store the running sum: stage 1 forms it from k_0 (same operations).
+// padded arrays of dof doubles each, then in a compact tile-major layout
+// (343 slots per tile in block order, the layout the RHS kernels write in
+// their gather mode) k_1, the running final sum, k_0 (later k_3) and k_2;
+// all addressed arithmetically so that only the caller's state needs a
+// pointer table.  With k_0 and k_2 kept, stages 0 and 2 store no running
+// sum: stage 1 forms it from k_0 and the last stage adds k_2 before k_3,
+// the same operations in the same order.
 struct Work {
   double* base;   // stage state (24 x dof, padded layout)
   uint64_t dof;
   double* kc;     // k of this stage, compact: 24 x nacc
   double* acc;    // running sum, compact: 24 x nacc
   uint64_t nacc;
-  double* k0;     // k of stage 0, compact: 24 x nacc
+  double* kx;     // stage 1: k_0; last stage: k_2 (compact)
 };
 
 // Per-thread cursor over the 24 variables of one point in the work memory.
@@ -96,7 +97,7 @@ struct AccCursor {
   }
 };
 
-enum StageKind { ST_FIRST = 0, ST_SECOND = 1, ST_MID = 2, ST_LAST = 3 };
+enum StageKind { ST_FIRST = 0, ST_SECOND = 1, ST_THIRD = 2, ST_LAST = 3 };
 
 // enforce_bssn_constraints on one point (bssn_constraints.cpp:12-172):
 // rescale gt to unit determinant, remove the trace of At, floor chi and
@@ -168,8 +169,9 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
     const double dt = gdt[B.group];
     const uint64_t co = (uint64_t)w * 343 + threadIdx.x;
     double un[24], kk[24];
-    if (KIND == ST_FIRST) {
-      // the running sum u_prev + (RK4_C[0] dt) k_0 is formed in stage 1
+    if (KIND == ST_FIRST || KIND == ST_THIRD) {
+      // no running sum here: stage 1 forms u_prev + (RK4_C[0] dt) k_0, the
+      // last stage adds (RK4_C[2] dt) k_2
       const double cu = c_rkU[stage] * dt;
       AccCursor c(W.kc, co);
 #pragma unroll
@@ -183,26 +185,24 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
 #pragma clang fp contract(off)
         un[v] = un[v] + cu * kk[v];
       }
-    } else if (KIND != ST_LAST) {
+    } else if (KIND == ST_SECOND) {
       const double cu = c_rkU[stage] * dt, cc = c_rkC[stage] * dt;
       double acc[24];
       AccCursor c(W.kc, co);
-      AccCursor a(KIND == ST_SECOND ? W.k0 : W.acc, co);
+      AccCursor a(W.kx, co);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
         un[v] = state.v[v][o];  // u_prev: the state at the start of the step
         kk[v] = c.acc();
-        acc[v] = a.acc();       // stage 1: k_0; later: the running sum
+        acc[v] = a.acc();       // k_0
         c.next(W.nacc);
         a.next(W.nacc);
       }
-      if (KIND == ST_SECOND) {
-        const double c0 = c_rkC[0] * dt;
+      const double c0 = c_rkC[0] * dt;
 #pragma unroll
-        for (int v = 0; v < 24; ++v) {
+      for (int v = 0; v < 24; ++v) {
 #pragma clang fp contract(off)
-          acc[v] = un[v] + c0 * acc[v];
-        }
+        acc[v] = un[v] + c0 * acc[v];
       }
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
@@ -217,19 +217,24 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
         d.next(W.nacc);
       }
     } else {
-      const double cc = c_rkC[3] * dt;
+      const double c2 = c_rkC[2] * dt, cc = c_rkC[3] * dt;
+      double k2[24];
       AccCursor c(W.kc, co);
       AccCursor a(W.acc, co);
+      AccCursor x(W.kx, co);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
         un[v] = a.acc();
         kk[v] = c.acc();
+        k2[v] = x.acc();
         c.next(W.nacc);
         a.next(W.nacc);
+        x.next(W.nacc);
       }
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
 #pragma clang fp contract(off)
+        un[v] = un[v] + c2 * k2[v];
         un[v] = un[v] + cc * kk[v];
       }
     }
@@ -313,7 +318,7 @@ struct dgr_rk4_s {
   double* buf = nullptr;        // stage state (24 x dof), k, running sum, k_0 (24 x 343 per tile each)
   Ptrs24 state{}, stg{}, k{};  // the state and the work arrays in buf
   uint64_t nacc = 0;           // compact running-sum slots per variable (343 per tile)
-  IOPtrs* d_io = nullptr;       // RHS pointer tables: [0] (u -> k), [1] (stage -> k)
+  IOPtrs* d_io = nullptr;       // RHS pointer table of each stage (input -> its k array)
   double* d_gdt = nullptr;      // time step per block group
   std::vector<double> h_gdt;
   bool gdt_set = false;
@@ -331,15 +336,16 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
   dgr_plan_t p = r->plan;
   const int nt = (int)p->ntiles;
   for (int s = 0; s < 4; ++s) {
-    // k_s = RHS(u_s): stage 0 reads the state and writes k_0, later stages
-    // read the stage buffer and write k
-    int rc = launch_rhs(p, r->d_io + (s == 0 ? 0 : 1), st, true);
+    // k_s = RHS(u_s): stage 0 reads the state, later stages the stage
+    // buffer; k_s goes to the array of table s (k_1, k_0 / k_3, k_2)
+    int rc = launch_rhs(p, r->d_io + s, st, true);
     if (rc) return rc;
     // stage combine + projection; the next RHS reads ghosts from the
     // neighbours' interiors (gather), so no refill runs between stages
-    double* const kbuf = r->buf + 24 * r->dof;
-    double* const k0 = r->buf + 24 * r->dof + 48 * r->nacc;
-    const Work W = {r->buf, r->dof, s == 0 ? k0 : kbuf, r->buf + 24 * r->dof + 24 * r->nacc, r->nacc, k0};
+    double* const c0 = r->buf + 24 * r->dof;
+    double* const k1 = c0, *const acc = c0 + 24 * r->nacc, *const k03 = c0 + 48 * r->nacc, *const k2 = c0 + 72 * r->nacc;
+    double* const kc = s == 0 || s == 3 ? k03 : (s == 1 ? k1 : k2);
+    const Work W = {r->buf, r->dof, kc, acc, r->nacc, s == 1 ? k03 : k2};
     const dim3 g(std::max(nt, 1)), b(384);
     if (s == 0)
       hipLaunchKernelGGL(k_stage<ST_FIRST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
@@ -347,8 +353,8 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
     else if (s == 1)
       hipLaunchKernelGGL(k_stage<ST_SECOND>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
                          p->chi_floor, s, r->d_flag);
-    else if (s < 3)
-      hipLaunchKernelGGL(k_stage<ST_MID>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
+    else if (s == 2)
+      hipLaunchKernelGGL(k_stage<ST_THIRD>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
                          p->chi_floor, s, r->d_flag);
     else
       hipLaunchKernelGGL(k_stage<ST_LAST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
@@ -441,9 +447,10 @@ extern "C" int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* 
   return flag ? DGR_E_CONSTRAINT : DGR_OK;
 }
 
-// stage state in the padded layout; k, the running sum and k_0 compact
+// stage state in the padded layout; k_1, the running sum, k_0 / k_3 and
+// k_2 compact
 extern "C" uint64_t dgr_rk4_work_doubles(dgr_plan_t plan) {
-  return plan ? 24ull * plan->dof_end + 72ull * 343ull * plan->ntiles : 0;
+  return plan ? 24ull * plan->dof_end + 96ull * 343ull * plan->ntiles : 0;
 }
 
 extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double* const* u, double* work,
@@ -460,7 +467,7 @@ extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double*
   r->buf = work;
   r->nacc = 343ull * plan->ntiles;
   if ((r->own_buf && hipMalloc(&r->buf, sizeof(double) * dgr_rk4_work_doubles(plan)) != hipSuccess) ||
-      hipMalloc(&r->d_io, sizeof(IOPtrs) * 2) != hipSuccess ||
+      hipMalloc(&r->d_io, sizeof(IOPtrs) * 4) != hipSuccess ||
       hipMalloc(&r->d_gdt, sizeof(double) * plan->ngroups) != hipSuccess || hipMalloc(&r->d_flag, 256) != hipSuccess) {
     destroy(r);
     return DGR_E_NOMEM;
@@ -471,12 +478,19 @@ extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double*
     stg.v[v] = r->buf + (size_t)v * r->dof;
     k.v[v] = r->buf + (size_t)24 * r->dof + (size_t)v * r->nacc;
   }
-  IOPtrs io[2];
+  // RHS of stage s: input (the state, then the stage buffer) -> k_1,
+  // k_0 / k_3 or k_2 (compact arrays after the stage buffer)
+  IOPtrs io[4];
+  double* const kb = r->buf + (size_t)24 * r->dof;
   for (int v = 0; v < 24; ++v) {
     io[0].u.v[v] = r->state.v[v];
-    io[0].r.v[v] = r->buf + (size_t)24 * r->dof + (size_t)(48 + v) * r->nacc;  // k_0
+    io[0].r.v[v] = kb + (size_t)(48 + v) * r->nacc;  // k_0
     io[1].u.v[v] = stg.v[v];
-    io[1].r.v[v] = k.v[v];
+    io[1].r.v[v] = k.v[v];                           // k_1
+    io[2].u.v[v] = stg.v[v];
+    io[2].r.v[v] = kb + (size_t)(72 + v) * r->nacc;  // k_2
+    io[3].u.v[v] = stg.v[v];
+    io[3].r.v[v] = kb + (size_t)(48 + v) * r->nacc;  // k_3 over k_0
   }
   hipError_t e = hipStreamCreateWithFlags(&r->st, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMemcpy(r->d_io, io, sizeof(io), hipMemcpyHostToDevice);
