@@ -1282,32 +1282,59 @@ AxisC host_axis(double h, int sz, unsigned bflag, int a) {
 }
 
 // Same-level neighbour table: blocks of equal size and spacing whose
-// corners are sz-7 points apart along an axis are neighbours.  Corners are
-// compared on the integer lattice pmin/h, spacings by a rounded log2 (equal
-// up to rounding).
+// corners are sz-7 points apart along an axis are neighbours.  Spacings are
+// compared by a rounded log2 (equal up to rounding); corners on the integer
+// lattice (pmin - org)/h of their class (size, spacing, group), org being
+// the corner of the class's first block.  A block whose corner lies off that
+// lattice by more than 1e-6 of a spacing gets no link and is no one's
+// neighbour: such blocks overlap their lattice neighbours, and a ghost copied
+// from one would come from a different place.
 void build_neighbours(const dgr_block* blocks, const std::vector<DevBlock>& hb, uint32_t nblocks,
                       std::vector<int32_t>& nbr) {
-  using Key = std::tuple<long long, long long, long long, uint32_t, uint32_t, uint32_t, long long, long long,
-                         long long, uint32_t>;
+  using Cls = std::tuple<uint32_t, uint32_t, uint32_t, long long, long long, long long, uint32_t>;
+  using Key = std::tuple<Cls, long long, long long, long long>;
   auto lev = [](double h) { return llround(std::log2(h) * 64.0); };
-  auto key = [&](uint32_t b, int dx, int dy, int dz) {
+  auto cls = [&](uint32_t b) {
     const DevBlock& B = hb[b];
-    const int d[3] = {dx, dy, dz};
-    long long K[3];
-    for (int a = 0; a < 3; ++a) K[a] = llround(B.pmin[a] / B.h[a]) + (long long)d[a] * (long long)(B.sz[a] - 7);
-    return Key(K[0], K[1], K[2], B.sz[0], B.sz[1], B.sz[2], lev(B.h[0]), lev(B.h[1]), lev(B.h[2]), B.group);
+    return Cls(B.sz[0], B.sz[1], B.sz[2], lev(B.h[0]), lev(B.h[1]), lev(B.h[2]), B.group);
+  };
+  std::map<Cls, uint32_t> first;
+  for (uint32_t b = 0; b < nblocks; ++b) first.emplace(cls(b), b);
+  // lattice coordinates of block b's corner; false when off the lattice
+  auto lattice = [&](uint32_t b, long long K[3]) {
+    const DevBlock& B = hb[b];
+    const DevBlock& O = hb[first[cls(b)]];
+    for (int a = 0; a < 3; ++a) {
+      const double s = (B.pmin[a] - O.pmin[a]) / B.h[a];
+      K[a] = llround(s);
+      if (!(std::fabs(s - (double)K[a]) < 1e-6)) return false;
+    }
+    return true;
   };
   std::map<Key, int32_t> pos;
-  for (uint32_t b = 0; b < nblocks; ++b) pos.emplace(key(b, 0, 0, 0), (int32_t)b);
+  std::vector<char> on(nblocks, 0);
+  for (uint32_t b = 0; b < nblocks; ++b) {
+    long long K[3];
+    if (!lattice(b, K)) continue;
+    on[b] = 1;
+    pos.emplace(Key(cls(b), K[0], K[1], K[2]), (int32_t)b);
+  }
   nbr.assign((size_t)nblocks * NNBR, -1);
-  for (uint32_t b = 0; b < nblocks; ++b)
+  for (uint32_t b = 0; b < nblocks; ++b) {
+    if (!on[b]) continue;
+    const DevBlock& B = hb[b];
+    long long K[3];
+    lattice(b, K);
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
           if (!dx && !dy && !dz) continue;
-          auto it = pos.find(key(b, dx, dy, dz));
+          const Key k(cls(b), K[0] + (long long)dx * (B.sz[0] - 7), K[1] + (long long)dy * (B.sz[1] - 7),
+                      K[2] + (long long)dz * (B.sz[2] - 7));
+          auto it = pos.find(k);
           if (it != pos.end()) nbr[(size_t)b * NNBR + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] = it->second;
         }
+  }
   (void)blocks;
 }
 
@@ -1563,6 +1590,26 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
     return rc;
   }
   *plan_out = plan;
+  return DGR_OK;
+}
+
+extern "C" int dgr_block_neighbours(const dgr_block* blocks, uint32_t nblocks, int32_t* out) {
+  if (!blocks || !out) return DGR_E_INVALID;
+  std::vector<DevBlock> hb(nblocks);
+  for (uint32_t b = 0; b < nblocks; ++b) {
+    const dgr_block& bl = blocks[b];
+    DevBlock& db = hb[b];
+    for (int d = 0; d < 3; ++d) {
+      if (bl.sz[d] < 2) return DGR_E_INVALID;
+      db.sz[d] = bl.sz[d];
+      db.pmin[d] = bl.pmin[d];
+      db.h[d] = (bl.pmax[d] - bl.pmin[d]) / (bl.sz[d] - 1);
+    }
+    db.group = bl.group;
+  }
+  std::vector<int32_t> nbr;
+  build_neighbours(blocks, hb, nblocks, nbr);
+  std::memcpy(out, nbr.data(), sizeof(int32_t) * nbr.size());
   return DGR_OK;
 }
 

@@ -148,6 +148,7 @@ def lib():
         L.dgr_rk4_steps.argtypes = [vp, ctypes.c_double, ctypes.c_int, vp]
         L.dgr_rk4_status.argtypes = [vp]
         L.dgr_rk4_destroy.argtypes = [vp]
+        L.dgr_block_neighbours.argtypes = [ctypes.POINTER(Block), ctypes.c_uint32, vp]
         _lib = L
     return _lib
 
@@ -215,6 +216,17 @@ def blocks_array(blocks):
         a["group"] = [int(b.get("group", 0)) for b in blocks]
     arr = (Block * len(a)).from_buffer_copy(a.tobytes())
     return arr
+
+
+def block_neighbours(blocks):
+    """Same-level neighbour table of a block list (dgr_block_neighbours,
+    host only): int32 array (nblocks, 27), entry (dx+1) + 3(dy+1) + 9(dz+1)
+    the neighbour's index or -1."""
+    import numpy as np
+    arr = blocks_array(blocks)
+    out = np.empty((len(arr), 27), dtype=np.int32)
+    check(lib().dgr_block_neighbours(arr, len(arr), out.ctypes.data_as(ctypes.c_void_p)), "dgr_block_neighbours")
+    return out
 
 
 def _stream_ptr(stream):

@@ -158,6 +158,16 @@ int dgr_ghost_refill(dgr_plan_t plan, double* const* u, void* stream);
  * recomputes these from the mesh on every call). */
 int dgr_plan_level_fills(dgr_plan_t plan, uint32_t* boxes, uint64_t* copies);
 
+/* Same-level neighbour table the ghost refill and the gathering RK4 stages
+ * use, computed on the host (no device call): out[b*27 + (dx+1) + 3(dy+1) +
+ * 9(dz+1)] = index of the block whose corner is sz-7 points away along each
+ * axis with d != 0 (equal size, spacing and group), or -1.  Corners are
+ * matched on the lattice of their class; a block off that lattice by more
+ * than 1e-6 of a spacing has no neighbours and is no one's neighbour.  No
+ * counterpart in the reference, whose unzip finds neighbours through the
+ * mesh's element-to-element map (Mesh::unzip, include/mesh.tcc:7436). */
+int dgr_block_neighbours(const dgr_block* blocks, uint32_t nblocks, int32_t* out);
+
 /* enforce_bssn_constraints (BSSN_GR/src/bssn_constraints.cpp:12-172) on
  * every interior point: det(gt) = 1, tr(At) = 0, chi and alpha floored at
  * params.chi_floor.  Returns DGR_E_CONSTRAINT where the reference exits

@@ -135,6 +135,52 @@ def test_kreiss_oliger_rows(axis):
         assert abs(mom[3] + m3) < 1e-14
 
 
+def sbp_dissipation(n):
+    """-(D3^T D3) on a half line of n points: D3 is the undivided third
+    difference (rows [-1, 3, -3, 1]), the construction of the narrow-stencil
+    artificial dissipation of Mattsson, Svard and Nordstrom (J. Sci. Comput.
+    21, 2004).  Its interior rows are minus the sixth difference, its first
+    three rows the boundary-modified ones."""
+    D3 = np.zeros((n - 3, n))
+    for i in range(n - 3):
+        D3[i, i:i + 4] = [-1.0, 3.0, -3.0, 1.0]
+    return -(D3.T @ D3)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_kreiss_oliger_rows_by_construction(axis):
+    """Every ko_deriv42 row (BSSN_GR/src/derivs.cpp:914-1077) equals a row of
+    -(D3^T D3) built independently above: interior rows divided by 64 h, the
+    three closure rows next to a flagged face divided by the weights
+    smr3 / smr2 / smr1 (= 59/48, 43/48, 49/48 x 64 h, derivs.cpp:918-925),
+    and the far-face rows the mirror images.  This fixes every coefficient,
+    including the ones the moment conditions leave free in the 5- and 6-point
+    closure rows."""
+    n = E - 3  # the line of interior points [3, E)
+    M = sbp_dissipation(n)
+    for q in range(6, E - 3):
+        want = np.zeros(SZ)
+        want[3:E] = M[q - 3] / 64.0
+        np.testing.assert_allclose(coeffs(KO, axis, q), want, rtol=1e-15, atol=0)
+    bf = LOW[axis] | HIGH[axis]
+    for r, smr in enumerate((SMR3, SMR2, SMR1)):
+        want = np.zeros(SZ)
+        want[3:E] = M[r] / smr
+        np.testing.assert_allclose(coeffs(KO, axis, 3 + r, bf), want, rtol=1e-15, atol=0)
+        mirror = np.zeros(SZ)
+        mirror[3:E] = (M[r] / smr)[::-1]
+        np.testing.assert_allclose(coeffs(KO, axis, E - 1 - r, bf), mirror, rtol=1e-15, atol=0)
+    # the construction is symmetric and annihilates constants through
+    # quadratics on every row, as a dissipation operator must
+    assert np.array_equal(M, M.T)
+    x = np.arange(n, dtype=float)
+    for k in range(3):
+        np.testing.assert_allclose(M @ x ** k, 0.0, atol=1e-9)
+    # a flag on one face only: the other side keeps interior rows
+    np.testing.assert_allclose(coeffs(KO, axis, E - 1, LOW[axis])[E - 4:E + 3],
+                               np.array([-1, 6, -15, 20, -15, 6, -1]) / -64.0, rtol=1e-15, atol=0)
+
+
 @pytest.mark.parametrize("axis", [0, 1, 2])
 def test_sixth_order_first_rows(axis):
     """deriv64_x/y/z (QuadGrav/src/derivs.cpp:2141-2420): 6th-order centred
