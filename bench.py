@@ -5,9 +5,10 @@ inputs resident in HBM): 4096 blocks of 13^3 points (one order-6 element
 each, 16x16x16 blocks tiling a cube, outer layer with boundary flags),
 gauge-wave data (A = 0.01) plus 1e-6 seeded white noise on all 24 variables,
 CFL 0.25, KO sigma 0.4, eta 2, 1+log lapse, Gamma-driver shift.  A step is
-four RHS evaluations, four stage combines with the constraint projection and
-four same-level ghost refills; `value` counts RHS interior-point evaluations
-per second (4 x interior points per step).
+four RHS evaluations (each reads the ghosts that have a same-level neighbour
+from that neighbour's interior, which is the refill between stages) and four
+stage combines with the constraint projection; `value` counts RHS
+interior-point evaluations per second (4 x interior points per step).
 
 Also reported on the same line:
   rhs            the batched single-launch RHS kernel alone (HIP events on
@@ -26,7 +27,12 @@ Also reported on the same line:
   serving        a bounded Poisson stream (20 jobs/s) of 50-step evolutions
                  of 64..4096 blocks under a 200 GB HBM budget: p50/p99 job
                  latency and aggregate RHS points/s;
-  cpu_baseline   the CPU oracle on a bounded sample of the headline blocks.
+  cpu_baseline   the CPU oracle timed on the host cores (all threads and one
+                 thread, median of 5) on the headline grid;
+  parity         per configuration (C0 64 blocks, C1 the headline grid, C2
+                 every 10th uneven block, C3 every 40th adaptive block):
+                 max over variables of max|dRHS| / max|RHS| against the
+                 oracle (computed in the CPU leg, the only leg that runs it).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -62,38 +68,131 @@ def measured_traffic():
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(blocks, U_cpu, seconds=8.0, nblk=512):
-    """CPU oracle (liboracle_fast.so: -O3 -march=native, FP contraction on)
-    with an OpenMP parallel-for over blocks on the host cores, on the first
-    `nblk` blocks of the workload, repeated for about `seconds`."""
+def host_cpu():
+    """CPU model, logical and physical core counts of this host."""
+    model, phys = None, set()
+    try:
+        cur = {}
+        for line in open("/proc/cpuinfo"):
+            if ":" not in line:
+                if cur:
+                    phys.add((cur.get("physical id"), cur.get("core id")))
+                cur = {}
+                continue
+            k, v = [x.strip() for x in line.split(":", 1)]
+            cur[k] = v
+            if k == "model name" and model is None:
+                model = v
+        if cur:
+            phys.add((cur.get("physical id"), cur.get("core id")))
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"model": model, "logical_cpus_available": avail, "physical_cores": len(phys) or None}
+
+
+def compact_blocks(U, blocks, ids):
+    """Variables of blocks `ids` of a (24, dof) host array packed back to
+    back, with the block list rewritten to the new offsets."""
+    import numpy as np
+    sub, cols, off = [], [], 0
+    for b in ids:
+        B = dict(blocks[b])
+        n = B["sz"][0] * B["sz"][1] * B["sz"][2]
+        cols.append(U[:, B["offset"]:B["offset"] + n])
+        B["offset"] = off
+        sub.append(B)
+        off += n
+    return sub, np.ascontiguousarray(np.concatenate(cols, axis=1))
+
+
+def sample_on_host(U, R, blocks, ids):
+    """(blocks, U, R) of blocks `ids` packed back to back on the host, cut
+    from the device arrays U (input) and R (GPU RHS)."""
+    import torch
+    sub, cu, cr, off = [], [], [], 0
+    for b in ids:
+        B = dict(blocks[b])
+        n = B["sz"][0] * B["sz"][1] * B["sz"][2]
+        cu.append(U[:, B["offset"]:B["offset"] + n])
+        cr.append(R[:, B["offset"]:B["offset"] + n])
+        B["offset"] = off
+        sub.append(B)
+        off += n
+    return sub, torch.cat(cu, 1).cpu().numpy(), torch.cat(cr, 1).cpu().numpy()
+
+
+def config0(prm):
+    """GPU RHS of the CPU-runnable case: 64 blocks of 13^3 (4x4x4, outer
+    faces flagged), flat space + 8 Gaussian bumps per variable (seed 0)."""
+    import torch
+    from pydgr import Plan, data, layout
+    blocks, dof = layout.uniform_grid(4, dx=1.0 / 64, outer_bflags=True)
+    U = data.flat_bumps(blocks, dof, seed=0, amp=1e-3, width=0.1, device="cuda:0")
+    R = torch.zeros_like(U)
+    plan = Plan(blocks, prm)
+    plan.rhs(R, U)
+    torch.cuda.synchronize()
+    plan.close()
+    return blocks, U.cpu().numpy(), R.cpu().numpy()
+
+
+def cpu_leg(blocks, U_cpu, checks, reps=5, nblk_1t=256):
+    """The CPU oracle on the GPU box's host: (1) the timed CPU baseline, the
+    restatement built -O3 -march=native with FP contraction on
+    (liboracle_fast.so), an OpenMP parallel-for over blocks as rhsTestBed
+    does, on every block of the headline grid with all available host
+    threads and on its first `nblk_1t` blocks with one thread; one warm-up,
+    then the median of `reps` runs each; (2) the parity check: the oracle
+    built without contraction (liboracle.so) on the inputs of every
+    configuration, against the GPU results in `checks` (name -> (blocks, U,
+    R_gpu) on the host), max over variables of max|dRHS| / max|RHS|."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle as po
     from pydgr import layout
 
-    sub = [dict(b) for b in blocks[:nblk]]
-    dof = layout.padded_points(sub)
-    U = np.ascontiguousarray(U_cpu[:, :dof].numpy())
-    out = np.zeros_like(U)
-    pts = layout.interior_points(sub)
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count()
-    cores = int(os.environ.get("OMP_NUM_THREADS", cores))
+    cpu = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", cpu["logical_cpus_available"]))
     prm = po.params(ko_sigma=0.4)
-    po.rhs_blocks(U, sub, prm, nthreads=cores, fast=True, out=out)  # warm-up
-    reps = 0
-    t0 = time.perf_counter()
-    while True:
-        po.rhs_blocks(U, sub, prm, nthreads=cores, fast=True, out=out)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    return {"value": reps * pts / el, "unit": "points/s", "cores": cores, "kind": "port",
-            "sample": "%d blocks of 13^3 (%d interior points) x %d repetitions, %.1f s, OpenMP dynamic over blocks, "
-                      "-O3 -march=native -ffp-contract=fast" % (nblk, pts, reps, el)}
+
+    def timed(sub, U, nthreads):
+        out = np.zeros_like(U)
+        po.rhs_blocks(U, sub, prm, nthreads=nthreads, fast=True, out=out)  # warm-up
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            po.rhs_blocks(U, sub, prm, nthreads=nthreads, fast=True, out=out)
+            ts.append(time.perf_counter() - t0)
+        return layout.interior_points(sub) / float(np.median(ts)), ts
+
+    U = U_cpu
+    v_all, ts_all = timed(blocks, U, threads)
+    sub1, U1 = compact_blocks(U, blocks, range(nblk_1t))
+    v_one, ts_one = timed(sub1, U1, 1)
+    base = {"value": v_all, "unit": "points/s", "cores": threads, "kind": "port",
+            "sample": "all %d blocks of 13^3 of the headline grid (%d interior points), one RHS per run, median of %d "
+                      "runs after one warm-up (%.3f s median); OpenMP parallel for, dynamic schedule over blocks; "
+                      "-O3 -march=native -ffp-contract=fast (FP contraction on)"
+                      % (len(blocks), layout.interior_points(blocks), reps, float(np.median(ts_all))),
+            "one_thread": {"value": v_one, "unit": "points/s", "blocks": nblk_1t,
+                           "median_s": float(np.median(ts_one)), "runs": reps},
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"), "cpu": cpu}
+    parity = {"metric": "max over variables of max|RHS_gpu - RHS_oracle| / max|RHS_oracle| over interior points",
+              "bar": 1e-11, "oracle": "liboracle.so (-O2, FP contraction off)"}
+    for name, (cb, cu, cr) in checks.items():
+        ref = po.rhs_blocks(cu, cb, prm, fill=0.0)
+        ii = layout.interior_index(cb)
+        d = np.abs(cr[:, ii] - ref[:, ii]).max(1)
+        sc = np.abs(ref[:, ii]).max(1)
+        rel = d / np.where(sc > 0, sc, 1.0)
+        parity[name] = {"max_rel": float(rel.max()), "worst_var": int(rel.argmax()), "blocks": len(cb),
+                        "interior_points": int(len(ii))}
+    parity["pass"] = all(parity[n]["max_rel"] <= 1e-11 for n in checks)
+    return base, parity
 
 
 def log(msg):
@@ -129,12 +228,13 @@ def pcie_inclusive(plan, R, U, stream, pts, reps=3):
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        U.copy_(hu, non_blocking=True)
-        plan.rhs(R, U, stream)
-        hr.copy_(R, non_blocking=True)
-    e1.record(stream)
+    with torch.cuda.stream(stream):  # the copies go on the RHS's stream
+        e0.record(stream)
+        for _ in range(reps):
+            U.copy_(hu, non_blocking=True)
+            plan.rhs(R, U, stream)
+            hr.copy_(R, non_blocking=True)
+        e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
     return {"ms": ms, "points_per_s": pts / (ms * 1e-3), "bytes_over_pcie": 2 * U.numel() * 8}
@@ -184,6 +284,8 @@ def uneven(stream, steps, autotune_on):
     ms = time_rhs(plan, R, U, stream, steps, 2)
     ms_pb = time_rhs(plan, R, U, stream, 1, 0, per_block=True)
     nbytes = algorithmic_bytes(blocks)
+    torch.cuda.synchronize()
+    check = sample_on_host(U, R, blocks, range(0, len(blocks), 10))
     del R
     rk = RK4(plan, U, use_graph=True)
     rk.step(0.25 / 64, 0)  # sets dt for every block
@@ -203,7 +305,7 @@ def uneven(stream, steps, autotune_on):
            "autotune": tune_rep}
     del U
     torch.cuda.empty_cache()
-    return out
+    return out, check
 
 
 def adaptive(stream, steps, autotune_on):
@@ -222,6 +324,8 @@ def adaptive(stream, steps, autotune_on):
     plan = Plan(blocks, prm, schedule=sched)
     ms = time_rhs(plan, R, U, stream, steps, 2)
     nbytes = algorithmic_bytes(blocks)
+    torch.cuda.synchronize()
+    check = sample_on_host(U, R, blocks, range(0, len(blocks), 40))
     del R
     dx_min = min((b["pmax"][0] - b["pmin"][0]) / (b["sz"][0] - 1) for b in blocks)
     rk = RK4(plan, U, use_graph=True)
@@ -234,7 +338,7 @@ def adaptive(stream, steps, autotune_on):
     counts = {}
     for l in lev:
         counts[l] = counts.get(l, 0) + 1
-    return {"workload": "Brill-Lindquist m1=m2=0.5, separation 8, on %d blocks of 13^3 over levels %s (2:1 balanced "
+    return check, {"workload": "Brill-Lindquist m1=m2=0.5, separation 8, on %d blocks of 13^3 over levels %s (2:1 balanced "
                         "octree of [-32,32]^3), chi floor 1e-4" % (len(blocks), counts),
             "interior_points": pts, "rhs_ms": ms, "rhs_points_per_s": pts / (ms * 1e-3),
             "rhs_hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -331,7 +435,7 @@ def main():
     ap.add_argument("--serving-budget-gb", type=float, default=200.0)
     ap.add_argument("--serving-max-points", type=int, default=3_000_000,
                     help="cap on resident interior points (0: memory budget only)")
-    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--cpu-reps", type=int, default=5, help="timed CPU-oracle runs (median reported)")
     ap.add_argument("--no-autotune", action="store_true", help="library default schedule instead of the autotuner's")
     ap.add_argument("--calibration-copy-gib", type=float, default=0.0,
                     help="first copy a float64 tensor of this many GiB three times: a plain copy of known size "
@@ -371,7 +475,13 @@ def main():
     if a.per_block_steps > 0:
         ms_pb = time_rhs(plan, R, U, stream, a.per_block_steps, 1, per_block=True)
         log("per-block RHS %.4f ms" % ms_pb)
-    U_cpu = U.cpu() if not a.no_cpu else None
+    checks = {}
+    U_cpu = None
+    if not a.no_cpu:
+        torch.cuda.synchronize()
+        U_cpu = U.cpu().numpy()
+        checks["C1"] = (blocks, U_cpu, R.cpu().numpy())  # GPU RHS of the whole headline grid
+        checks["C0"] = config0(prm)
     pcie = pcie_inclusive(plan, R, U, stream, pts)
     log("PCIe-inclusive RHS (host arrays staged in and out) %.2f ms" % pcie["ms"])
     del R
@@ -409,7 +519,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded gauge wave + 1e-6 white noise on padded blocks)",
         "config": {"workload": "uniform grid: %d blocks of 13^3 (one order-6 element each), gauge wave, one full RK4 "
-                               "step per step (hipGraph): 4 RHS + 4 stage combines + 4 ghost refills" % len(blocks),
+                               "step per step (hipGraph): 4 RHS (each gathering same-level ghosts from neighbour "
+                               "interiors) + 4 stage combines with the constraint projection; one same-level ghost "
+                               "refill of the state per run() call" % len(blocks),
                    "blocks": len(blocks), "block_size": 13, "interior_points": pts, "ko_sigma": 0.4,
                    "cfl": 0.25, "dx": dx, "deriv": "deriv42"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -428,16 +540,19 @@ def main():
         "extras": ext,
     }
     if not a.no_uneven:
-        line["uneven"] = uneven(stream, a.steps, not a.no_autotune)
+        line["uneven"], checks["C2"] = uneven(stream, a.steps, not a.no_autotune)
         log("uneven: RHS %.4f ms, RK4 %.4f ms" % (line["uneven"]["rhs_ms"], line["uneven"]["rk4_ms_per_step"]))
     if not a.no_adaptive:
-        line["adaptive"] = adaptive(stream, a.steps, not a.no_autotune)
+        checks["C3"], line["adaptive"] = adaptive(stream, a.steps, not a.no_autotune)
         log("adaptive: RHS %.4f ms, RK4 %.4f ms" % (line["adaptive"]["rhs_ms"], line["adaptive"]["rk4_ms_per_step"]))
     if a.serving_jobs > 0:
         line["serving"] = serving(a.serving_jobs, a.serving_budget_gb, a.serving_max_points or None)
         log("serving: p50 %.3f s p99 %.3f s" % (line["serving"]["p50_latency_s"], line["serving"]["p99_latency_s"]))
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(blocks, U_cpu, seconds=a.cpu_seconds)
+        line["cpu_baseline"], line["parity"] = cpu_leg(blocks, U_cpu, checks, reps=a.cpu_reps)
+        log("CPU oracle %.3g points/s on %d threads (%.3g on one); parity %s" % (
+            line["cpu_baseline"]["value"], line["cpu_baseline"]["cores"], line["cpu_baseline"]["one_thread"]["value"],
+            {k: "%.2e" % v["max_rel"] for k, v in line["parity"].items() if isinstance(v, dict)}))
     print(json.dumps(line), flush=True)
 
 

@@ -1226,6 +1226,7 @@ void free_plan(dgr_plan_s* plan) {
   (void)hipFree(plan->d_norm);
   (void)hipFree(plan->d_tiles_sub);
   (void)hipFree(plan->d_io_blk);
+  (void)hipFree(plan->d_flag);
   if (plan->ev_fork) (void)hipEventDestroy(plan->ev_fork);
   if (plan->ev_join) (void)hipEventDestroy(plan->ev_join);
   if (plan->aux) (void)hipStreamDestroy(plan->aux);
@@ -1561,7 +1562,8 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
       hipMalloc(&plan->d_tiles_blk, sizeof(DevTile) * ht.size()) != hipSuccess ||
       hipMalloc(&plan->d_tile_slot, sizeof(uint32_t) * std::max<size_t>(1, hslot.size())) != hipSuccess ||
       hipMalloc(&plan->d_io, sizeof(IOPtrs)) != hipSuccess ||
-      hipMalloc(&plan->d_nbr, sizeof(int32_t) * plan->nbr.size()) != hipSuccess) {
+      hipMalloc(&plan->d_nbr, sizeof(int32_t) * plan->nbr.size()) != hipSuccess ||
+      hipMalloc(&plan->d_flag, 256) != hipSuccess) {
     free_plan(plan);
     return DGR_E_NOMEM;
   }

@@ -139,6 +139,8 @@ struct dgr_plan_s {
   // the interior kernel's tail leaves idle (default priority)
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // flag word of dgr_enforce_constraints (allocated with the plan)
+  unsigned* d_flag = nullptr;
 };
 
 namespace dgr_internal {

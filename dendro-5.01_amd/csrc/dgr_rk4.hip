@@ -394,12 +394,6 @@ int check_ptrs(double* const* u) {
   return DGR_OK;
 }
 
-// Device flag word of a one-off kernel call, freed on return.
-struct TmpFlag {
-  unsigned* d = nullptr;
-  ~TmpFlag() { (void)hipFree(d); }
-};
-
 void destroy(dgr_rk4_s* r) {
   if (r->exec) (void)hipGraphExecDestroy(r->exec);
   if (r->graph) (void)hipGraphDestroy(r->graph);
@@ -433,9 +427,9 @@ extern "C" int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* 
   hipStream_t st = (hipStream_t)stream;
   Ptrs24 h;
   for (int v = 0; v < 24; ++v) h.v[v] = u[v];
-  TmpFlag t;
-  DGR_HIP(hipMalloc(&t.d, 256));
-  unsigned* d_flag = t.d;
+  // the plan's flag word: no allocation per call (calls on one plan are
+  // ordered by their synchronisation below)
+  unsigned* d_flag = plan->d_flag;
   DGR_HIP(hipMemsetAsync(d_flag, 0, sizeof(unsigned), st));
   const int nt = (int)plan->ntiles;
   hipLaunchKernelGGL(k_enforce, dim3(grid_for(plan, nt)), dim3(384), 0, st, plan->d_blocks, plan->d_tiles_blk, nt, h,

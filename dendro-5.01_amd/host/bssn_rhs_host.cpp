@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -53,12 +55,61 @@ bool is_device(const void* p) {
   return a.type == hipMemoryTypeDevice;
 }
 
+// The last block list's plan and the host-path staging buffers, kept
+// between calls: rkBSSN calls bssnRHS four times per step on one block list
+// (BSSN_GR/src/rkBSSN.cpp:647-783), so only the first call of a list pays
+// for the plan (block and tile tables, neighbour search, a stream) and the
+// device buffers.  A different block list, parameter set or device
+// rebuilds the plan; staging buffers only grow.  Calls are serialised by a
+// mutex (the reference's bssnrhs is called from OpenMP threads,
+// BSSN_GR/src/rhsTestBed.cpp:181).
+struct ShimCache {
+  std::mutex m;
+  std::vector<dgr_block> blocks;
+  dgr_bssn_params p{};
+  int device = -1;
+  dgr_plan_t plan = nullptr;
+  double* du = nullptr;
+  double* dr = nullptr;
+  size_t cap = 0;  // doubles per variable in du / dr
+  ~ShimCache() {
+    // the HIP runtime may already be gone at process exit: no calls here
+  }
+};
+
+ShimCache& cache() {
+  static ShimCache c;
+  return c;
+}
+
+dgr_plan_t cached_plan(ShimCache& c, const std::vector<dgr_block>& blocks, const dgr_bssn_params& p) {
+  int dev = 0;
+  hipcheck(hipGetDevice(&dev), "hipGetDevice");
+  const bool same = c.plan && dev == c.device && blocks.size() == c.blocks.size() &&
+                    std::memcmp(blocks.data(), c.blocks.data(), sizeof(dgr_block) * blocks.size()) == 0 &&
+                    std::memcmp(&p, &c.p, sizeof(p)) == 0;
+  if (same) return c.plan;
+  if (c.plan) {
+    dgr_plan_destroy(c.plan);
+    c.plan = nullptr;
+  }
+  check(dgr_plan_create(&c.plan, blocks.data(), (uint32_t)blocks.size(), &p), "bssnRHS");
+  c.blocks = blocks;
+  c.p = p;
+  c.device = dev;
+  return c.plan;
+}
+
 void run(double** rhs, const double** u, const std::vector<dgr_block>& blocks) {
   const dgr_bssn_params p = to_params(dgr::bssn_params());
   bool dev = true;
   for (int v = 0; v < DGR_NUM_VARS; ++v) dev = dev && is_device(u[v]) && is_device(rhs[v]);
+  ShimCache& c = cache();
+  std::lock_guard<std::mutex> lock(c.m);
   if (dev) {
-    check(dgr_bssn_rhs_blocks(rhs, u, blocks.data(), (uint32_t)blocks.size(), &p, nullptr), "bssnRHS");
+    dgr_plan_t plan = cached_plan(c, blocks, p);
+    check(dgr_bssn_rhs(plan, rhs, u, nullptr), "bssnRHS");
+    hipcheck(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
     return;
   }
   // host arrays: stage the DOF range the blocks cover through device memory
@@ -70,32 +121,27 @@ void run(double** rhs, const double** u, const std::vector<dgr_block>& blocks) {
   const size_t n = (size_t)(hi - lo);
   std::vector<dgr_block> shifted(blocks);
   for (auto& b : shifted) b.offset -= lo;
-  double* du = nullptr;
-  double* dr = nullptr;
-  hipcheck(hipMalloc(&du, sizeof(double) * n * DGR_NUM_VARS), "hipMalloc");
-  if (hipMalloc(&dr, sizeof(double) * n * DGR_NUM_VARS) != hipSuccess) {
-    (void)hipFree(du);
-    throw std::runtime_error("bssnRHS: hipMalloc failed");
+  dgr_plan_t plan = cached_plan(c, shifted, p);
+  if (n > c.cap) {
+    (void)hipFree(c.du);
+    (void)hipFree(c.dr);
+    c.du = c.dr = nullptr;
+    c.cap = 0;
+    hipcheck(hipMalloc(&c.du, sizeof(double) * n * DGR_NUM_VARS), "hipMalloc");
+    hipcheck(hipMalloc(&c.dr, sizeof(double) * n * DGR_NUM_VARS), "hipMalloc");
+    c.cap = n;
   }
   const double* up[DGR_NUM_VARS];
   double* rp[DGR_NUM_VARS];
-  try {
-    for (int v = 0; v < DGR_NUM_VARS; ++v) {
-      up[v] = du + (size_t)v * n;
-      rp[v] = dr + (size_t)v * n;
-      hipcheck(hipMemcpy((void*)up[v], u[v] + lo, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
-      hipcheck(hipMemcpy(rp[v], rhs[v] + lo, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
-    }
-    check(dgr_bssn_rhs_blocks(rp, up, shifted.data(), (uint32_t)shifted.size(), &p, nullptr), "bssnRHS");
-    for (int v = 0; v < DGR_NUM_VARS; ++v)
-      hipcheck(hipMemcpy(rhs[v] + lo, rp[v], sizeof(double) * n, hipMemcpyDeviceToHost), "hipMemcpy");
-  } catch (...) {
-    (void)hipFree(du);
-    (void)hipFree(dr);
-    throw;
+  for (int v = 0; v < DGR_NUM_VARS; ++v) {
+    up[v] = c.du + (size_t)v * c.cap;
+    rp[v] = c.dr + (size_t)v * c.cap;
+    hipcheck(hipMemcpy((void*)up[v], u[v] + lo, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
+    hipcheck(hipMemcpy(rp[v], rhs[v] + lo, sizeof(double) * n, hipMemcpyHostToDevice), "hipMemcpy");
   }
-  (void)hipFree(du);
-  (void)hipFree(dr);
+  check(dgr_bssn_rhs(plan, rp, up, nullptr), "bssnRHS");
+  for (int v = 0; v < DGR_NUM_VARS; ++v)
+    hipcheck(hipMemcpy(rhs[v] + lo, rp[v], sizeof(double) * n, hipMemcpyDeviceToHost), "hipMemcpy");
 }
 
 }  // namespace

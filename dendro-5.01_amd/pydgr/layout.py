@@ -63,6 +63,34 @@ def mixed_blocks(nblocks, seed=0, dist=((13, 0.60), (19, 0.25), (25, 0.10), (37,
     return blocks, off
 
 
+def mixed_lattices(dx=1.0 / 64, counts=((13, 3), (19, 2), (25, 2), (37, 1))):
+    """Blocks of uneven padded sizes, each size on a cube lattice of its own
+    (n^3 blocks of that size, neighbours sz-7 points apart) and the lattices
+    side by side along x, from x = 2 dx so that no point of a flagged face
+    sits at the origin, where the Sommerfeld term divides by r: a mixed
+    batch in which every size has same-level neighbours.  Boundary flags on
+    every outer face of each lattice."""
+    blocks = []
+    off = 0
+    x0 = 2 * dx
+    for sz, n in counts:
+        e = (sz - 1 - 2 * GHOST) * dx
+        for bz in range(n):
+            for by in range(n):
+                for bx in range(n):
+                    bf = 0
+                    for d, bc in enumerate((bx, by, bz)):
+                        if bc == 0:
+                            bf |= 1 << (2 * d)
+                        if bc == n - 1:
+                            bf |= 1 << (2 * d + 1)
+                    corner = [x0 + bx * e, -0.5 * n * e + by * e, -0.5 * n * e + bz * e]
+                    blocks.append(block_dict(off, [sz] * 3, bf, corner, dx, [sz - 1 - 2 * GHOST] * 3))
+                    off += sz ** 3
+        x0 += n * e + 8 * dx
+    return blocks, off
+
+
 def interior_points(blocks):
     return int(sum((b["sz"][0] - 6) * (b["sz"][1] - 6) * (b["sz"][2] - 6) for b in blocks))
 

@@ -204,11 +204,51 @@ def ghost_refill_uniform(U, nb, sz=13):
     return U
 
 
-def rk4_step_uniform(U, blocks, nb, dt, prm=None, chi_floor=0.1, nthreads=0):
-    """One RK4 step of rkBSSN.cpp:803-1011 on unzipped uniform-grid blocks:
+def ghost_refill_blocks(U, blocks):
+    """Same-level ghost refill on any block list, found from the geometry
+    alone: a ghost box of block b (direction d in {-1,0,1}^3 \ 0) takes the
+    interior values of the block of equal size, spacing and group whose
+    interior corner lies (sz-7) spacings away along every axis with d != 0
+    (the copy Mesh::unzip makes between same-level neighbours,
+    include/mesh.tcc:7436).  Boxes without such a neighbour are left as they
+    are.  Blocks must be on a common lattice per size (no partial overlap)."""
+    key = {}
+    for n, b in enumerate(blocks):
+        h = (b["pmax"][0] - b["pmin"][0]) / (b["sz"][0] - 1)
+        c = tuple(int(round(b["pmin"][a] / h)) for a in range(3))
+        key[(tuple(b["sz"]), round(h, 15), b.get("group", 0)) + c] = n
+    for n, b in enumerate(blocks):
+        s0, s1, s2 = b["sz"]
+        h = (b["pmax"][0] - b["pmin"][0]) / (s0 - 1)
+        c = [int(round(b["pmin"][a] / h)) for a in range(3)]
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    if dx == dy == dz == 0:
+                        continue
+                    d = (dx, dy, dz)
+                    k = (tuple(b["sz"]), round(h, 15), b.get("group", 0)) + tuple(
+                        c[a] + d[a] * (b["sz"][a] - 7) for a in range(3))
+                    m = key.get(k)
+                    if m is None:
+                        continue
+                    src = blocks[m]
+                    V = U[:, b["offset"]:b["offset"] + s0 * s1 * s2].reshape(U.shape[0], s2, s1, s0)
+                    W = U[:, src["offset"]:src["offset"] + s0 * s1 * s2].reshape(U.shape[0], s2, s1, s0)
+                    rng = []
+                    for a, sa in zip((dz, dy, dx), (s2, s1, s0)):
+                        lo, hi = {-1: (0, 3), 0: (3, sa - 3), 1: (sa - 3, sa)}[a]
+                        rng.append((lo, hi, a * (sa - 7)))
+                    (k0, k1, ek), (j0, j1, ej), (i0, i1, ei) = rng
+                    V[:, k0:k1, j0:j1, i0:i1] = W[:, k0 - ek:k1 - ek, j0 - ej:j1 - ej, i0 - ei:i1 - ei]
+    return U
+
+
+def rk4_step_blocks(U, blocks, dt, refill, prm=None, chi_floor=0.1, nthreads=0):
+    """One RK4 step of rkBSSN.cpp:803-1011 on unzipped blocks:
     stage combines u_prev + (U[s+1]*dt)*k_s with the constraint projection,
-    the final u_prev + sum_s (C[s]*dt)*k_s in order s = 0..3, and a
-    same-level ghost refill after every stage.  Returns (u_new, bad)."""
+    the final u_prev + sum_s (C[s]*dt)*k_s in order s = 0..3, and the ghost
+    refill `refill(U)` after every stage.  Returns (u_new, bad)."""
     from pydgr import layout  # noqa: WPS433 (test infrastructure)
     Uc = [0.0, 0.5, 0.5, 1.0]
     Cc = [1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0]
@@ -231,11 +271,16 @@ def rk4_step_uniform(U, blocks, nb, dt, prm=None, chi_floor=0.1, nthreads=0):
             un, b = enforce_constraints(np.ascontiguousarray(un), chi_floor)
             bad |= b
             stage[:, ii] = un
-            ghost_refill_uniform(stage, nb)
+            refill(stage)
             cur = stage
     unew = uprev.copy()
     fin, b = enforce_constraints(np.ascontiguousarray(acc), chi_floor)
     bad |= b
     unew[:, ii] = fin
-    ghost_refill_uniform(unew, nb)
+    refill(unew)
     return unew, bad
+
+
+def rk4_step_uniform(U, blocks, nb, dt, prm=None, chi_floor=0.1, nthreads=0):
+    """rk4_step_blocks on the nb^3 uniform grid of pydgr.layout.uniform_grid."""
+    return rk4_step_blocks(U, blocks, dt, lambda A: ghost_refill_uniform(A, nb), prm, chi_floor, nthreads)

@@ -40,3 +40,23 @@ def test_rk4_step_flat_space_is_static():
     U2, bad = po.rk4_step_uniform(U, blocks, nb, dt=0.25 / 32)
     assert bad == 0
     assert np.array_equal(U2, U)
+
+
+def test_geometric_refill_equals_uniform_refill():
+    """The oracle's geometric same-level refill (any block list) gives the
+    uniform-grid refill bitwise, and copies only between equal sizes."""
+    from pydgr import data, layout
+    blocks, dof = layout.uniform_grid(3, dx=1.0 / 32)
+    U = data.flat_bumps(blocks, dof, seed=1, amp=0.05, width=0.3).numpy()
+    ii = layout.interior_index(blocks)
+    g = np.ones(dof, bool)
+    g[ii] = False
+    W = U.copy()
+    W[:, g] = -1.0
+    assert np.array_equal(po.ghost_refill_uniform(W.copy(), 3), po.ghost_refill_blocks(W.copy(), blocks))
+    mb, mdof = layout.mixed_lattices(dx=1.0 / 64)
+    V = np.random.default_rng(0).standard_normal((2, mdof))
+    mi = layout.interior_index(mb)
+    V2 = po.ghost_refill_blocks(V.copy(), mb)
+    assert np.array_equal(V2[:, mi], V[:, mi])
+    assert not np.array_equal(V2, V)
