@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -282,6 +283,14 @@ __device__ __forceinline__ double point_coords(const double* geo, int gi, int gj
   return sqrt(x * x + y * y + z * z);
 }
 
+// (ETA_R0 / r)^ETA_DAMPING_EXP of the damped eta (rhs.cpp:219-223).  The
+// default exponent 1 needs no pow (pow(x, 1) is x exactly); any other goes
+// through an out-of-line pow, so that its polynomial constants are not
+// hoisted into registers across the tile loop (where they were spilled and
+// reloaded behind the staging loads).
+__device__ __noinline__ double eta_pow(double x, double e) { return pow(x, e); }
+__device__ __forceinline__ double eta_damping(double x, double e) { return e == 1.0 ? x : eta_pow(x, e); }
+
 // rhs = eq + (KO), or (Sommerfeld + KO) on flagged faces (rhs.cpp:265-373)
 __device__ __forceinline__ void emit(const IOPtrs* __restrict__ io, const Ctx& C, int v, double eq, double ex) {
 #pragma clang fp contract(off)
@@ -289,16 +298,15 @@ __device__ __forceinline__ void emit(const IOPtrs* __restrict__ io, const Ctx& C
   __builtin_nontemporal_store(r, gmem(io->r.v[v]) + C.oo);
 }
 // Outputs that become final several variables later park their dissipation
-// (+ Sommerfeld) term in the output array and pick it up at emission.
+// (+ Sommerfeld) term in the output array; the variable that completes them
+// loads the parked terms back (process_var).
 __device__ __forceinline__ void park(const IOPtrs* __restrict__ io, const Ctx& C, int v, double ex) {
   gmem(io->r.v[v])[C.oo] = ex;
 }
-__device__ __forceinline__ void emit_parked(const IOPtrs* __restrict__ io, const Ctx& C, int v, double eq) {
-  emit(io, C, v, eq, gmem(io->r.v[v])[C.oo]);
-}
 
 // Per-thread LDS slots: element i of a thread's array sits at p[i * NPT]
-// (one slot column per tile point).
+// (one slot column per tile point).  The constraint monitor keeps each
+// array in slots of its own:
 constexpr int NSLOT = 48;  // C2 18, R 6, gt_rhs 6, At_rhs 6, small accumulators 12
 template <int N>
 struct LdsArr {
@@ -311,6 +319,32 @@ struct LdsStore {
   using A = LdsArr<N>;
 };
 using KPoint = dgr::PointT<LdsStore>;
+
+// The right-hand side packs the arrays by lifetime into 41 slots, which
+// leaves room for a second staging buffer.  In the processing order below
+// the Christoffels (C2, slots 0-17) die with alpha; the accumulators that
+// start later reuse their slots: the -lambda3 beta.dGt part of B_rhs (from
+// Gt, slots 0-2), d_j d_k beta^k (3-5) and div beta (6), the last two
+// cleared when beta^0 starts.  R 18-23, gt_rhs 24-29, At_rhs 30-35,
+// Gt_rhs 36-38, chi_rhs 39, K_rhs 40.
+constexpr int NSLOT_RHS = 41;
+struct AccArr {  // the 12 small accumulators (PointT::A_*) in their slots
+  double* p;     // slot 0 of the thread
+  static __device__ __forceinline__ constexpr int slot(int i) {
+    return i < 3 ? 36 + i : (i < 10 ? i - 3 : 29 + i);
+  }
+  __device__ __forceinline__ double& operator[](int i) { return p[slot(i) * NPT]; }
+  __device__ __forceinline__ const double& operator[](int i) const { return p[slot(i) * NPT]; }
+};
+struct RhsStore {
+  template <int N>
+  using A = typename std::conditional<N == 12, AccArr, LdsArr<N>>::type;
+};
+using KPointR = dgr::PointT<RhsStore>;
+static_assert(AccArr::slot(KPointR::A_GT) == 36 && AccArr::slot(KPointR::A_B) == 0 &&
+                  AccArr::slot(KPointR::A_WB) == 3 && AccArr::slot(KPointR::A_DIVB) == 6 &&
+                  AccArr::slot(KPointR::A_CHI) == 39 && AccArr::slot(KPointR::A_K) == 40,
+              "accumulator slots");
 
 // All operators along one axis through the point: first derivative (if
 // D1), second derivative (if D2), upwinded derivative and Kreiss-Oliger
@@ -403,11 +437,34 @@ struct Mixed {
 };
 
 template <int V, int FM, int ORD>
-__device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su, const double* sgx,
+__device__ __forceinline__ void process_var(KPointR& s, Ctx& C, const double* su, const double* sgx,
                                             const double* sgy, const Mixed& M, const TileGeom& T,
                                             const IOPtrs* __restrict__ io) {
   using namespace dgr;
   if (!C.act) return;
+  // parked terms of the outputs this variable completes (gt_rhs and
+  // chi_rhs at beta^2, Gt_rhs at K), all loaded here at once: emitted one
+  // by one, each load would wait behind the previous emit's store, since
+  // the compiler cannot tell the output arrays apart
+  constexpr int NPK = V == V_BETA2 ? 7 : (V == V_K ? 3 : 0);
+  double pk[NPK > 0 ? NPK : 1];
+  if constexpr (V == V_BETA2) {
+#pragma unroll
+    for (int p = 0; p < 6; ++p) pk[p] = gmem(io->r.v[V_SG0 + p])[C.oo];
+    pk[6] = gmem(io->r.v[V_CHI])[C.oo];
+  }
+  if constexpr (V == V_K) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) pk[k] = gmem(io->r.v[V_GT0 + k])[C.oo];
+  }
+  // B^k only enters b_rhs^k and B_rhs^k
+  if constexpr (V >= V_BETA0 && V <= V_BETA2) s.B[V - V_BETA0] = gmem(io->u.v[V_B0 + (V - V_BETA0)])[C.o];
+  if constexpr (V == V_BETA0) {
+    // the shift accumulators start here, in slots the Christoffels used
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s.acc[KPointR::A_WB + j] = 0.0;
+    s.acc[KPointR::A_DIVB] = 0.0;
+  }
   const AxisC* ax = C.ax;
   const int ti = C.ti, tj = C.tj, tk = C.tk, gi = C.gi, gj = C.gj, gk = C.gk;
   // su index of this point; lines through it along x, y, z are addressed
@@ -487,19 +544,18 @@ __device__ __forceinline__ void process_var(KPoint& s, Ctx& C, const double* su,
     park(io, C, V, ex);
   }
   if constexpr (V >= V_BETA0 && V <= V_BETA2) {
-    s.B[V - V_BETA0] = gmem(io->u.v[V_B0 + (V - V_BETA0)])[C.o];  // B^k only enters b_rhs^k and B_rhs^k
     double gto[6], chio = 0.0;
     emit(io, C, V, s.add_beta(C.P, V - V_BETA0, d1, d2, adv, gto, &chio), ex);
     if constexpr (V == V_BETA2) {
 #pragma unroll
-      for (int p = 0; p < 6; ++p) emit_parked(io, C, V_SG0 + p, gto[p]);
-      emit_parked(io, C, V_CHI, chio);
+      for (int p = 0; p < 6; ++p) emit(io, C, V_SG0 + p, gto[p], pk[p]);
+      emit(io, C, V_CHI, chio, pk[6]);
     }
   }
   if constexpr (V == V_K) {
     emit(io, C, V_K, s.add_K(d1, adv), ex);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) emit_parked(io, C, V_GT0 + k, s.Gt_out(k));
+    for (int k = 0; k < 3; ++k) emit(io, C, V_GT0 + k, s.Gt_out(k), pk[k]);
   }
   if constexpr (V >= V_B0 && V <= V_B2) {
     s.B[V - V_B0] = su[c];
@@ -595,22 +651,29 @@ __device__ __forceinline__ void plane_y(const double* su, double* sgy, const Pla
   }
 }
 
+// Variables are staged through two LDS buffers: variable I is read from
+// one while every thread writes variable I+1 (loaded into registers during
+// variable I) into the other at the end of its own work on I, so one
+// barrier per variable (plus one after the derivative planes) suffices.
 template <int I, int FM, int ORD, bool GATHER>
-__device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], double* su, double* sgx,
-                                         double* sgy, const TileGeom& T, const IOPtrs* __restrict__ io,
-                                         int t, const Slots& sl, const PlaneSlots<ORD>& ps, uint32_t zmask) {
+__device__ __forceinline__ void run_vars(KPointR& s, Ctx& C, double (&pf)[NPF], double* su0, double* su1,
+                                         double* sgx, double* sgy, const TileGeom& T,
+                                         const IOPtrs* __restrict__ io, int t, const Slots& sl,
+                                         const PlaneSlots<ORD>& ps, uint32_t zmask) {
   if constexpr (I < 24) {
     constexpr int V = h_order[I];
     using G = Planes<ORD>;
     constexpr bool BND = FM != 0, CX = (FM & 1) != 0, CY = (FM & 2) != 0, CZ = (FM & 4) != 0;
-    stage(pf, su, T, t, zmask);
+    double* const su = (I & 1) ? su1 : su0;  // variable I
+    double* const sn = (I & 1) ? su0 : su1;  // variable I+1, written below
+    // variable I is staged by every thread, nobody reads variable I-1 any
+    // more (its buffer takes I+1), the planes of the last variable are read
     __syncthreads();
     Mixed M;
     if constexpr (has_d2(V) && G::SHARED) {
       // d_x plane -> d_x u, d_y d_x u, d_z d_x u; then the d_y plane in the
       // same buffer -> d_y u, d_z d_y u
       plane_x<BND, CX, ORD>(su, sgx, ps, T, C.ax, t);
-      if constexpr (I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
       __syncthreads();
       if (C.act) {
         const int ti = C.ti, tj = C.tj, tk = C.tk;
@@ -626,24 +689,20 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
         M.d1y = sgx[ti + TX * (tj + TY * (tk + G::GW))];
         M.dyz = d1_op<CZ, ORD>(sgx, ti + TX * (tj + TY * (G::GW - T.k0)), TX * TY, C.gk, C.ax[2]);
       }
-    } else {
-      // gathering interior tiles issue the next variable's loads before the
-      // planes (a longer overlap; measured -1.4% per RK4 step; +0.7% on the
-      // plain RHS, and the boundary tiles' spills leave no room for it)
-      if constexpr (GATHER && FM == 0 && I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
-      if constexpr (has_d2(V)) {
-        plane_x<BND, CX, ORD>(su, sgx, ps, T, C.ax, t);
-        plane_y<BND, CY, ORD>(su, sgy, ps, T, C.ax, t);
-      }
-      if constexpr ((!GATHER || FM != 0) && I + 1 < 24) prefetch(pf, io->u.v[h_order[I + 1]], T, sl);
-      if constexpr (has_d2(V)) __syncthreads();
+    } else if constexpr (has_d2(V)) {
+      plane_x<BND, CX, ORD>(su, sgx, ps, T, C.ax, t);
+      plane_y<BND, CY, ORD>(su, sgy, ps, T, C.ax, t);
+      __syncthreads();
     }
     if constexpr (V == dgr::V_CHI) {
       if (C.act) s.finish_metric();
     }
     process_var<V, FM, ORD>(s, C, su, sgx, sgy, M, T, io);
-    __syncthreads();
-    run_vars<I + 1, FM, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, io, t, sl, ps, zmask);
+    if constexpr (I + 1 < 24) {
+      stage(pf, sn, T, t, zmask);
+      if constexpr (I + 2 < 24) prefetch(pf, io->u.v[h_order[I + 2]], T, sl);
+    }
+    run_vars<I + 1, FM, ORD, GATHER>(s, C, pf, su0, su1, sgx, sgy, T, io, t, sl, ps, zmask);
   }
 }
 
@@ -653,24 +712,54 @@ __device__ __forceinline__ void run_vars(KPoint& s, Ctx& C, double (&pf)[NPF], d
 // edge and corner blocks (the bodies are run_vars<0, 1|2|4|7, ...>).
 constexpr int FM_FACES = 8;
 
+// A tile descriptor read as four 32-bit words (one scalar load of uniform
+// data) and unpacked.
+__device__ __forceinline__ DevTile load_tile(const DevTile* p) {
+  static_assert(sizeof(DevTile) == 16, "tile descriptor is four words");
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+  const uint32_t w0 = q[0], w1 = q[1], w2 = q[2], w3 = q[3];
+  DevTile t;
+  t.block = w0;
+  t.i0 = (uint16_t)(w1 & 0xffffu);
+  t.j0 = (uint16_t)(w1 >> 16);
+  t.k0 = (uint16_t)(w2 & 0xffffu);
+  t.nx = (uint8_t)((w2 >> 16) & 0xffu);
+  t.ny = (uint8_t)(w2 >> 24);
+  t.nz = (uint8_t)(w3 & 0xffu);
+  t.pad = (uint8_t)((w3 >> 8) & 0xffu);
+  return t;
+}
+
 template <int FM, int ORD, bool GATHER>
 __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ blocks,
                                                  const DevTile* __restrict__ tiles, int tile_begin, int ntiles,
                                                  const IOPtrs* __restrict__ io, DevParams P,
-                                                 const int32_t* __restrict__ nbr,
+                                                 const long long* __restrict__ delta,
                                                  const uint32_t* __restrict__ slot) {
-  __shared__ double su[NSU];
+  __shared__ double sstage[2][NSU];  // two staging buffers (variables alternate)
+  double* const su0 = sstage[0];
+  double* const su1 = sstage[1];
   __shared__ double splanes[Planes<ORD>::N];
   double* const sgx = splanes;
   double* const sgy = Planes<ORD>::SHARED ? splanes : splanes + Planes<ORD>::NGX;
-  __shared__ double sacc[NSLOT * NPT];
+  __shared__ double sacc[NSLOT_RHS * NPT];
   __shared__ IOPtrs s_io;           // array pointers of this launch
-  __shared__ AxisC s_ax[3];         // stencil constants of the current block
-  __shared__ double s_geo[6];       // pmin[3], h[3] of the current block
+  // pmin[3], h[3] and the per-axis stencil constants of the current block,
+  // copied from the block table as one run of 8-byte words
+  constexpr int NBW = (int)((offsetof(DevBlock, ax) + sizeof(AxisC) * 3 - offsetof(DevBlock, pmin)) / 8);
+  static_assert(offsetof(DevBlock, h) == offsetof(DevBlock, pmin) + 24 &&
+                    offsetof(DevBlock, ax) == offsetof(DevBlock, h) + 24 && sizeof(AxisC) % 8 == 0,
+                "block geometry and axis constants are contiguous");
+  __shared__ unsigned long long s_blk[NBW];
+  double* const s_geo = reinterpret_cast<double*>(s_blk);
+  AxisC* const s_ax = reinterpret_cast<AxisC*>(s_blk + 6);
   __shared__ long long s_delta[GATHER ? NNBR : 1];  // gather mode: source distance per ghost direction
 
   constexpr bool BND = FM != 0;
   const int t = threadIdx.x;
+  // the array pointers are the same for every tile of the launch
+  if (t < 24) s_io.u.v[t] = io->u.v[t];
+  else if (t < 48) s_io.r.v[t - 24] = io->r.v[t - 24];
   if constexpr (BND) {
     // closure rows for the boundary stencils (visible after the first
     // barrier of the tile loop)
@@ -696,8 +785,9 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
   C.P.lambda_f[1] = P.lambda_f[1];
   C.sigma = P.ko_sigma;
 
+  __syncthreads();  // s_io (and the closure rows) visible
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
-    const DevTile tl = tiles[tile_begin + w];
+    const DevTile tl = load_tile(tiles + tile_begin + w);
     const DevBlock& B = blocks[tl.block];
     TileGeom T;
     T.off = B.offset;
@@ -720,58 +810,25 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
     C.act = t < NPT && C.ti < T.nx && C.tj < T.ny && C.tk < T.nz;
     C.gi = T.i0 + C.ti; C.gj = T.j0 + C.tj; C.gk = T.k0 + C.tk;
     C.bflag = B.bflag;
-    if (t < 24) s_io.u.v[t] = io->u.v[t];
-    else if (t < 48) s_io.r.v[t - 24] = io->r.v[t - 24];
-    else if (t < 51) s_ax[t - 48] = B.ax[t - 48];
-    else if (t < 54) s_geo[t - 51] = B.pmin[t - 51];
-    else if (t < 57) s_geo[t - 51] = B.h[t - 54];
+    // one load per thread, all in flight together
+    if (t < NBW) s_blk[t] = reinterpret_cast<const unsigned long long*>(&B.pmin[0])[t];
     C.ax = s_ax;
     C.geo = s_geo;
 
     if constexpr (GATHER) {
       // source distance of each ghost direction: the same point of the
       // same-level neighbour's interior (corners sz-7 apart), or 0
-      if (t < NNBR) {
-        const int n = nbr[(size_t)tl.block * NNBR + t];
-        long long dl = 0;
-        if (n >= 0) {
-          const long long dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-          dl = (long long)blocks[n].offset - (long long)T.off -
-               (dx * (T.sz0 - 7) + (long long)T.sz0 * (dy * (T.sz1 - 7) + (long long)T.sz1 * dz * (T.sz2 - 7)));
-        }
-        s_delta[t] = dl;
-      }
+      // (plan table, computed on the host from the neighbour links)
+      if (t < NNBR) s_delta[t] = delta[(size_t)tl.block * NNBR + t];
       __syncthreads();
     }
     const Slots sl = make_slots<GATHER>(T, t, s_delta);
     double pf[NPF];
     prefetch(pf, io->u.v[h_order[0]], T, sl);
-    __syncthreads();
-
-    // point coordinates, eta (rhs.cpp:214-223) and the Sommerfeld face test
-    KPoint s;
-    // inactive threads (t >= NPT) never touch their slots
-    s.C2.p = sacc + t;
-    s.R.p = sacc + 18 * NPT + t;
-    s.gt_rhs.p = sacc + 24 * NPT + t;
-    s.At_rhs.p = sacc + 30 * NPT + t;
-    s.acc.p = sacc + 36 * NPT + t;
+    KPointR s;
     C.o = T.off + (uint64_t)C.gi + (uint64_t)T.sz0 * (C.gj + (uint64_t)T.sz1 * C.gk);
-    // RK4 stages (gather) write k in the compact tile-major layout of the
-    // stage kernels: slot of the tile in block order x 343 + local index
-    C.oo = GATHER ? (uint64_t)slot[tile_begin + w] * NPT + (uint64_t)(C.ti + TX * (C.tj + TY * C.tk)) : C.o;
-    C.on_face = false;
-    C.eta = P.eta_const;
     if (C.act) {
-#pragma clang fp contract(off)
-      double x, y, z;
-      const double geo[6] = {B.pmin[0], B.pmin[1], B.pmin[2], B.h[0], B.h[1], B.h[2]};
-      const double rr = point_coords(geo, C.gi, C.gj, C.gk, x, y, z);
-      if (rr >= P.eta_r0) C.eta *= pow(P.eta_r0 / rr, P.eta_damping_exp);
-      const unsigned bf = C.bflag;
-      C.on_face = ((bf & 1u) && C.gi == 3) || ((bf & 2u) && C.gi == T.sz0 - 4) || ((bf & 4u) && C.gj == 3) ||
-                  ((bf & 8u) && C.gj == T.sz1 - 4) || ((bf & 16u) && C.gk == 3) ||
-                  ((bf & 32u) && C.gk == T.sz2 - 4);
+      // the state at the point, in flight with the first variable's loads
       const uint64_t o = C.o;
       s.alpha = gmem(s_io.u.v[dgr::V_ALPHA])[o];
       s.chi = gmem(s_io.u.v[dgr::V_CHI])[o];
@@ -783,23 +840,53 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
         s.gt[p] = gmem(s_io.u.v[dgr::V_SG0 + p])[o];
         s.At[p] = gmem(s_io.u.v[dgr::V_SA0 + p])[o];
       }
+    }
+    __syncthreads();
+
+    // point coordinates, eta (rhs.cpp:214-223) and the Sommerfeld face test
+    // inactive threads (t >= NPT) never touch their slots
+    s.C2.p = sacc + t;
+    s.R.p = sacc + 18 * NPT + t;
+    s.gt_rhs.p = sacc + 24 * NPT + t;
+    s.At_rhs.p = sacc + 30 * NPT + t;
+    s.acc.p = sacc + t;
+    // RK4 stages (gather) write k in the compact tile-major layout of the
+    // stage kernels: slot of the tile in block order x 343 + local index
+    C.oo = GATHER ? (uint64_t)slot[tile_begin + w] * NPT + (uint64_t)(C.ti + TX * (C.tj + TY * C.tk)) : C.o;
+    C.on_face = false;
+    C.eta = P.eta_const;
+    if (C.act) {
+#pragma clang fp contract(off)
+      double x, y, z;
+      const double geo[6] = {B.pmin[0], B.pmin[1], B.pmin[2], B.h[0], B.h[1], B.h[2]};
+      const double rr = point_coords(geo, C.gi, C.gj, C.gk, x, y, z);
+      if (rr >= P.eta_r0) C.eta *= eta_damping(P.eta_r0 / rr, P.eta_damping_exp);
+      const unsigned bf = C.bflag;
+      C.on_face = ((bf & 1u) && C.gi == 3) || ((bf & 2u) && C.gi == T.sz0 - 4) || ((bf & 4u) && C.gj == 3) ||
+                  ((bf & 8u) && C.gj == T.sz1 - 4) || ((bf & 16u) && C.gk == 3) ||
+                  ((bf & 32u) && C.gk == T.sz2 - 4);
       s.begin();
     }
     const uint32_t zmask = BND ? beyond_face_mask(T, B.bflag, t) : 0u;
     PlaneSlots<ORD> ps;
     if constexpr (!BND) ps = make_plane_slots<ORD>(T, t);
+    // the first variable into buffer 0, the second on its way
+    stage(pf, su0, T, t, zmask);
+    prefetch(pf, io->u.v[h_order[1]], T, sl);
     if constexpr (FM == FM_FACES) {
       // most boundary blocks have flagged faces along one axis only: their
       // tiles skip the closure tests along the other two
       const unsigned bf = C.bflag;
       const int fm = __builtin_amdgcn_readfirstlane(((bf & 3u) ? 1 : 0) | ((bf & 12u) ? 2 : 0) | ((bf & 48u) ? 4 : 0));
-      if (fm == 1) run_vars<0, 1, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, ps, zmask);
-      else if (fm == 2) run_vars<0, 2, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, ps, zmask);
-      else if (fm == 4) run_vars<0, 4, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, ps, zmask);
-      else run_vars<0, 7, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, ps, zmask);
+      if (fm == 1) run_vars<0, 1, ORD, GATHER>(s, C, pf, su0, su1, sgx, sgy, T, &s_io, t, sl, ps, zmask);
+      else if (fm == 2) run_vars<0, 2, ORD, GATHER>(s, C, pf, su0, su1, sgx, sgy, T, &s_io, t, sl, ps, zmask);
+      else if (fm == 4) run_vars<0, 4, ORD, GATHER>(s, C, pf, su0, su1, sgx, sgy, T, &s_io, t, sl, ps, zmask);
+      else run_vars<0, 7, ORD, GATHER>(s, C, pf, su0, su1, sgx, sgy, T, &s_io, t, sl, ps, zmask);
     } else {
-      run_vars<0, FM, ORD, GATHER>(s, C, pf, su, sgx, sgy, T, &s_io, t, sl, ps, zmask);
+      run_vars<0, FM, ORD, GATHER>(s, C, pf, su0, su1, sgx, sgy, T, &s_io, t, sl, ps, zmask);
     }
+    // the next tile rewrites the shared block constants and buffer 0
+    __syncthreads();
   }
 }
 
@@ -1219,6 +1306,7 @@ void free_plan(dgr_plan_s* plan) {
   (void)hipFree(plan->d_tile_slot);
   (void)hipFree(plan->d_io);
   (void)hipFree(plan->d_nbr);
+  (void)hipFree(plan->d_delta);
   (void)hipFree(plan->d_fill_items);
   (void)hipFree(plan->d_fill_groups);
   (void)hipFree(plan->d_fill_copies);
@@ -1488,8 +1576,12 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
     for (uint32_t b = 0; b < nblocks; ++b) {
       const dgr_block& bl = blocks[b];
       if ((pass == 0) != (bl.bflag == 0)) continue;
+      // the reference's bssnrhs takes any block with its 3-point padding;
+      // one-sided closures need at least 6 interior points along an axis
+      // with flagged faces (sz >= 12), as its own bssnrhs does
+      const uint32_t szmin = bl.bflag ? 12u : 7u;
       for (int d = 0; d < 3; ++d) {
-        if (bl.sz[d] < 13 || bl.sz[d] > 65535) { delete plan; return DGR_E_INVALID; }
+        if (bl.sz[d] < szmin || bl.sz[d] > 65535) { delete plan; return DGR_E_INVALID; }
       }
       DevBlock& db = hb[b];
       db.offset = bl.offset;
@@ -1563,6 +1655,7 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
       hipMalloc(&plan->d_tile_slot, sizeof(uint32_t) * std::max<size_t>(1, hslot.size())) != hipSuccess ||
       hipMalloc(&plan->d_io, sizeof(IOPtrs)) != hipSuccess ||
       hipMalloc(&plan->d_nbr, sizeof(int32_t) * plan->nbr.size()) != hipSuccess ||
+      hipMalloc(&plan->d_delta, sizeof(long long) * plan->nbr.size()) != hipSuccess ||
       hipMalloc(&plan->d_flag, 256) != hipSuccess) {
     free_plan(plan);
     return DGR_E_NOMEM;
@@ -1573,6 +1666,22 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
     e = hipMemcpy(plan->d_tiles_blk, ht.data(), sizeof(DevTile) * ht.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(plan->d_nbr, plan->nbr.data(), sizeof(int32_t) * plan->nbr.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    // gathering RHS: index distance from a ghost point to the same point of
+    // the neighbour's interior, per block and ghost direction (0 where no
+    // neighbour: the block's own padding)
+    std::vector<long long> hd(plan->nbr.size(), 0);
+    for (uint32_t b = 0; b < nblocks; ++b)
+      for (int d = 0; d < NNBR; ++d) {
+        const int n = plan->nbr[(size_t)b * NNBR + d];
+        if (n < 0) continue;
+        const long long dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+        const long long s0 = hb[b].sz[0], s1 = hb[b].sz[1], s2 = hb[b].sz[2];
+        hd[(size_t)b * NNBR + d] = (long long)hb[n].offset - (long long)hb[b].offset -
+                                   (dx * (s0 - 7) + s0 * (dy * (s1 - 7) + s1 * dz * (s2 - 7)));
+      }
+    e = hipMemcpy(plan->d_delta, hd.data(), sizeof(long long) * hd.size(), hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess && !hslot.empty())
     e = hipMemcpy(plan->d_tile_slot, hslot.data(), sizeof(uint32_t) * hslot.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -1664,7 +1773,7 @@ int dgr_internal::launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st
   auto grid = [&](int n) { return plan->grid_mult == 0 ? n : std::min(plan->grid * (int)plan->grid_mult, n); };
   const auto kin = rhs_kernel(false, plan->order, gather);
   const auto kbd = rhs_kernel(true, plan->order, gather);
-  const int32_t* nb = gather ? plan->d_nbr : nullptr;
+  const long long* nb = gather ? plan->d_delta : nullptr;
   const uint32_t* sl = gather ? plan->d_tile_slot : nullptr;
   // both ranges present: the boundary kernel (the longer tiles, launched
   // first) runs on the plan's second stream beside the interior kernel
@@ -1708,7 +1817,7 @@ extern "C" int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const
     const int nt = (int)plan->block_tile_count[b];
     const auto k = rhs_kernel(tb >= (int)plan->ntiles_interior, plan->order, false);
     hipLaunchKernelGGL(k, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, tb, nt, plan->d_io,
-                       plan->params, (const int32_t*)nullptr, (const uint32_t*)nullptr);
+                       plan->params, (const long long*)nullptr, (const uint32_t*)nullptr);
   }
   DGR_HIP(hipGetLastError());
   return DGR_OK;
@@ -1743,10 +1852,10 @@ extern "C" int dgr_bssn_rhs_blkwise(dgr_plan_t plan, double* const* rhs, const d
   auto grid = [&](int m) { return std::min(plan->grid * 2, m); };
   if (nin > 0)
     hipLaunchKernelGGL(kin, dim3(grid((int)nin)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_sub, 0, (int)nin,
-                       plan->d_io, plan->params, (const int32_t*)nullptr, (const uint32_t*)nullptr);
+                       plan->d_io, plan->params, (const long long*)nullptr, (const uint32_t*)nullptr);
   if (nbd > 0)
     hipLaunchKernelGGL(kbd, dim3(grid(nbd)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_sub, (int)nin, nbd,
-                       plan->d_io, plan->params, (const int32_t*)nullptr, (const uint32_t*)nullptr);
+                       plan->d_io, plan->params, (const long long*)nullptr, (const uint32_t*)nullptr);
   DGR_HIP(hipGetLastError());
   return DGR_OK;
 }
@@ -1775,7 +1884,7 @@ extern "C" int dgr_bssn_rhs_blk(dgr_plan_t plan, double* out, const double* in, 
   DGR_HIP(hipMemcpy(plan->d_io_blk, &io, sizeof(IOPtrs), hipMemcpyHostToDevice));
   const auto k = rhs_kernel(tb >= plan->ntiles_interior, plan->order, false);
   hipLaunchKernelGGL(k, dim3(tc), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, (int)tb, (int)tc,
-                     plan->d_io_blk, plan->params, (const int32_t*)nullptr, (const uint32_t*)nullptr);
+                     plan->d_io_blk, plan->params, (const long long*)nullptr, (const uint32_t*)nullptr);
   DGR_HIP(hipGetLastError());
   return DGR_OK;
 }

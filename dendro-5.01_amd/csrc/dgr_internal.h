@@ -112,6 +112,7 @@ struct dgr_plan_s {
   dgr_internal::IOPtrs h_io{};           // last uploaded pointers
   bool io_valid = false;
   int32_t* d_nbr = nullptr;              // NNBR neighbour slots per block
+  long long* d_delta = nullptr;          // NNBR gather distances per block (gathering RHS)
   std::vector<int32_t> nbr;              // host copy of the neighbour table
   uint64_t dof_end = 0;                  // one past the last padded point of any block
   double chi_floor = 0.1;                // CHI_FLOOR of the constraint projection

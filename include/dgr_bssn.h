@@ -102,7 +102,8 @@ int dgr_init(int device);
 /* Build a plan for a block list.  Replaces the per-call setup of
  * cuda::computeRHS (BSSN_GR/cuda_gr/src/rhs_cuda.cu:35-143): the block table
  * is uploaded once and split into tiles of at most 7x7x7 interior points.
- * Blocks need sz >= 13 in every direction. */
+ * Blocks need sz >= 7 in every direction, sz >= 12 along every direction
+ * if any face is flagged (one-sided closures, as in bssnrhs). */
 int dgr_plan_create(dgr_plan_t* plan, const dgr_block* blocks, uint32_t nblocks,
                     const dgr_bssn_params* params);
 /* Same with an explicit schedule (NULL = default); results are identical

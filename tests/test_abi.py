@@ -41,7 +41,10 @@ def test_plan_rejects_bad_input_without_device_work():
     h = ctypes.c_void_p()
     p = native.make_params()
     assert L.dgr_plan_create(ctypes.byref(h), None, 0, ctypes.byref(p)) == -1
-    b = native.blocks_array([{"offset": 0, "sz": [9, 13, 13], "bflag": 0, "pmin": [0, 0, 0], "pmax": [1, 1, 1]}])
+    b = native.blocks_array([{"offset": 0, "sz": [6, 13, 13], "bflag": 0, "pmin": [0, 0, 0], "pmax": [1, 1, 1]}])
+    assert L.dgr_plan_create(ctypes.byref(h), b, 1, ctypes.byref(p)) == -1
+    # flagged faces need 6 interior points for the one-sided closures
+    b = native.blocks_array([{"offset": 0, "sz": [11, 13, 13], "bflag": 1, "pmin": [0, 0, 0], "pmax": [1, 1, 1]}])
     assert L.dgr_plan_create(ctypes.byref(h), b, 1, ctypes.byref(p)) == -1
     b = native.blocks_array([{"offset": 0, "sz": [13, 13, 13], "bflag": 0, "pmin": [0, 0, 0], "pmax": [1, 1, 1]}])
     bad = native.Schedule.from_dict({"tile_max": [8, 7, 7, 7], "order": 0, "grid_mult": 1})

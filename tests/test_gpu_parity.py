@@ -184,3 +184,18 @@ def test_adaptive_octree_brill_lindquist(cuda):
     U = data.brill_lindquist(blocks, dof, m1=0.5, m2=0.0, sep=1.4)
     for order in (42, 64):
         run_case(cuda, blocks, dof, U, {"deriv_order": order})
+
+
+def test_small_blocks_order4_elements(cuda):
+    """Blocks of order-4 elements (sz = 11, 5^3 interior points, the
+    reference's own test bed: BSSN_GR/src/rhsTestBed.cpp:63) without flagged
+    faces, and sz = 12 blocks (the smallest the one-sided closures allow)
+    with every face flagged, in one batch."""
+    from pydgr.layout import block_dict
+    blocks, off = [], 0
+    dx = 1.0 / 48
+    for q, (sz, bf) in enumerate([(11, 0)] * 6 + [(12, 63)] * 3 + [(7, 0)]):
+        corner = [0.05 + 0.23 * q, -0.1 + 0.01 * q, 0.02 * q]
+        blocks.append(block_dict(off, [sz] * 3, bf, corner, dx, [sz - 7] * 3))
+        off += sz ** 3
+    run_case(cuda, blocks, off, data.flat_bumps(blocks, off, seed=11, amp=1e-2, width=0.3))
