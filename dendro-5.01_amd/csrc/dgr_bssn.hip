@@ -35,6 +35,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <vector>
 
@@ -1370,6 +1371,21 @@ AxisC host_axis(double h, int sz, unsigned bflag, int a) {
   return c;
 }
 
+// Device descriptor of a block: offset, sizes, spacing and per-axis stencil
+// constants.
+void make_devblock(const dgr_block& bl, DevBlock& db) {
+  db.offset = bl.offset;
+  for (int d = 0; d < 3; ++d) {
+    db.sz[d] = bl.sz[d];
+    db.pmin[d] = bl.pmin[d];
+    db.h[d] = (bl.pmax[d] - bl.pmin[d]) / (bl.sz[d] - 1);  // rhs.cpp:148-150
+  }
+  for (int d = 0; d < 3; ++d) db.ax[d] = host_axis(db.h[d], (int)bl.sz[d], bl.bflag, d);
+  db.bflag = bl.bflag;
+  db.group = bl.group;
+  db.pad = 0;
+}
+
 // Same-level neighbour table: blocks of equal size and spacing whose
 // corners are sz-7 points apart along an axis are neighbours.  Spacings are
 // compared by a rounded log2 (equal up to rounding); corners on the integer
@@ -1454,12 +1470,17 @@ extern "C" const char* dgr_error_string(int code) {
   }
 }
 
+namespace {
+int noalloc_init(int dev);
+}  // namespace
+
 extern "C" int dgr_init(int device) {
   int n = 0;
   DGR_HIP(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return DGR_E_INVALID;
   DGR_HIP(hipSetDevice(device));
-  return DGR_OK;
+  // set-up of the no-allocation entry point on this device
+  return noalloc_init(device);
 }
 
 extern "C" void dgr_schedule_default(dgr_schedule* s) {
@@ -1584,16 +1605,7 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
         if (bl.sz[d] < szmin || bl.sz[d] > 65535) { delete plan; return DGR_E_INVALID; }
       }
       DevBlock& db = hb[b];
-      db.offset = bl.offset;
-      for (int d = 0; d < 3; ++d) {
-        db.sz[d] = bl.sz[d];
-        db.pmin[d] = bl.pmin[d];
-        db.h[d] = (bl.pmax[d] - bl.pmin[d]) / (bl.sz[d] - 1);  // rhs.cpp:148-150
-      }
-      for (int d = 0; d < 3; ++d) db.ax[d] = host_axis(db.h[d], (int)bl.sz[d], bl.bflag, d);
-      db.bflag = bl.bflag;
-      db.group = bl.group;
-      db.pad = 0;
+      make_devblock(bl, db);
       plan->ngroups = std::max<uint32_t>(plan->ngroups, bl.group + 1);
       plan->block_tile_begin[b] = (uint32_t)ht.size();
       const int tmax = (int)sc.tile_max[size_class(bl)];
@@ -1964,4 +1976,148 @@ extern "C" int dgr_bssn_rhs_blocks(double* const* rhs, const double* const* u, c
   }
   dgr_plan_destroy(plan);
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// One-shot bssnRHS without device allocation (no counterpart in the
+// reference).  The block table, the tile table and the 48 array pointers go
+// to `scratch`, a file-scope device array; each GPU holds its own copy of it
+// until the process ends.  Host code never passes the array's name to a
+// kernel or a copy: it uses the device address hipGetSymbolAddress returns,
+// looked up once per device with that device current.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t NA_OFF_TILES = sizeof(DevBlock) * DGR_NOALLOC_MAX_BLOCKS;
+constexpr size_t NA_OFF_IO = NA_OFF_TILES + sizeof(DevTile) * DGR_NOALLOC_MAX_TILES;
+constexpr size_t NA_BYTES = (NA_OFF_IO + sizeof(IOPtrs) + 255) / 256 * 256;
+static_assert(NA_OFF_TILES % 256 == 0 && NA_OFF_IO % 256 == 0, "scratch sections 256-byte aligned");
+static_assert(NA_BYTES == DGR_NOALLOC_SCRATCH_BYTES, "DGR_NOALLOC_SCRATCH_BYTES matches the scratch layout");
+
+__device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+struct NoAlloc {
+  std::mutex m;                       // host tables and the per-device addresses
+  unsigned char* ptr[NA_MAX_DEVICES] = {};  // device address of `scratch` per device
+  int ncu[NA_MAX_DEVICES] = {};
+  DevBlock hb[DGR_NOALLOC_MAX_BLOCKS];
+  DevTile ht[DGR_NOALLOC_MAX_TILES];
+};
+NoAlloc g_na;  // static storage: the entry point allocates nothing on the host either
+
+// Device address of `scratch` on the current device (looked up on the first
+// call on that device).  Caller holds g_na.m.
+int noalloc_ptr(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= NA_MAX_DEVICES) return DGR_E_UNSUPPORTED;
+  if (!g_na.ptr[dev]) {
+    void* p = nullptr;
+    DGR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    int ncu = 0;
+    DGR_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    g_na.ncu[dev] = ncu;
+    g_na.ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_na.ptr[dev];
+  return DGR_OK;
+}
+
+int noalloc_init(int dev) {
+  std::lock_guard<std::mutex> lock(g_na.m);
+  unsigned char* p = nullptr;
+  return noalloc_ptr(dev, &p);
+}
+}  // namespace
+
+extern "C" int dgr_bssn_rhs_blocks_noalloc(double* const* rhs, const double* const* u, const dgr_block* blocks,
+                                           uint32_t nblocks, const dgr_bssn_params* p, void* stream) {
+  if (!rhs || !u || !blocks || !p || nblocks == 0) return DGR_E_INVALID;
+  if (nblocks > DGR_NOALLOC_MAX_BLOCKS) return DGR_E_INVALID;
+  if (p->deriv_order != 42 && p->deriv_order != 64) return DGR_E_UNSUPPORTED;
+  IOPtrs io;
+  for (int v = 0; v < 24; ++v) {
+    if (!rhs[v] || !u[v]) return DGR_E_INVALID;
+    io.u.v[v] = u[v];
+    io.r.v[v] = rhs[v];
+  }
+  // tiles of at most 7 points per axis (the default schedule, block order)
+  auto parts = [](uint32_t n) { return (n + TX - 1) / TX; };
+  uint64_t ntiles = 0;
+  for (uint32_t b = 0; b < nblocks; ++b) {
+    const dgr_block& bl = blocks[b];
+    const uint32_t szmin = bl.bflag ? 12u : 7u;
+    for (int d = 0; d < 3; ++d)
+      if (bl.sz[d] < szmin || bl.sz[d] > 65535) return DGR_E_INVALID;
+    ntiles += (uint64_t)parts(bl.sz[0] - 6) * parts(bl.sz[1] - 6) * parts(bl.sz[2] - 6);
+  }
+  if (ntiles > DGR_NOALLOC_MAX_TILES) return DGR_E_INVALID;
+  const hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lock(g_na.m);
+  int dev = 0;
+  DGR_HIP(hipGetDevice(&dev));
+  unsigned char* ptr = nullptr;
+  const int rc = noalloc_ptr(dev, &ptr);
+  if (rc != DGR_OK) return rc;
+  // interior blocks' tiles first, then the boundary blocks' (the two
+  // kernels), each block's tiles as dgr_plan_create cuts them
+  uint32_t nt = 0, ni = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (uint32_t b = 0; b < nblocks; ++b) {
+      const dgr_block& bl = blocks[b];
+      if ((pass == 0) != (bl.bflag == 0)) continue;
+      make_devblock(bl, g_na.hb[b]);
+      uint32_t np[3], n[3];
+      for (int d = 0; d < 3; ++d) {
+        n[d] = bl.sz[d] - 6;
+        np[d] = parts(n[d]);
+      }
+      uint32_t c0[3] = {3, 3, 3};
+      for (uint32_t pz = 0; pz < np[2]; ++pz) {
+        const uint32_t lz = n[2] / np[2] + (pz < n[2] % np[2] ? 1 : 0);
+        c0[1] = 3;
+        for (uint32_t py = 0; py < np[1]; ++py) {
+          const uint32_t ly = n[1] / np[1] + (py < n[1] % np[1] ? 1 : 0);
+          c0[0] = 3;
+          for (uint32_t px = 0; px < np[0]; ++px) {
+            const uint32_t lx = n[0] / np[0] + (px < n[0] % np[0] ? 1 : 0);
+            DevTile& t = g_na.ht[nt++];
+            t.block = b;
+            t.i0 = (uint16_t)c0[0]; t.j0 = (uint16_t)c0[1]; t.k0 = (uint16_t)c0[2];
+            t.nx = (uint8_t)lx; t.ny = (uint8_t)ly; t.nz = (uint8_t)lz;
+            t.pad = face_map_id(bl, t);
+            c0[0] += lx;
+          }
+          c0[1] += ly;
+        }
+        c0[2] += lz;
+      }
+    }
+    if (pass == 0) ni = nt;
+  }
+  DevBlock* d_blocks = reinterpret_cast<DevBlock*>(ptr);
+  DevTile* d_tiles = reinterpret_cast<DevTile*>(ptr + NA_OFF_TILES);
+  IOPtrs* d_io = reinterpret_cast<IOPtrs*>(ptr + NA_OFF_IO);
+  DGR_HIP(hipMemcpyAsync(d_blocks, g_na.hb, sizeof(DevBlock) * nblocks, hipMemcpyHostToDevice, st));
+  DGR_HIP(hipMemcpyAsync(d_tiles, g_na.ht, sizeof(DevTile) * nt, hipMemcpyHostToDevice, st));
+  DGR_HIP(hipMemcpyAsync(d_io, &io, sizeof(IOPtrs), hipMemcpyHostToDevice, st));
+  DevParams P;
+  for (int i = 0; i < 4; ++i) P.lambda[i] = p->lambda[i];
+  P.lambda_f[0] = p->lambda_f[0];
+  P.lambda_f[1] = p->lambda_f[1];
+  P.eta_const = p->eta_const;
+  P.eta_r0 = p->eta_r0;
+  P.eta_damping_exp = p->eta_damping_exp;
+  P.ko_sigma = p->ko_sigma;
+  // persistent grid of one workgroup per CU, both tile ranges on the
+  // caller's stream one after the other
+  const int order = (int)p->deriv_order;
+  const int nbd = (int)(nt - ni);
+  auto grid = [&](int n) { return std::min(std::max(g_na.ncu[dev], 1), n); };
+  if (ni > 0)
+    hipLaunchKernelGGL(rhs_kernel(false, order, false), dim3(grid((int)ni)), dim3(NT), 0, st, d_blocks, d_tiles, 0,
+                       (int)ni, d_io, P, nullptr, nullptr);
+  if (nbd > 0)
+    hipLaunchKernelGGL(rhs_kernel(true, order, false), dim3(grid(nbd)), dim3(NT), 0, st, d_blocks, d_tiles,
+                       (int)ni, nbd, d_io, P, nullptr, nullptr);
+  DGR_HIP(hipGetLastError());
+  return DGR_OK;
 }

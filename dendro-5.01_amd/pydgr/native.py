@@ -121,6 +121,8 @@ def lib():
             getattr(L, f).argtypes = [vp, pp, pp, vp]
         L.dgr_bssn_rhs_blocks.argtypes = [pp, pp, ctypes.POINTER(Block), ctypes.c_uint32,
                                           ctypes.POINTER(BSSNParams), vp]
+        L.dgr_bssn_rhs_blocks_noalloc.argtypes = [pp, pp, ctypes.POINTER(Block), ctypes.c_uint32,
+                                                  ctypes.POINTER(BSSNParams), vp]
         L.dgr_ghost_refill.argtypes = [vp, pp, vp]
         L.dgr_plan_level_fills.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]
         L.dgr_enforce_constraints.argtypes = [vp, pp, vp]
@@ -346,6 +348,18 @@ def rhs_blocks(rhs, u, blocks, params, stream=None):
     up, _u = ptr_array(var_ptrs(u))
     check(L.dgr_bssn_rhs_blocks(rp, up, barr, len(blocks), ctypes.byref(params), _stream_ptr(stream)),
           "dgr_bssn_rhs_blocks")
+
+
+def rhs_blocks_noalloc(rhs, u, blocks, params, stream=None):
+    """One-shot bssnRHS that allocates no device memory
+    (dgr_bssn_rhs_blocks_noalloc): asynchronous on `stream`; at most
+    DGR_NOALLOC_MAX_BLOCKS blocks and DGR_NOALLOC_MAX_TILES tiles."""
+    L = lib()
+    barr = blocks_array(blocks)
+    rp, _r = ptr_array(var_ptrs(rhs))
+    up, _u = ptr_array(var_ptrs(u))
+    check(L.dgr_bssn_rhs_blocks_noalloc(rp, up, barr, len(blocks), ctypes.byref(params), _stream_ptr(stream)),
+          "dgr_bssn_rhs_blocks_noalloc")
 
 
 class RK4:

@@ -145,6 +145,25 @@ int dgr_bssn_rhs_blk(dgr_plan_t plan, double* out, const double* in, uint32_t do
 int dgr_bssn_rhs_blocks(double* const* rhs, const double* const* u, const dgr_block* blocks,
                         uint32_t nblocks, const dgr_bssn_params* params, void* stream);
 
+/* One-shot bssnRHS that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels and bitwise the same result as
+ * dgr_bssn_rhs_blocks.  The block, tile and pointer tables go to a
+ * file-scope device array of DGR_NOALLOC_SCRATCH_BYTES (about 4.1 MB) that
+ * each GPU the library runs on holds until the process ends; at most
+ * DGR_NOALLOC_MAX_BLOCKS blocks cut into at most DGR_NOALLOC_MAX_TILES
+ * tiles of up to 7^3 interior points (a 13^3 block is one tile, a 37^3
+ * block 125), else DGR_E_INVALID before any device work.  Asynchronous on
+ * `stream` (the tables are copied from host memory before the call
+ * returns).  Every call on a GPU shares that GPU's array: calls on one
+ * stream are ordered, calls that overlap on different streams race. */
+#define DGR_NOALLOC_MAX_BLOCKS 8192u
+#define DGR_NOALLOC_MAX_TILES 65536u
+#define DGR_NOALLOC_SCRATCH_BYTES 4063744u
+int dgr_bssn_rhs_blocks_noalloc(double* const* rhs, const double* const* u, const dgr_block* blocks,
+                                uint32_t nblocks, const dgr_bssn_params* params, void* stream);
+
 /* "Unzip" of the padding between RK stages (ghost exchange + unzip,
  * BSSN_GR/src/rkBSSN.cpp:909-914, Mesh::unzip include/mesh.tcc:7436):
  * every ghost point of every block (outside [3, sz-3)^3) with a same-level

@@ -53,3 +53,29 @@ def test_plan_rejects_bad_input_without_device_work():
     assert L.dgr_plan_create_ex(ctypes.byref(h), b, 1, ctypes.byref(p), ctypes.byref(bad)) == -1
     p.deriv_order = 66  # only 42 and 64 exist
     assert L.dgr_plan_create(ctypes.byref(h), b, 1, ctypes.byref(p)) == -4
+
+
+def test_noalloc_rejects_inputs_beyond_its_limits_without_device_work():
+    """dgr_bssn_rhs_blocks_noalloc returns DGR_E_INVALID before any HIP call
+    when the block or tile tables would not fit its device array (the array
+    pointers are never dereferenced on that path)."""
+    import numpy as np
+    from pydgr import native
+    L = native.lib()
+    prm = native.make_params()
+    fake = (ctypes.c_void_p * 24)(*[ctypes.c_void_p(0x1000 + 8 * v) for v in range(24)])
+    pp = ctypes.cast(fake, ctypes.POINTER(ctypes.c_void_p))
+    big = np.zeros(8193, dtype=native._block_dtype())  # DGR_NOALLOC_MAX_BLOCKS + 1
+    big["sz"] = 13
+    big["pmax"] = 1.0
+    arr = native.blocks_array(big)
+    assert L.dgr_bssn_rhs_blocks_noalloc(pp, pp, arr, len(big), ctypes.byref(prm), None) == -1
+    # 8192 blocks of 61^3: 8^3 tiles each, beyond DGR_NOALLOC_MAX_TILES
+    many = np.zeros(8192, dtype=native._block_dtype())
+    many["sz"] = 61
+    many["pmax"] = 1.0
+    arr = native.blocks_array(many)
+    assert L.dgr_bssn_rhs_blocks_noalloc(pp, pp, arr, len(many), ctypes.byref(prm), None) == -1
+    # a block below the smallest size is rejected as by dgr_plan_create
+    one = native.blocks_array([{"offset": 0, "sz": [6, 13, 13], "bflag": 0, "pmin": [0, 0, 0], "pmax": [1, 1, 1]}])
+    assert L.dgr_bssn_rhs_blocks_noalloc(pp, pp, one, 1, ctypes.byref(prm), None) == -1
