@@ -205,7 +205,7 @@ def ghost_refill_uniform(U, nb, sz=13):
 
 
 def ghost_refill_blocks(U, blocks):
-    """Same-level ghost refill on any block list, found from the geometry
+    r"""Same-level ghost refill on any block list, found from the geometry
     alone: a ghost box of block b (direction d in {-1,0,1}^3 \ 0) takes the
     interior values of the block of equal size, spacing and group whose
     interior corner lies (sz-7) spacings away along every axis with d != 0
