@@ -293,16 +293,40 @@ __device__ __noinline__ double eta_pow(double x, double e) { return pow(x, e); }
 __device__ __forceinline__ double eta_damping(double x, double e) { return e == 1.0 ? x : eta_pow(x, e); }
 
 // rhs = eq + (KO), or (Sommerfeld + KO) on flagged faces (rhs.cpp:265-373)
-__device__ __forceinline__ void emit(const IOPtrs* __restrict__ io, const Ctx& C, int v, double eq, double ex) {
+__device__ __forceinline__ double rhs_value(const Ctx& C, double eq, double ex) {
 #pragma clang fp contract(off)
-  const double r = C.on_face ? ex : (eq + ex);
-  __builtin_nontemporal_store(r, gmem(io->r.v[v]) + C.oo);
+  return C.on_face ? ex : (eq + ex);
 }
 // Outputs that become final several variables later park their dissipation
 // (+ Sommerfeld) term in the output array; the variable that completes them
 // loads the parked terms back (process_var).
-__device__ __forceinline__ void park(const IOPtrs* __restrict__ io, const Ctx& C, int v, double ex) {
-  gmem(io->r.v[v])[C.oo] = ex;
+
+// Right-hand sides a variable completes, stored after the next variable is
+// staged (run_vars): a store still in flight when the staging waits for its
+// prefetched registers would be waited for as well, since loads and stores
+// share one counter and the wait for an older load cannot count past a
+// younger store.  Per processed variable V, the output arrays and values
+// (NOUT of them); parked terms are plain stores, final values nontemporal.
+template <int V>
+struct Outs {
+  static constexpr int N = V == dgr::V_BETA2 ? 8 : (V == dgr::V_K ? 4 : 1);
+  static constexpr bool PARK = (V >= dgr::V_SG0 && V <= dgr::V_SG5) || V == dgr::V_CHI ||
+                               (V >= dgr::V_GT0 && V <= dgr::V_GT2);
+  static __device__ __forceinline__ constexpr int var(int k) {
+    return k == 0 ? V : (V == dgr::V_BETA2 ? (k < 7 ? dgr::V_SG0 + (k - 1) : dgr::V_CHI) : dgr::V_GT0 + (k - 1));
+  }
+  double val[N];
+};
+template <int V>
+__device__ __forceinline__ void store_outs(const IOPtrs* __restrict__ io, const Ctx& C, const Outs<V>& o) {
+  if (!C.act) return;
+  if constexpr (Outs<V>::PARK) {
+    gmem(io->r.v[V])[C.oo] = o.val[0];
+  } else {
+#pragma unroll
+    for (int k = 0; k < Outs<V>::N; ++k)
+      __builtin_nontemporal_store(o.val[k], gmem(io->r.v[Outs<V>::var(k)]) + C.oo);
+  }
 }
 
 // Per-thread LDS slots: element i of a thread's array sits at p[i * NPT]
@@ -440,7 +464,7 @@ struct Mixed {
 template <int V, int FM, int ORD>
 __device__ __forceinline__ void process_var(KPointR& s, Ctx& C, const double* su, const double* sgx,
                                             const double* sgy, const Mixed& M, const TileGeom& T,
-                                            const IOPtrs* __restrict__ io) {
+                                            const IOPtrs* __restrict__ io, Outs<V>& out) {
   using namespace dgr;
   if (!C.act) return;
   // parked terms of the outputs this variable completes (gt_rhs and
@@ -530,39 +554,40 @@ __device__ __forceinline__ void process_var(KPointR& s, Ctx& C, const double* su
     }
   }
   __builtin_amdgcn_sched_barrier(0);
-  // feed the staged equations; emit every right-hand side that is final
+  // feed the staged equations; every right-hand side that is final (and the
+  // dissipation terms parked in the output array) goes to `out`
   if constexpr (V >= V_SG0 && V <= V_SG5) {
     s.add_gt(V - V_SG0, d1, d2, adv);
-    park(io, C, V, ex);
+    out.val[0] = ex;
   }
   if constexpr (V == V_CHI) {
     s.add_chi(d1, d2, adv);
-    park(io, C, V, ex);
+    out.val[0] = ex;
   }
-  if constexpr (V == V_ALPHA) emit(io, C, V_ALPHA, s.add_alpha(C.P, d1, d2, adv), ex);
+  if constexpr (V == V_ALPHA) out.val[0] = rhs_value(C, s.add_alpha(C.P, d1, d2, adv), ex);
   if constexpr (V >= V_GT0 && V <= V_GT2) {
     s.add_Gt(C.P, V - V_GT0, d1, adv);
-    park(io, C, V, ex);
+    out.val[0] = ex;
   }
   if constexpr (V >= V_BETA0 && V <= V_BETA2) {
     double gto[6], chio = 0.0;
-    emit(io, C, V, s.add_beta(C.P, V - V_BETA0, d1, d2, adv, gto, &chio), ex);
+    out.val[0] = rhs_value(C, s.add_beta(C.P, V - V_BETA0, d1, d2, adv, gto, &chio), ex);
     if constexpr (V == V_BETA2) {
 #pragma unroll
-      for (int p = 0; p < 6; ++p) emit(io, C, V_SG0 + p, gto[p], pk[p]);
-      emit(io, C, V_CHI, chio, pk[6]);
+      for (int p = 0; p < 6; ++p) out.val[1 + p] = rhs_value(C, gto[p], pk[p]);
+      out.val[7] = rhs_value(C, chio, pk[6]);
     }
   }
   if constexpr (V == V_K) {
-    emit(io, C, V_K, s.add_K(d1, adv), ex);
+    out.val[0] = rhs_value(C, s.add_K(d1, adv), ex);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) emit(io, C, V_GT0 + k, s.Gt_out(k), pk[k]);
+    for (int k = 0; k < 3; ++k) out.val[1 + k] = rhs_value(C, s.Gt_out(k), pk[k]);
   }
   if constexpr (V >= V_B0 && V <= V_B2) {
     s.B[V - V_B0] = su[c];
-    emit(io, C, V, s.add_B(C.P, V - V_B0, adv, C.eta), ex);
+    out.val[0] = rhs_value(C, s.add_B(C.P, V - V_B0, adv, C.eta), ex);
   }
-  if constexpr (V >= V_SA0 && V <= V_SA5) emit(io, C, V, s.add_At(V - V_SA0, adv), ex);
+  if constexpr (V >= V_SA0 && V <= V_SA5) out.val[0] = rhs_value(C, s.add_At(V - V_SA0, adv), ex);
 }
 
 // Widened first-derivative planes for the mixed derivatives, walked in
@@ -698,11 +723,13 @@ __device__ __forceinline__ void run_vars(KPointR& s, Ctx& C, double (&pf)[NPF], 
     if constexpr (V == dgr::V_CHI) {
       if (C.act) s.finish_metric();
     }
-    process_var<V, FM, ORD>(s, C, su, sgx, sgy, M, T, io);
-    if constexpr (I + 1 < 24) {
-      stage(pf, sn, T, t, zmask);
-      if constexpr (I + 2 < 24) prefetch(pf, io->u.v[h_order[I + 2]], T, sl);
-    }
+    Outs<V> out;
+    process_var<V, FM, ORD>(s, C, su, sgx, sgy, M, T, io, out);
+    // the next variable into the other buffer, then this variable's stores,
+    // then the loads of the variable after next
+    if constexpr (I + 1 < 24) stage(pf, sn, T, t, zmask);
+    store_outs<V>(io, C, out);
+    if constexpr (I + 2 < 24) prefetch(pf, io->u.v[h_order[I + 2]], T, sl);
     run_vars<I + 1, FM, ORD, GATHER>(s, C, pf, su0, su1, sgx, sgy, T, io, t, sl, ps, zmask);
   }
 }
