@@ -225,12 +225,19 @@ __device__ __forceinline__ Slots make_slots(const TileGeom& T, int t, const long
 // The region is walked in the fixed SX x SY x SZ layout of the LDS buffer;
 // slots outside a partial tile's region are skipped and their LDS slots are
 // never read.
+// Plain inputs are read once (non-temporal loads); in gather mode (CACHED)
+// ghost slots read neighbours' interiors that the neighbours' own tiles,
+// scheduled on the same L2, read too, so those loads are ordinary.
+template <bool CACHED = false>
 __device__ __forceinline__ void prefetch(double (&r)[NPF], const double* __restrict__ g, const TileGeom& T,
                                          const Slots& S) {
   const double* base = g + T.off;
 #pragma unroll
   for (int m = 0; m < NPF; ++m)
-    if ((S.valid >> m) & 1u) r[m] = __builtin_nontemporal_load(gmem(base) + S.off[m]);
+    if ((S.valid >> m) & 1u) {
+      if constexpr (CACHED) r[m] = gmem(base)[S.off[m]];
+      else r[m] = __builtin_nontemporal_load(gmem(base) + S.off[m]);
+    }
 }
 
 // zmask (boundary tiles): bit m set when slot m lies beyond a flagged face.
@@ -729,7 +736,7 @@ __device__ __forceinline__ void run_vars(KPointR& s, Ctx& C, double (&pf)[NPF], 
     // then the loads of the variable after next
     if constexpr (I + 1 < 24) stage(pf, sn, T, t, zmask);
     store_outs<V>(io, C, out);
-    if constexpr (I + 2 < 24) prefetch(pf, io->u.v[h_order[I + 2]], T, sl);
+    if constexpr (I + 2 < 24) prefetch<GATHER>(pf, io->u.v[h_order[I + 2]], T, sl);
     run_vars<I + 1, FM, ORD, GATHER>(s, C, pf, su0, su1, sgx, sgy, T, io, t, sl, ps, zmask);
   }
 }
@@ -852,7 +859,7 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
     }
     const Slots sl = make_slots<GATHER>(T, t, s_delta);
     double pf[NPF];
-    prefetch(pf, io->u.v[h_order[0]], T, sl);
+    prefetch<GATHER>(pf, io->u.v[h_order[0]], T, sl);
     KPointR s;
     C.o = T.off + (uint64_t)C.gi + (uint64_t)T.sz0 * (C.gj + (uint64_t)T.sz1 * C.gk);
     if (C.act) {
@@ -900,7 +907,7 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
     if constexpr (!BND) ps = make_plane_slots<ORD>(T, t);
     // the first variable into buffer 0, the second on its way
     stage(pf, su0, T, t, zmask);
-    prefetch(pf, io->u.v[h_order[1]], T, sl);
+    prefetch<GATHER>(pf, io->u.v[h_order[1]], T, sl);
     if constexpr (FM == FM_FACES) {
       // most boundary blocks have flagged faces along one axis only: their
       // tiles skip the closure tests along the other two
