@@ -1255,9 +1255,12 @@ struct Ptr6 {
 };
 constexpr int NORM_T = 256;
 
+// Each unique node once: only points that own their node (the zip's owner
+// flags) enter the sums, as computeConstraintL2Norm sums over the zipped
+// vector (BSSN_GR/include/grUtils.tcc:40-88).
 __global__ void __launch_bounds__(NORM_T) k_con_norm_partial(const DevBlock* __restrict__ blocks,
                                                              const DevTile* __restrict__ tiles, int ntiles,
-                                                             const int32_t* __restrict__ nbr, Ptr6 con,
+                                                             const uint8_t* __restrict__ own, Ptr6 con,
                                                              const double* __restrict__ mask, double thr,
                                                              double* __restrict__ part) {
   __shared__ double red[NORM_T];
@@ -1265,16 +1268,12 @@ __global__ void __launch_bounds__(NORM_T) k_con_norm_partial(const DevBlock* __r
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const DevTile tl = tiles[w];
     const DevBlock& B = blocks[tl.block];
-    const int32_t* nb = nbr + (size_t)tl.block * NNBR;
     double sq[6] = {0, 0, 0, 0, 0, 0}, mx[6] = {0, 0, 0, 0, 0, 0}, cnt = 0.0;
     const int np = tl.nx * tl.ny * tl.nz;
     for (int e = t; e < np; e += NORM_T) {
       const int i = tl.i0 + e % tl.nx, j = tl.j0 + (e / tl.nx) % tl.ny, k = tl.k0 + e / (tl.nx * tl.ny);
-      if ((i == (int)B.sz[0] - 4 && nb[14] >= 0) || (j == (int)B.sz[1] - 4 && nb[16] >= 0) ||
-          (k == (int)B.sz[2] - 4 && nb[22] >= 0))
-        continue;
       const uint64_t o = B.offset + (uint64_t)i + (uint64_t)B.sz[0] * (j + (uint64_t)B.sz[1] * k);
-      if (mask[o] < thr) continue;
+      if (!own[o] || mask[o] < thr) continue;
       cnt += 1.0;
       for (int q = 0; q < 6; ++q) {
         const double v = con.v[q][o];
@@ -1294,6 +1293,14 @@ __global__ void __launch_bounds__(NORM_T) k_con_norm_partial(const DevBlock* __r
       __syncthreads();
     }
   }
+}
+
+// Mesh::zip (include/mesh.tcc:3775-3822): node k of the zipped vector takes
+// the value at padded index idx[k] (its owner's copy).
+__global__ void __launch_bounds__(256) k_zip(const uint64_t* __restrict__ idx, uint64_t n,
+                                             const double* __restrict__ in, double* __restrict__ out) {
+  for (uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (uint64_t)gridDim.x * 256)
+    out[k] = in[idx[k]];
 }
 
 // Sum of the per-tile partials; out[13] = L2[6], max[6], count.  One
@@ -1350,6 +1357,8 @@ void free_plan(dgr_plan_s* plan) {
   (void)hipFree(plan->d_tiles_sub);
   (void)hipFree(plan->d_io_blk);
   (void)hipFree(plan->d_flag);
+  (void)hipFree(plan->d_own);
+  (void)hipFree(plan->d_zidx);
   if (plan->ev_fork) (void)hipEventDestroy(plan->ev_fork);
   if (plan->ev_join) (void)hipEventDestroy(plan->ev_join);
   if (plan->aux) (void)hipStreamDestroy(plan->aux);
@@ -1746,6 +1755,7 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
     free_plan(plan);
     return rc;
   }
+  plan->h_blocks = std::move(hb);
   *plan_out = plan;
   return DGR_OK;
 }
@@ -1966,9 +1976,94 @@ extern "C" int dgr_physcon(dgr_plan_t plan, double* const* con, const double* co
   return DGR_OK;
 }
 
+namespace {
+// Owner of every unique node (the zipped vector's entries), from the block
+// geometry the plan already holds.  A node on the interior boundary of a
+// block can be shared:
+//   - with same-level neighbours (face, edge or corner contact): of the
+//     blocks around the node the one lying furthest in +z, then +y, then +x
+//     owns it, so a block skips the node when a same-level neighbour that
+//     also holds it lies in a direction whose last nonzero component (z
+//     before y before x) is positive (the copies are equal);
+//   - across a level jump: a fine block's node on a side whose neighbour is
+//     coarser either lies between the coarse nodes (a hanging node, which has
+//     no zipped entry: Mesh::zip copies only nodes whose owning element is
+//     the block's own, include/mesh.tcc:3805-3812) or coincides with a coarse
+//     node, which the coarse block owns; the fine block owns neither.
+// Zipped order: blocks in order, each block's owned points x fastest.
+int build_zip(dgr_plan_t plan) {
+  if (plan->zip_built) return DGR_OK;
+  std::vector<uint8_t> own(plan->dof_end, 0);
+  std::vector<uint64_t> idx;
+  idx.reserve(plan->points);
+  for (uint32_t b = 0; b < plan->nblocks; ++b) {
+    const DevBlock& B = plan->h_blocks[b];
+    const int32_t* nb = &plan->nbr[(size_t)b * NNBR];
+    const uint32_t cd = plan->coarse_dirs.empty() ? 0u : plan->coarse_dirs[b];
+    const int s0 = (int)B.sz[0], s1 = (int)B.sz[1], s2 = (int)B.sz[2];
+    for (int k = 3; k < s2 - 3; ++k)
+      for (int j = 3; j < s1 - 3; ++j)
+        for (int i = 3; i < s0 - 3; ++i) {
+          // interior points hold no shared node
+          if (i > 3 && i < s0 - 4 && j > 3 && j < s1 - 4 && k > 3 && k < s2 - 4) {
+            const uint64_t o = B.offset + (uint64_t)i + (uint64_t)s0 * (j + (uint64_t)s1 * k);
+            own[o] = 1;
+            idx.push_back(o);
+            continue;
+          }
+          bool skip = false;
+          for (int d = 0; d < NNBR && !skip; ++d) {
+            if (d == 13) continue;
+            const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+            const bool on = (dx == 0 || i == (dx < 0 ? 3 : s0 - 4)) && (dy == 0 || j == (dy < 0 ? 3 : s1 - 4)) &&
+                            (dz == 0 || k == (dz < 0 ? 3 : s2 - 4));
+            if (!on) continue;
+            const bool up = dz > 0 || (dz == 0 && (dy > 0 || (dy == 0 && dx > 0)));
+            skip = ((cd >> d) & 1u) || (up && nb[d] >= 0);
+          }
+          if (skip) continue;
+          const uint64_t o = B.offset + (uint64_t)i + (uint64_t)s0 * (j + (uint64_t)s1 * k);
+          own[o] = 1;
+          idx.push_back(o);
+        }
+  }
+  DGR_HIP(hipMalloc(&plan->d_own, std::max<size_t>(1, own.size())));
+  DGR_HIP(hipMalloc(&plan->d_zidx, sizeof(uint64_t) * std::max<size_t>(1, idx.size())));
+  DGR_HIP(hipMemcpy(plan->d_own, own.data(), own.size(), hipMemcpyHostToDevice));
+  if (!idx.empty()) DGR_HIP(hipMemcpy(plan->d_zidx, idx.data(), sizeof(uint64_t) * idx.size(), hipMemcpyHostToDevice));
+  plan->nzip = idx.size();
+  plan->zip_built = true;
+  return DGR_OK;
+}
+}  // namespace
+
+extern "C" int dgr_zip_size(dgr_plan_t plan, uint64_t* n) {
+  if (!plan || !n) return DGR_E_INVALID;
+  const int rc = build_zip(plan);
+  if (rc != DGR_OK) return rc;
+  *n = plan->nzip;
+  return DGR_OK;
+}
+
+extern "C" int dgr_zip(dgr_plan_t plan, double* zipped, const double* unzipped, void* stream) {
+  if (!plan || !zipped || !unzipped) return DGR_E_INVALID;
+  const int rc = build_zip(plan);
+  if (rc != DGR_OK) return rc;
+  if (plan->nzip == 0) return DGR_OK;
+  const uint64_t nwg = std::min<uint64_t>((plan->nzip + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_zip, dim3((unsigned)nwg), dim3(256), 0, (hipStream_t)stream, plan->d_zidx, plan->nzip,
+                     unzipped, zipped);
+  DGR_HIP(hipGetLastError());
+  return DGR_OK;
+}
+
 extern "C" int dgr_physcon_norms(dgr_plan_t plan, const double* const* con, const double* mask, double threshold,
                                  double* out, void* stream) {
   if (!plan || !con || !mask || !out) return DGR_E_INVALID;
+  {
+    const int rc = build_zip(plan);
+    if (rc != DGR_OK) return rc;
+  }
   hipStream_t st = (hipStream_t)stream;
   Ptr6 c6;
   for (int q = 0; q < DGR_NUM_CONSTRAINTS; ++q) {
@@ -1983,7 +2078,7 @@ extern "C" int dgr_physcon_norms(dgr_plan_t plan, const double* const* con, cons
   if (!plan->d_norm) DGR_HIP(hipMalloc(&plan->d_norm, sizeof(double) * 13 * ((size_t)nt + 1)));
   double* res = plan->d_norm + (size_t)13 * nt;
   hipLaunchKernelGGL(k_con_norm_partial, dim3(std::min(nt, 8192)), dim3(NORM_T), 0, st, plan->d_blocks,
-                     plan->d_tiles_blk, nt, plan->d_nbr, c6, mask, threshold, plan->d_norm);
+                     plan->d_tiles_blk, nt, plan->d_own, c6, mask, threshold, plan->d_norm);
   hipLaunchKernelGGL(k_con_norm_final, dim3(13), dim3(NORM_T), 0, st, plan->d_norm, nt, res);
   DGR_HIP(hipGetLastError());
   double h[13];

@@ -142,6 +142,16 @@ struct dgr_plan_s {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // flag word of dgr_enforce_constraints (allocated with the plan)
   unsigned* d_flag = nullptr;
+  // zip to unique nodal values (built on first use): host block table, per
+  // block the bit set of the 27 directions whose neighbour is coarser (from
+  // the level-jump fills), the owner flag of every padded point and the
+  // padded index of every unique node in zipped order
+  std::vector<dgr_internal::DevBlock> h_blocks;
+  std::vector<uint32_t> coarse_dirs;
+  bool zip_built = false;
+  uint64_t nzip = 0;
+  uint8_t* d_own = nullptr;
+  uint64_t* d_zidx = nullptr;
 };
 
 namespace dgr_internal {

@@ -183,6 +183,7 @@ int dgr_internal::build_level_fill(dgr_plan_t plan, const dgr_block* blocks, con
 
   std::vector<FillItem> items;
   std::vector<FillCopy> copies;
+  plan->coarse_dirs.assign(nb, 0u);
   for (uint32_t b = 0; b < nb; ++b) {
     const DevBlock& B = hb[b];
     if (!regular(B)) continue;
@@ -223,6 +224,7 @@ int dgr_internal::build_level_fill(dgr_plan_t plan, const dgr_block* blocks, con
         it.dst = B.offset + lo[0] + (uint64_t)B.sz[0] * (lo[1] + (uint64_t)B.sz[1] * lo[2]);
         it.src = C.offset + 3 + (uint64_t)C.sz[0] * (3 + (uint64_t)C.sz[1] * 3);
         items.push_back(it);
+        plan->coarse_dirs[b] |= 1u << d;
         continue;
       }
       // finer neighbours: each ghost point is a node of a block of spacing h/2

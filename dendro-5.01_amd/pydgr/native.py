@@ -151,6 +151,8 @@ def lib():
         L.dgr_rk4_status.argtypes = [vp]
         L.dgr_rk4_destroy.argtypes = [vp]
         L.dgr_block_neighbours.argtypes = [ctypes.POINTER(Block), ctypes.c_uint32, vp]
+        L.dgr_zip_size.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+        L.dgr_zip.argtypes = [vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -316,6 +318,19 @@ class Plan:
         cp, _c = ptr_array([con[q].data_ptr() for q in range(NUM_CONSTRAINTS)])
         up, _u = ptr_array(var_ptrs(u))
         check(lib().dgr_physcon(self.handle, cp, up, _stream_ptr(stream)), "dgr_physcon")
+
+    def zip_size(self):
+        """Number of unique nodes of the plan's blocks (dgr_zip_size)."""
+        n = ctypes.c_uint64()
+        check(lib().dgr_zip_size(self.handle, ctypes.byref(n)), "dgr_zip_size")
+        return int(n.value)
+
+    def zip(self, zipped, unzipped, stream=None):
+        """Mesh::zip of one unzipped float64 device array into `zipped`
+        (float64 device tensor of zip_size() elements)."""
+        assert zipped.numel() >= self.zip_size()
+        check(lib().dgr_zip(self.handle, ctypes.c_void_p(zipped.data_ptr()), ctypes.c_void_p(unzipped.data_ptr()),
+                            _stream_ptr(stream)), "dgr_zip")
 
     def physcon_norms(self, con, mask, threshold=0.3, stream=None):
         """Masked L2 and max norms of the 6 constraint outputs over the

@@ -206,12 +206,22 @@ int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* stream);
 int dgr_physcon(dgr_plan_t plan, double* const* con, const double* const* u, void* stream);
 /* Masked constraint norms (computeConstraintL2Norm,
  * BSSN_GR/include/grUtils.tcc:40-88, called per constraint by
- * extractConstraints, grUtils.tcc:91-130): over the interior points whose
- * mask value (the reference's EXTRACTION_VAR_ID, alpha by default,
- * parameters.cpp:157-158) is >= threshold, each node once (a block's high
- * face is left to a same-level neighbour on that side).  out[13] (host):
- * sqrt(mean of squares) of the 6 outputs, their max |.|, and the point
+ * extractConstraints, grUtils.tcc:91-130): over the unique nodes of the
+ * zipped vector (dgr_zip: each node once, hanging nodes at level jumps
+ * excluded) whose mask value (the reference's EXTRACTION_VAR_ID, alpha by
+ * default, parameters.cpp:157-158) is >= threshold.  out[13] (host):
+ * sqrt(mean of squares) of the 6 outputs, their max |.|, and the node
  * count.  Deterministic (fixed reduction order).  Synchronous. */
+/* Mesh::zip (include/mesh.tcc:3775-3822): the unique nodal values of one
+ * unzipped array (device, the plan's layout) into zipped[0..n) (device),
+ * n from dgr_zip_size.  A node shared by same-level blocks is taken from the
+ * block that has it on its low side; a fine-block node on a side whose
+ * neighbour is coarser is either a hanging node (no zipped entry) or a
+ * coarse node, taken from the coarse block.  Order: blocks in plan order,
+ * each block's nodes x fastest.  The node tables are built on the first call
+ * (synchronous then); asynchronous on `stream` afterwards. */
+int dgr_zip_size(dgr_plan_t plan, uint64_t* n);
+int dgr_zip(dgr_plan_t plan, double* zipped, const double* unzipped, void* stream);
 int dgr_physcon_norms(dgr_plan_t plan, const double* const* con, const double* mask, double threshold,
                       double* out, void* stream);
 
