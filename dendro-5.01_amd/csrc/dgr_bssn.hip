@@ -324,6 +324,16 @@ struct Outs {
   }
   double val[N];
 };
+// Output k of variable V: kept for store_outs (deriv42), or stored where it
+// is computed (the 6th-order bodies, whose register allocation prefers it).
+template <int ORD, int V>
+__device__ __forceinline__ void put_out(const IOPtrs* __restrict__ io, const Ctx& C, Outs<V>& o, int k, double v) {
+  o.val[k] = v;
+  if constexpr (ORD != 42) {
+    if constexpr (Outs<V>::PARK) gmem(io->r.v[V])[C.oo] = v;
+    else __builtin_nontemporal_store(v, gmem(io->r.v[Outs<V>::var(k)]) + C.oo);
+  }
+}
 template <int V>
 __device__ __forceinline__ void store_outs(const IOPtrs* __restrict__ io, const Ctx& C, const Outs<V>& o) {
   if (!C.act) return;
@@ -565,36 +575,36 @@ __device__ __forceinline__ void process_var(KPointR& s, Ctx& C, const double* su
   // dissipation terms parked in the output array) goes to `out`
   if constexpr (V >= V_SG0 && V <= V_SG5) {
     s.add_gt(V - V_SG0, d1, d2, adv);
-    out.val[0] = ex;
+    put_out<ORD>(io, C, out, 0, ex);
   }
   if constexpr (V == V_CHI) {
     s.add_chi(d1, d2, adv);
-    out.val[0] = ex;
+    put_out<ORD>(io, C, out, 0, ex);
   }
-  if constexpr (V == V_ALPHA) out.val[0] = rhs_value(C, s.add_alpha(C.P, d1, d2, adv), ex);
+  if constexpr (V == V_ALPHA) put_out<ORD>(io, C, out, 0, rhs_value(C, s.add_alpha(C.P, d1, d2, adv), ex));
   if constexpr (V >= V_GT0 && V <= V_GT2) {
     s.add_Gt(C.P, V - V_GT0, d1, adv);
-    out.val[0] = ex;
+    put_out<ORD>(io, C, out, 0, ex);
   }
   if constexpr (V >= V_BETA0 && V <= V_BETA2) {
     double gto[6], chio = 0.0;
-    out.val[0] = rhs_value(C, s.add_beta(C.P, V - V_BETA0, d1, d2, adv, gto, &chio), ex);
+    put_out<ORD>(io, C, out, 0, rhs_value(C, s.add_beta(C.P, V - V_BETA0, d1, d2, adv, gto, &chio), ex));
     if constexpr (V == V_BETA2) {
 #pragma unroll
-      for (int p = 0; p < 6; ++p) out.val[1 + p] = rhs_value(C, gto[p], pk[p]);
-      out.val[7] = rhs_value(C, chio, pk[6]);
+      for (int p = 0; p < 6; ++p) put_out<ORD>(io, C, out, 1 + p, rhs_value(C, gto[p], pk[p]));
+      put_out<ORD>(io, C, out, 7, rhs_value(C, chio, pk[6]));
     }
   }
   if constexpr (V == V_K) {
-    out.val[0] = rhs_value(C, s.add_K(d1, adv), ex);
+    put_out<ORD>(io, C, out, 0, rhs_value(C, s.add_K(d1, adv), ex));
 #pragma unroll
-    for (int k = 0; k < 3; ++k) out.val[1 + k] = rhs_value(C, s.Gt_out(k), pk[k]);
+    for (int k = 0; k < 3; ++k) put_out<ORD>(io, C, out, 1 + k, rhs_value(C, s.Gt_out(k), pk[k]));
   }
   if constexpr (V >= V_B0 && V <= V_B2) {
     s.B[V - V_B0] = su[c];
-    out.val[0] = rhs_value(C, s.add_B(C.P, V - V_B0, adv, C.eta), ex);
+    put_out<ORD>(io, C, out, 0, rhs_value(C, s.add_B(C.P, V - V_B0, adv, C.eta), ex));
   }
-  if constexpr (V >= V_SA0 && V <= V_SA5) out.val[0] = rhs_value(C, s.add_At(V - V_SA0, adv), ex);
+  if constexpr (V >= V_SA0 && V <= V_SA5) put_out<ORD>(io, C, out, 0, rhs_value(C, s.add_At(V - V_SA0, adv), ex));
 }
 
 // Widened first-derivative planes for the mixed derivatives, walked in
@@ -732,10 +742,13 @@ __device__ __forceinline__ void run_vars(KPointR& s, Ctx& C, double (&pf)[NPF], 
     }
     Outs<V> out;
     process_var<V, FM, ORD>(s, C, su, sgx, sgy, M, T, io, out);
-    // the next variable into the other buffer, then this variable's stores,
-    // then the loads of the variable after next
+    // deriv42: the next variable into the other buffer, then this
+    // variable's stores, then the loads of the variable after next.  The
+    // 6th-order bodies store each output where it is computed (put_out):
+    // their larger register windows spill more when the outputs stay live
+    // across the staging (measured).
     if constexpr (I + 1 < 24) stage(pf, sn, T, t, zmask);
-    store_outs<V>(io, C, out);
+    if constexpr (ORD == 42) store_outs<V>(io, C, out);
     if constexpr (I + 2 < 24) prefetch<GATHER>(pf, io->u.v[h_order[I + 2]], T, sl);
     run_vars<I + 1, FM, ORD, GATHER>(s, C, pf, su0, su1, sgx, sgy, T, io, t, sl, ps, zmask);
   }
@@ -1715,17 +1728,28 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
     free_plan(plan);
     return DGR_E_NOMEM;
   }
-  e = hipMemcpy(plan->d_blocks, hb.data(), sizeof(DevBlock) * nblocks, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(plan->d_tiles, hs.data(), sizeof(DevTile) * hs.size(), hipMemcpyHostToDevice);
+  // the uploads run on the plan's own non-blocking stream, which is then
+  // synchronised alone: a copy on the legacy default stream could wait for
+  // unrelated work that shares its hardware queue
+  e = hipStreamCreateWithFlags(&plan->aux, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&plan->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&plan->ev_join, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    free_plan(plan);
+    return hip_fail(e, "hipStreamCreate(plan)");
+  }
+  const hipStream_t up = plan->aux;
+  e = hipMemcpyAsync(plan->d_blocks, hb.data(), sizeof(DevBlock) * nblocks, hipMemcpyHostToDevice, up);
+  if (e == hipSuccess) e = hipMemcpyAsync(plan->d_tiles, hs.data(), sizeof(DevTile) * hs.size(), hipMemcpyHostToDevice, up);
   if (e == hipSuccess)
-    e = hipMemcpy(plan->d_tiles_blk, ht.data(), sizeof(DevTile) * ht.size(), hipMemcpyHostToDevice);
+    e = hipMemcpyAsync(plan->d_tiles_blk, ht.data(), sizeof(DevTile) * ht.size(), hipMemcpyHostToDevice, up);
   if (e == hipSuccess)
-    e = hipMemcpy(plan->d_nbr, plan->nbr.data(), sizeof(int32_t) * plan->nbr.size(), hipMemcpyHostToDevice);
+    e = hipMemcpyAsync(plan->d_nbr, plan->nbr.data(), sizeof(int32_t) * plan->nbr.size(), hipMemcpyHostToDevice, up);
+  // gathering RHS: index distance from a ghost point to the same point of
+  // the neighbour's interior, per block and ghost direction (0 where no
+  // neighbour: the block's own padding); alive until the uploads are done
+  std::vector<long long> hd(plan->nbr.size(), 0);
   if (e == hipSuccess) {
-    // gathering RHS: index distance from a ghost point to the same point of
-    // the neighbour's interior, per block and ghost direction (0 where no
-    // neighbour: the block's own padding)
-    std::vector<long long> hd(plan->nbr.size(), 0);
     for (uint32_t b = 0; b < nblocks; ++b)
       for (int d = 0; d < NNBR; ++d) {
         const int n = plan->nbr[(size_t)b * NNBR + d];
@@ -1735,20 +1759,14 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
         hd[(size_t)b * NNBR + d] = (long long)hb[n].offset - (long long)hb[b].offset -
                                    (dx * (s0 - 7) + s0 * (dy * (s1 - 7) + s1 * dz * (s2 - 7)));
       }
-    e = hipMemcpy(plan->d_delta, hd.data(), sizeof(long long) * hd.size(), hipMemcpyHostToDevice);
+    e = hipMemcpyAsync(plan->d_delta, hd.data(), sizeof(long long) * hd.size(), hipMemcpyHostToDevice, up);
   }
   if (e == hipSuccess && !hslot.empty())
-    e = hipMemcpy(plan->d_tile_slot, hslot.data(), sizeof(uint32_t) * hslot.size(), hipMemcpyHostToDevice);
+    e = hipMemcpyAsync(plan->d_tile_slot, hslot.data(), sizeof(uint32_t) * hslot.size(), hipMemcpyHostToDevice, up);
+  if (e == hipSuccess) e = hipStreamSynchronize(up);
   if (e != hipSuccess) {
     free_plan(plan);
     return hip_fail(e, "hipMemcpy(plan)");
-  }
-  e = hipStreamCreateWithFlags(&plan->aux, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&plan->ev_fork, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&plan->ev_join, hipEventDisableTiming);
-  if (e != hipSuccess) {
-    free_plan(plan);
-    return hip_fail(e, "hipStreamCreate(plan)");
   }
   const int rc = build_level_fill(plan, blocks, hb);
   if (rc != DGR_OK) {

@@ -184,7 +184,18 @@ int dgr_internal::build_level_fill(dgr_plan_t plan, const dgr_block* blocks, con
   std::vector<FillItem> items;
   std::vector<FillCopy> copies;
   plan->coarse_dirs.assign(nb, 0u);
-  for (uint32_t b = 0; b < nb; ++b) {
+  // one spacing (per group) only: no level jumps to look for
+  bool jumps = false;
+  long long lev0 = 0;
+  bool any = false;
+  for (uint32_t b = 0; b < nb && !jumps; ++b) {
+    if (!regular(hb[b])) continue;
+    const long long l = Lattice::level(hb[b].h[0]);
+    if (any && l != lev0) jumps = true;
+    lev0 = l;
+    any = true;
+  }
+  for (uint32_t b = 0; b < nb && jumps; ++b) {
     const DevBlock& B = hb[b];
     if (!regular(B)) continue;
     const int n = 13;
@@ -192,6 +203,11 @@ int dgr_internal::build_level_fill(dgr_plan_t plan, const dgr_block* blocks, con
     for (int d = 0; d < NNBR; ++d) {
       if (d == 13 || plan->nbr[(size_t)b * NNBR + d] >= 0) continue;
       const int dd[3] = {d % 3 - 1, (d / 3) % 3 - 1, d / 9 - 1};
+      // beyond a flagged (domain-boundary) face there is nothing to fill
+      bool outside = false;
+      for (int a = 0; a < 3; ++a)
+        if (dd[a] != 0 && ((B.bflag >> (2 * a + (dd[a] > 0 ? 1 : 0))) & 1u)) outside = true;
+      if (outside) continue;
       int lo[3], cnt[3];
       for (int a = 0; a < 3; ++a) {
         lo[a] = dd[a] < 0 ? 0 : (dd[a] > 0 ? n - 3 : 3);
@@ -283,16 +299,22 @@ int dgr_internal::build_level_fill(dgr_plan_t plan, const dgr_block* blocks, con
   plan->nfill_items = (uint32_t)items.size();
   plan->nfill_groups = (uint32_t)groups.size() - 1;
   plan->nfill_copies = copies.size();
+  // uploads on the plan's own stream (see dgr_plan_create_ex)
+  const hipStream_t up = plan->aux;
   if (!items.empty()) {
     DGR_HIP(hipMalloc(&plan->d_fill_items, sizeof(FillItem) * items.size()));
-    DGR_HIP(hipMemcpy(plan->d_fill_items, items.data(), sizeof(FillItem) * items.size(), hipMemcpyHostToDevice));
+    DGR_HIP(hipMemcpyAsync(plan->d_fill_items, items.data(), sizeof(FillItem) * items.size(), hipMemcpyHostToDevice,
+                           up));
     DGR_HIP(hipMalloc(&plan->d_fill_groups, sizeof(uint32_t) * groups.size()));
-    DGR_HIP(hipMemcpy(plan->d_fill_groups, groups.data(), sizeof(uint32_t) * groups.size(), hipMemcpyHostToDevice));
+    DGR_HIP(hipMemcpyAsync(plan->d_fill_groups, groups.data(), sizeof(uint32_t) * groups.size(),
+                           hipMemcpyHostToDevice, up));
   }
   if (!copies.empty()) {
     DGR_HIP(hipMalloc(&plan->d_fill_copies, sizeof(FillCopy) * copies.size()));
-    DGR_HIP(hipMemcpy(plan->d_fill_copies, copies.data(), sizeof(FillCopy) * copies.size(), hipMemcpyHostToDevice));
+    DGR_HIP(hipMemcpyAsync(plan->d_fill_copies, copies.data(), sizeof(FillCopy) * copies.size(),
+                           hipMemcpyHostToDevice, up));
   }
+  DGR_HIP(hipStreamSynchronize(up));
   return DGR_OK;
 }
 

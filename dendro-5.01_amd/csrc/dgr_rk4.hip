@@ -486,13 +486,17 @@ extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double*
     io[3].u.v[v] = stg.v[v];
     io[3].r.v[v] = kb + (size_t)(48 + v) * r->nacc;  // k_3 over k_0
   }
+  // set-up copies on the evolver's own non-blocking stream, synchronised
+  // alone (a copy on the legacy default stream could wait for unrelated work
+  // sharing its hardware queue)
   hipError_t e = hipStreamCreateWithFlags(&r->st, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMemcpy(r->d_io, io, sizeof(io), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(r->d_flag, 0, 256);
+  if (e == hipSuccess) e = hipMemcpyAsync(r->d_io, io, sizeof(io), hipMemcpyHostToDevice, r->st);
+  if (e == hipSuccess) e = hipMemsetAsync(r->d_flag, 0, 256, r->st);
   // ghosts the refill never reaches (no same-level neighbour) keep the
   // state's values in the stage buffer
   for (int v = 0; v < 24 && e == hipSuccess; ++v)
-    e = hipMemcpy(stg.v[v], r->state.v[v], nb, hipMemcpyDeviceToDevice);
+    e = hipMemcpyAsync(stg.v[v], r->state.v[v], nb, hipMemcpyDeviceToDevice, r->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(r->st);
   if (e != hipSuccess) {
     destroy(r);
     return hip_fail(e, "dgr_rk4_create");
@@ -503,6 +507,14 @@ extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double*
 
 extern "C" int dgr_rk4_create(dgr_rk4_t* out, dgr_plan_t plan, double* const* u, int use_graph) {
   return dgr_rk4_create_with_work(out, plan, u, nullptr, use_graph);
+}
+
+extern "C" int dgr_rk4_reset_stage(dgr_rk4_t r, void* stream) {
+  if (!r) return DGR_E_INVALID;
+  const hipStream_t st = (hipStream_t)stream;
+  const size_t nb = sizeof(double) * r->dof;
+  for (int v = 0; v < 24; ++v) DGR_HIP(hipMemcpyAsync(r->stg.v[v], r->state.v[v], nb, hipMemcpyDeviceToDevice, st));
+  return DGR_OK;
 }
 
 extern "C" int dgr_rk4_set_group_dt(dgr_rk4_t r, const double* dt, uint32_t ngroups) {

@@ -145,6 +145,7 @@ def lib():
         L.dgr_rk4_work_doubles.argtypes = [vp]
         L.dgr_rk4_create_with_work.argtypes = [ctypes.POINTER(vp), vp, pp, vp, ctypes.c_int]
         L.dgr_rk4_set_group_dt.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32]
+        L.dgr_rk4_reset_stage.argtypes = [vp, vp]
         L.dgr_rk4_run.argtypes = [vp, ctypes.c_int, vp]
         L.dgr_rk4_step.argtypes = [vp, ctypes.c_double, vp]
         L.dgr_rk4_steps.argtypes = [vp, ctypes.c_double, ctypes.c_int, vp]
@@ -407,6 +408,12 @@ class RK4:
 
     def run(self, nsteps=1, stream=None):
         check(lib().dgr_rk4_run(self.handle, int(nsteps), _stream_ptr(stream)), "dgr_rk4_run")
+
+    def reset_stage(self, stream=None):
+        """Copy the state into the stage buffer (after writing a new state
+        into this evolver: ghosts no neighbour fills keep the new state's
+        values)."""
+        check(lib().dgr_rk4_reset_stage(self.handle, _stream_ptr(stream)), "dgr_rk4_reset_stage")
 
     def status_ok(self):
         rc = lib().dgr_rk4_status(self.handle)

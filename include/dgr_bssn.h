@@ -257,9 +257,18 @@ int dgr_init_puncture(dgr_plan_t plan, double* const* u, const dgr_puncture* hol
 typedef struct dgr_rk4_s* dgr_rk4_t;
 int dgr_rk4_create(dgr_rk4_t* rk, dgr_plan_t plan, double* const* u, int use_graph);
 /* Same, with caller-owned work memory of at least dgr_rk4_work_doubles(plan)
- * doubles = 48 x dof + 24 x 343 x tiles (no allocation inside). */
+ * doubles = 24 x dof (the stage state) + 4 x 24 x 343 x tiles (k and the
+ * running sum in the compact tile-major layout). */
 uint64_t dgr_rk4_work_doubles(dgr_plan_t plan);
 int dgr_rk4_create_with_work(dgr_rk4_t* rk, dgr_plan_t plan, double* const* u, double* work, int use_graph);
+/* Copy the state into the stage buffer, stream-ordered on `stream`.  Ghost
+ * points that no neighbouring block fills and that lie beyond no flagged
+ * face (a concave corner of a block set that is not a full octree) are read
+ * by stages 2-4 from the stage buffer, where they keep the state's values;
+ * creation does this copy once, and a caller that writes a new state into an
+ * existing evolver (a served job reusing an evolver) calls this after it.
+ * No counterpart in the reference, whose meshes have no such ghosts. */
+int dgr_rk4_reset_stage(dgr_rk4_t rk, void* stream);
 /* Per-group time steps dt[g], g < ngroups (every group in the plan must be
  * covered); dgr_rk4_step(s) set the same dt for all groups. */
 int dgr_rk4_set_group_dt(dgr_rk4_t rk, const double* dt, uint32_t ngroups);
