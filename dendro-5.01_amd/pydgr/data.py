@@ -25,20 +25,29 @@ def coords(blocks, device="cpu"):
             runs[-1][1].append(b)
         else:
             runs.append((key, [b]))
+    # built where they are used: on a GPU only the block table crosses PCIe,
+    # from pinned memory without waiting (a blocking copy on a stream that
+    # shares a hardware queue with other work would wait for that work)
+    f64 = dict(dtype=torch.float64, device=device)
+    dev = torch.device(device)
+
+    def up(t):
+        if dev.type != "cuda":
+            return t
+        return t.pin_memory().to(dev, non_blocking=True)
     for (s0, s1, s2), bl in runs:
-        k, j, i = torch.meshgrid(torch.arange(s2, dtype=torch.float64), torch.arange(s1, dtype=torch.float64),
-                                 torch.arange(s0, dtype=torch.float64), indexing="ij")
+        k, j, i = torch.meshgrid(torch.arange(s2, **f64), torch.arange(s1, **f64), torch.arange(s0, **f64),
+                                 indexing="ij")
         i, j, k = i.reshape(1, -1), j.reshape(1, -1), k.reshape(1, -1)
-        pmin = torch.tensor([b["pmin"] for b in bl], dtype=torch.float64)
-        pmax = torch.tensor([b["pmax"] for b in bl], dtype=torch.float64)
-        h = (pmax - pmin) / torch.tensor([s0 - 1, s1 - 1, s2 - 1], dtype=torch.float64)
+        pm = up(torch.tensor([b["pmin"] + b["pmax"] for b in bl], dtype=torch.float64))
+        pmin, pmax = pm[:, 0:3], pm[:, 3:6]
+        h = torch.stack([(pmax[:, a] - pmin[:, a]) / float(n - 1) for a, n in enumerate((s0, s1, s2))], dim=1)
         xs.append((pmin[:, 0:1] + i * h[:, 0:1]).reshape(-1))
         ys.append((pmin[:, 1:2] + j * h[:, 1:2]).reshape(-1))
         zs.append((pmin[:, 2:3] + k * h[:, 2:3]).reshape(-1))
-        off = torch.tensor([b["offset"] for b in bl], dtype=torch.int64).reshape(-1, 1)
-        idx.append((off + torch.arange(s0 * s1 * s2, dtype=torch.int64).reshape(1, -1)).reshape(-1))
-    cat = lambda a: torch.cat(a).to(device)
-    return cat(xs), cat(ys), cat(zs), cat(idx)
+        off = up(torch.tensor([b["offset"] for b in bl], dtype=torch.int64)).reshape(-1, 1)
+        idx.append((off + torch.arange(s0 * s1 * s2, dtype=torch.int64, device=device).reshape(1, -1)).reshape(-1))
+    return torch.cat(xs), torch.cat(ys), torch.cat(zs), torch.cat(idx)
 
 
 def _flat(dof, device):
@@ -62,14 +71,19 @@ def flat_bumps(blocks, dof, seed=0, amp=1e-3, width=0.1, nbumps=8, box=None, dev
     else:
         lo, hi = box
     rng = np.random.default_rng(seed)
-    cen = rng.uniform(lo, hi, size=(NUM_VARS, nbumps, 3))
+    cen = torch.tensor(rng.uniform(lo, hi, size=(NUM_VARS, nbumps, 3)), dtype=torch.float64)
+    if torch.device(device).type == "cuda":
+        cen = cen.pin_memory().to(device, non_blocking=True)
     U = _flat(dof, device)
     iw2 = 1.0 / (width * width)
-    for v in range(NUM_VARS):
-        acc = torch.zeros_like(x)
-        for c in cen[v]:
-            acc += torch.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) * iw2)
-        U[v, idx] += amp * acc
+    # all variables at once, bump after bump (the same sums in the same order
+    # as one variable at a time)
+    acc = torch.zeros((NUM_VARS, x.numel()), dtype=torch.float64, device=device)
+    for b in range(nbumps):
+        c = cen[:, b, :]
+        acc += torch.exp(-((x[None, :] - c[:, 0:1]) ** 2 + (y[None, :] - c[:, 1:2]) ** 2 +
+                           (z[None, :] - c[:, 2:3]) ** 2) * iw2)
+    U[:, idx] += amp * acc
     return U
 
 

@@ -44,6 +44,40 @@ def uniform_grid(nb, ele_order=6, dx=1.0 / 64, center=True, outer_bflags=True):
     return blocks, off
 
 
+def block_grid(nblocks, ele_order=6, dx=1.0 / 64, center=True):
+    """Exactly `nblocks` blocks of one element each: the first nblocks cells
+    of an nb^3 cube (nb = ceil(nblocks^(1/3))) in x-fastest order (full
+    z-layers, then full rows of the last layer, then a partial row), like an
+    SFC-contiguous share of a cube.  Every face without a neighbouring block
+    carries its boundary flag (Sommerfeld terms, as on the cube's outer
+    faces).  Where the last layer or row ends, a block can have ghost points
+    beyond no flagged face and with no block to copy from (a concave
+    corner); they keep the values they are given (the initial data).  A
+    perfect cube gives uniform_grid(nb)."""
+    nb = 1
+    while nb ** 3 < nblocks:
+        nb += 1
+    sz = ele_order + 1 + 2 * GHOST
+    L = nb * ele_order * dx
+    o = -0.5 * L if center else 0.0
+    cells = [(bx, by, bz) for bz in range(nb) for by in range(nb) for bx in range(nb)][:nblocks]
+    have = set(cells)
+    blocks = []
+    off = 0
+    for (bx, by, bz) in cells:
+        bf = 0
+        for d in range(3):
+            for s, bit in ((-1, 2 * d), (1, 2 * d + 1)):
+                q = [bx, by, bz]
+                q[d] += s
+                if tuple(q) not in have:
+                    bf |= 1 << bit
+        corner = [o + bx * ele_order * dx, o + by * ele_order * dx, o + bz * ele_order * dx]
+        blocks.append(block_dict(off, [sz] * 3, bf, corner, dx, [ele_order] * 3))
+        off += sz ** 3
+    return blocks, off
+
+
 def mixed_blocks(nblocks, seed=0, dist=((13, 0.60), (19, 0.25), (25, 0.10), (37, 0.05)), dx=1.0 / 64,
                  box=1.0):
     """Blocks of uneven padded sizes drawn from `dist` (seeded), placed at

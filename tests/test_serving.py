@@ -28,7 +28,10 @@ def test_poisson_jobs_are_seeded():
     j1 = poisson_jobs(10, seed=3)
     j2 = poisson_jobs(10, seed=3)
     assert [(j.arrival, j.nb, j.kind) for j in j1] == [(j.arrival, j.nb, j.kind) for j in j2]
-    assert all(4 <= j.nb <= 16 for j in j1)
+    assert all(64 <= j.nblocks <= 4096 and (j.nb - 1) ** 3 < j.nblocks <= j.nb ** 3 for j in j1)
+    for j in j1[:3]:
+        blocks, _ = layout.block_grid(j.nblocks)
+        assert len(blocks) == j.nblocks
     assert all(np.diff([j.arrival for j in j1]) > 0)
 
 
@@ -39,15 +42,42 @@ def test_job_results_identical_alone_and_in_mix(cuda):
                 Job(job_id=1, arrival=0.0, nb=3, steps=2, kind="bl", mass_ratio=2.0),
                 Job(job_id=2, arrival=0.0, nb=2, steps=4, kind="bl", mass_ratio=1.3),
                 Job(job_id=3, arrival=0.0, nb=2, steps=1, kind="flat", seed=5)]
-    # budget for about two small jobs at a time: later jobs wait for memory
-    budget = 80 * 2197 * 768
+    # budget for the 27-block job and one 8-block job at a time
+    budget = 40 * 10 ** 6
     mix = mk()
-    stats = Server(budget, device=cuda).run(mix, realtime=False)
-    assert stats["jobs"] == 4 and stats["plan_rebuilds"] >= 3
+    srv = Server(budget, device=cuda)
+    stats = srv.run(mix, realtime=False)
+    srv.close()
+    # the three geometries (47.7 MB) never fit together: a job waited for
+    # memory or a free slot of another geometry was evicted
+    assert stats["jobs"] == 4 and stats["admission_waits"] + stats["slots_evicted"] >= 1, stats
     for j in mk():
-        Server(budget, device=cuda).run([j], realtime=False)
+        srv = Server(budget, device=cuda)
+        srv.run([j], realtime=False)
+        srv.close()
         m = [x for x in mix if x.job_id == j.job_id][0]
         assert torch.equal(j.result, m.result), j.job_id
+
+
+@pytest.mark.gpu
+def test_same_geometry_reuses_its_slot_and_partial_grids(cuda):
+    """Jobs of one geometry share a slot (plan and graph built once); a
+    block count that is not a cube runs as a partial cube, whose concave
+    corners keep the job's own initial values, so a job gives the same bits
+    in a reused slot as in a fresh one."""
+    jobs = [Job(job_id=i, arrival=0.0, nblocks=10, steps=2, kind="flat", seed=i) for i in range(3)]
+    # a budget for one slot of this geometry (11 MB): the jobs take turns in it
+    srv = Server(15 * 10 ** 6, device=cuda)
+    stats = srv.run(jobs, realtime=False)
+    srv.close()
+    assert stats["jobs"] == 3 and stats["slots_built"] == 1 and stats["admission_waits"] >= 1, stats
+    for j in jobs:
+        assert len(j.blocks) == 10 and torch.isfinite(j.result).all()
+    again = Job(job_id=9, arrival=0.0, nblocks=10, steps=2, kind="flat", seed=1)
+    srv = Server(10 ** 9, device=cuda)
+    srv.run([again], realtime=False)
+    srv.close()
+    assert torch.equal(again.result, jobs[1].result)
 
 
 @pytest.mark.gpu
