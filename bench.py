@@ -24,9 +24,11 @@ Also reported on the same line:
                  region; cached per device and workload shape);
   adaptive       Brill-Lindquist punctures on a 2:1-balanced octree of about
                  21000 blocks of 13^3 over 6 levels: RHS and 10 RK4 steps;
-  serving        a bounded Poisson stream (20 jobs/s) of 50-step evolutions
-                 of 64..4096 blocks under a 200 GB HBM budget: p50/p99 job
-                 latency and aggregate RHS points/s;
+  serving        Poisson streams of 200 50-step evolutions of 64..4096
+                 blocks under a 200 GB HBM budget, at 20 jobs/s (more than
+                 the GPU sustains) and at 4 jobs/s: p50/p99 job latency,
+                 aggregate RHS points/s against the uniform RK4 rate, host
+                 admission time;
   cpu_baseline   the CPU oracle timed on the host cores (all threads and one
                  thread, median of 5) on the headline grid;
   parity         per configuration (C0 64 blocks, C1 the headline grid, C2
@@ -50,6 +52,7 @@ HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
 REF_FLOP_PER_POINT = 6490   # reference deriv42 bssnrhs, bflag 0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bssn_rhs_uniform4096_traffic.txt")
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "k_bssn_rhs_uniform4096_counters.md")
 
 
 def algorithmic_bytes(blocks):
@@ -65,6 +68,18 @@ def measured_traffic():
     except OSError:
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, re.M)
+    return float(m.group(1)) if m else None
+
+
+def counted_fp64_flops():
+    """FP64 lane operations per RHS launch on the headline workload, from the
+    committed rocprofv3 counter summary (SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes
+    over the interior and boundary kernels; null if absent)."""
+    try:
+        txt = open(COUNTERS_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"^fp64_flops_per_launch:\s*([0-9.eE+]+)", txt, re.M)
     return float(m.group(1)) if m else None
 
 
@@ -349,17 +364,20 @@ def adaptive(stream, steps, autotune_on):
             "autotune": tune_rep}
 
 
-def serving(njobs, budget_gb, max_points):
+def serving(njobs, budget_gb, max_points, rate):
     import torch
     from pydgr.serving import Server, poisson_jobs
-    jobs = poisson_jobs(njobs, rate=20.0, seed=0, min_blocks=64, max_blocks=4096, steps=50)
+    jobs = poisson_jobs(njobs, rate=rate, seed=0, min_blocks=64, max_blocks=4096, steps=50)
     srv = Server(int(budget_gb * 1e9), device="cuda:0", keep_results=False, max_resident_points=max_points)
     stats = srv.run(jobs, realtime=True)
+    srv.close()
     del srv
     torch.cuda.empty_cache()
-    stats["workload"] = ("%d jobs, Poisson arrivals 20/s (seed 0), 64..4096 blocks of 13^3, 50 RK4 steps each, "
-                         "perturbed flat space or Brill-Lindquist (mass ratio 1..3), HBM budget %g GB, at most %s "
-                         "resident interior points" % (njobs, budget_gb, max_points))
+    stats["rate_jobs_per_s"] = rate
+    stats["main_loop_host_share"] = stats["main_loop_host_s"] / stats["makespan_s"]
+    stats["workload"] = ("%d jobs, Poisson arrivals %g/s (seed 0), 64..4096 blocks of 13^3 (exact counts), 50 RK4 "
+                         "steps each, perturbed flat space or Brill-Lindquist (mass ratio 1..3), HBM budget %g GB, at "
+                         "most %s running interior points" % (njobs, rate, budget_gb, max_points))
     return stats
 
 
@@ -431,7 +449,9 @@ def main():
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-adaptive", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the constraint-monitor / initial-data / RK3 timings")
-    ap.add_argument("--serving-jobs", type=int, default=24, help="0 skips the serving run")
+    ap.add_argument("--serving-jobs", type=int, default=200, help="jobs per serving run; 0 skips serving")
+    ap.add_argument("--serving-rates", type=float, nargs="+", default=[20.0, 4.0],
+                    help="Poisson arrival rates (jobs/s), one serving run each")
     ap.add_argument("--serving-budget-gb", type=float, default=200.0)
     ap.add_argument("--serving-max-points", type=int, default=3_000_000,
                     help="cap on resident interior points (0: memory budget only)")
@@ -505,6 +525,7 @@ def main():
     nbytes = algorithmic_bytes(blocks)
     achieved = nbytes / (rhs_ms * 1e-3) / 1e9
     traffic = measured_traffic()
+    fp64_counted = counted_fp64_flops()
     line = {
         "metric": "FP64 BSSN RHS points/s: batched vs per-block; p50/p99 evolve-job latency",
         "value": value,
@@ -532,7 +553,11 @@ def main():
         "pcie_inclusive": pcie,
         "fp64": {"reference_flop_per_point": REF_FLOP_PER_POINT,
                  "achieved_tflops": REF_FLOP_PER_POINT * rhs_pps / 1e12,
-                 "frac_of_peak": REF_FLOP_PER_POINT * rhs_pps / 1e12 / FP64_PEAK_TFLOPS},
+                 "frac_of_peak": REF_FLOP_PER_POINT * rhs_pps / 1e12 / FP64_PEAK_TFLOPS,
+                 "counted_flops_per_launch": fp64_counted,
+                 "counted_tflops": None if fp64_counted is None else fp64_counted / (rhs_ms * 1e-3) / 1e12,
+                 "counted_frac_of_peak": None if fp64_counted is None
+                 else fp64_counted / (rhs_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
         "per_block": None if ms_pb is None else {"ms": ms_pb, "points_per_s": pts / (ms_pb * 1e-3),
                                                  "batched_speedup": ms_pb / rhs_ms,
                                                  "rk4_graph_speedup_per_rhs": ms_pb / (ms / 4)},
@@ -546,8 +571,14 @@ def main():
         checks["C3"], line["adaptive"] = adaptive(stream, a.steps, not a.no_autotune)
         log("adaptive: RHS %.4f ms, RK4 %.4f ms" % (line["adaptive"]["rhs_ms"], line["adaptive"]["rk4_ms_per_step"]))
     if a.serving_jobs > 0:
-        line["serving"] = serving(a.serving_jobs, a.serving_budget_gb, a.serving_max_points or None)
-        log("serving: p50 %.3f s p99 %.3f s" % (line["serving"]["p50_latency_s"], line["serving"]["p99_latency_s"]))
+        line["serving"] = {}
+        for rate in a.serving_rates:
+            st = serving(a.serving_jobs, a.serving_budget_gb, a.serving_max_points or None, rate)
+            st["throughput_vs_uniform_rk4"] = st["rhs_points_per_s"] / value
+            line["serving"]["%g_per_s" % rate] = st
+            log("serving %g/s: p50 %.3f s p99 %.3f s, %.3g RHS pts/s (%.2f of uniform RK4), main-loop host time "
+                "%.1f%% of makespan" % (rate, st["p50_latency_s"], st["p99_latency_s"], st["rhs_points_per_s"],
+                                        st["throughput_vs_uniform_rk4"], 100 * st["main_loop_host_share"]))
     if not a.no_cpu:
         line["cpu_baseline"], line["parity"] = cpu_leg(blocks, U_cpu, checks, reps=a.cpu_reps)
         log("CPU oracle %.3g points/s on %d threads (%.3g on one); parity %s" % (
