@@ -50,8 +50,17 @@ def coords(blocks, device="cpu"):
     return torch.cat(xs), torch.cat(ys), torch.cat(zs), torch.cat(idx)
 
 
-def _flat(dof, device):
-    U = torch.zeros((NUM_VARS, dof), dtype=torch.float64, device=device)
+def _zeros(dof, device, out):
+    """`out` (24, dof) cleared, or a new zero array."""
+    if out is None:
+        return torch.zeros((NUM_VARS, dof), dtype=torch.float64, device=device)
+    if tuple(out.shape) != (NUM_VARS, dof) or out.dtype != torch.float64:
+        raise ValueError("out must be float64 of shape (24, %d)" % dof)
+    return out.zero_()
+
+
+def _flat(dof, device, out=None):
+    U = _zeros(dof, device, out)
     U[ALPHA] = 1.0
     U[CHI] = 1.0
     U[SG0 + 0] = 1.0
@@ -60,10 +69,11 @@ def _flat(dof, device):
     return U
 
 
-def flat_bumps(blocks, dof, seed=0, amp=1e-3, width=0.1, nbumps=8, box=None, device="cpu"):
+def flat_bumps(blocks, dof, seed=0, amp=1e-3, width=0.1, nbumps=8, box=None, device="cpu", out=None):
     """Flat space (alpha=chi=1, gt=delta, rest 0) plus, on every variable,
     the sum of `nbumps` Gaussians amp*exp(-|x-c|^2/width^2) with seeded
-    centres uniform in the bounding box of the blocks."""
+    centres uniform in the bounding box of the blocks.  Written into `out`
+    (24, dof) when given; temporaries hold a few variables at a time."""
     x, y, z, idx = coords(blocks, device)
     if box is None:
         lo = [min(b["pmin"][d] for b in blocks) for d in range(3)]
@@ -74,16 +84,19 @@ def flat_bumps(blocks, dof, seed=0, amp=1e-3, width=0.1, nbumps=8, box=None, dev
     cen = torch.tensor(rng.uniform(lo, hi, size=(NUM_VARS, nbumps, 3)), dtype=torch.float64)
     if torch.device(device).type == "cuda":
         cen = cen.pin_memory().to(device, non_blocking=True)
-    U = _flat(dof, device)
+    U = _flat(dof, device, out)
     iw2 = 1.0 / (width * width)
-    # all variables at once, bump after bump (the same sums in the same order
-    # as one variable at a time)
-    acc = torch.zeros((NUM_VARS, x.numel()), dtype=torch.float64, device=device)
-    for b in range(nbumps):
-        c = cen[:, b, :]
-        acc += torch.exp(-((x[None, :] - c[:, 0:1]) ** 2 + (y[None, :] - c[:, 1:2]) ** 2 +
-                           (z[None, :] - c[:, 2:3]) ** 2) * iw2)
-    U[:, idx] += amp * acc
+    # four variables at a time, bump after bump (the same sums in the same
+    # order as one variable at a time)
+    G = 4
+    acc = torch.empty((G, x.numel()), dtype=torch.float64, device=device)
+    for v in range(0, NUM_VARS, G):
+        acc.zero_()
+        for b in range(nbumps):
+            c = cen[v:v + G, b, :]
+            acc += torch.exp(-((x[None, :] - c[:, 0:1]) ** 2 + (y[None, :] - c[:, 1:2]) ** 2 +
+                               (z[None, :] - c[:, 2:3]) ** 2) * iw2)
+        U[v:v + G, idx] += amp * acc
     return U
 
 
@@ -119,17 +132,18 @@ def gauge_wave(blocks, dof, amp=0.01, length=None, noise=1e-6, seed=0, device="c
     return U
 
 
-def brill_lindquist(blocks, dof, m1=0.5, m2=0.5, sep=8.0, offset=(0.0137, 0.0211, 0.0173), device="cpu"):
+def brill_lindquist(blocks, dof, m1=0.5, m2=0.5, sep=8.0, offset=(0.0137, 0.0211, 0.0173), device="cpu",
+                    out=None):
     """Brill-Lindquist punctures on the x axis at +-sep/2 (shifted by
     `offset` so no grid point sits on a puncture): chi = psi^-4 with
     psi = 1 + m1/(2 r1) + m2/(2 r2), alpha = chi^(1/2), gt = delta, all
-    else zero."""
+    else zero.  Written into `out` (24, dof) when given."""
     x, y, z, idx = coords(blocks, device)
     r1 = torch.sqrt((x - 0.5 * sep - offset[0]) ** 2 + (y - offset[1]) ** 2 + (z - offset[2]) ** 2)
     r2 = torch.sqrt((x + 0.5 * sep - offset[0]) ** 2 + (y - offset[1]) ** 2 + (z - offset[2]) ** 2)
     psi = 1.0 + m1 / (2.0 * r1) + m2 / (2.0 * r2)
     chi = psi ** -4
-    U = torch.zeros((NUM_VARS, dof), dtype=torch.float64, device=device)
+    U = _zeros(dof, device, out)
     U[CHI, idx] = chi
     U[ALPHA, idx] = torch.sqrt(chi)
     U[SG0 + 0, idx] = 1.0

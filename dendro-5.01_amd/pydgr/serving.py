@@ -89,13 +89,14 @@ class Job:
             self.blocks = blocks
         return self.blocks
 
-    def initial_state(self, device):
+    def initial_state(self, device, out=None):
+        """The job's 24 initial arrays (written into `out` when given)."""
         blocks = self.build()
         if self.kind == "bl":
             q = self.mass_ratio
             m1, m2 = q / (1.0 + q), 1.0 / (1.0 + q)
-            return data.brill_lindquist(blocks, self.dof, m1=m1, m2=m2, sep=8.0, device=device)
-        return data.flat_bumps(blocks, self.dof, seed=self.seed, amp=1e-3, width=0.1, device=device)
+            return data.brill_lindquist(blocks, self.dof, m1=m1, m2=m2, sep=8.0, device=device, out=out)
+        return data.flat_bumps(blocks, self.dof, seed=self.seed, amp=1e-3, width=0.1, device=device, out=out)
 
 
 class PoolAllocator:
@@ -165,8 +166,9 @@ class Slot:
         self.last_used = 0
 
     def prepare(self, job, stream):
-        """Initial data of `job` into the slot, enqueued on `stream`, which
-        the job's steps then use."""
+        """Initial data of `job` written into the slot's state, enqueued on
+        `stream`, which the job's steps then use (the temporaries hold a few
+        variables at a time: torch caches freed memory per stream)."""
         self.job = job
         self.stream = stream
         self.steps_left = job.steps
@@ -174,7 +176,7 @@ class Slot:
             self.rk.set_group_dt([job.dt])  # stream-ordered on the evolver's stream
             self.dt = job.dt
         with torch.cuda.stream(self.stream):
-            self.U.copy_(job.initial_state(self.U.device))
+            job.initial_state(self.U.device, out=self.U)
         # ghosts no neighbour fills (concave corners of a partial cube) keep
         # this job's initial values in the stage buffer too
         self.rk.reset_stage(stream=self.stream)

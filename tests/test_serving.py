@@ -35,6 +35,18 @@ def test_poisson_jobs_are_seeded():
     assert all(np.diff([j.arrival for j in j1]) > 0)
 
 
+def test_initial_state_written_in_place_is_the_same():
+    # the server writes a job's initial data straight into its pool range
+    for kind in ("flat", "bl"):
+        j = Job(0, 0.0, nblocks=70, kind=kind, seed=5, mass_ratio=2.0)
+        ref = j.initial_state("cpu")
+        out = torch.full((24, j.dof), 3.0, dtype=torch.float64)
+        assert j.initial_state("cpu", out=out) is out
+        assert torch.equal(out, ref)
+    with pytest.raises(ValueError):
+        j.initial_state("cpu", out=torch.zeros((24, j.dof + 1), dtype=torch.float64))
+
+
 @pytest.mark.gpu
 def test_job_results_identical_alone_and_in_mix(cuda):
     def mk():
