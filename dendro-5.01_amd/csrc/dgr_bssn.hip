@@ -197,8 +197,13 @@ struct Slots {
   uint32_t valid;
 };
 
+// In gather mode sd holds the plan's distance per ghost direction (27,
+// slot 13 = the block itself).  With stage_in the input is an RK4 stage
+// buffer: a nonzero distance is taken from the compact index the point
+// would have in its own block (its source is an interior point, in the
+// buffer's compact region), a zero one keeps the block's own padding.
 template <bool GATHER>
-__device__ __forceinline__ Slots make_slots(const TileGeom& T, int t, const long long* sd) {
+__device__ __forceinline__ Slots make_slots(const TileGeom& T, int t, const long long* sd, bool stage_in) {
   Slots S;
   S.valid = 0;
 #pragma unroll
@@ -213,7 +218,11 @@ __device__ __forceinline__ Slots make_slots(const TileGeom& T, int t, const long
       const int dx = i < 3 ? 0 : (i >= T.sz0 - 3 ? 2 : 1);
       const int dy = j < 3 ? 0 : (j >= T.sz1 - 3 ? 2 : 1);
       const int dz = k < 3 ? 0 : (k >= T.sz2 - 3 ? 2 : 1);
-      o += sd[dx + 3 * dy + 9 * dz];
+      const long long d = sd[dx + 3 * dy + 9 * dz];
+      if (stage_in && d > 0)
+        o = (long long)(i - 3) + (long long)(T.sz0 - 6) * ((j - 3) + (long long)(T.sz1 - 6) * (k - 3)) + d;
+      else
+        o += d;
     }
     S.off[m] = (int32_t)o;
     if (e < NSU && a < T.rx && b < T.ry && c < T.rz) S.valid |= 1u << m;
@@ -783,7 +792,7 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
                                                  const DevTile* __restrict__ tiles, int tile_begin, int ntiles,
                                                  const IOPtrs* __restrict__ io, DevParams P,
                                                  const long long* __restrict__ delta,
-                                                 const uint32_t* __restrict__ slot) {
+                                                 const uint32_t* __restrict__ slot, int stage_in) {
   __shared__ double sstage[2][NSU];  // two staging buffers (variables alternate)
   double* const su0 = sstage[0];
   double* const su1 = sstage[1];
@@ -870,11 +879,16 @@ __global__ void __launch_bounds__(NT) k_bssn_rhs(const DevBlock* __restrict__ bl
       if (t < NNBR) s_delta[t] = delta[(size_t)tl.block * NNBR + t];
       __syncthreads();
     }
-    const Slots sl = make_slots<GATHER>(T, t, s_delta);
+    const bool from_stage = GATHER && stage_in;
+    const Slots sl = make_slots<GATHER>(T, t, s_delta, from_stage);
     double pf[NPF];
     prefetch<GATHER>(pf, io->u.v[h_order[0]], T, sl);
     KPointR s;
-    C.o = T.off + (uint64_t)C.gi + (uint64_t)T.sz0 * (C.gj + (uint64_t)T.sz1 * C.gk);
+    // the point's own values: padded index, or its compact slot in a stage buffer
+    C.o = from_stage ? T.off + (uint64_t)((long long)(C.gi - 3) +
+                                   (long long)(T.sz0 - 6) * ((C.gj - 3) + (long long)(T.sz1 - 6) * (C.gk - 3)) +
+                                   s_delta[13])
+              : T.off + (uint64_t)C.gi + (uint64_t)T.sz0 * (C.gj + (uint64_t)T.sz1 * C.gk);
     if (C.act) {
       // the state at the point, in flight with the first variable's loads
       const uint64_t o = C.o;
@@ -1210,7 +1224,7 @@ __global__ void __launch_bounds__(NT) k_physcon(const DevBlock* __restrict__ blo
     else if (t < 39) s_geo[t - 33] = B.h[t - 36];
     C.ax = s_ax;
     C.geo = s_geo;
-    const Slots sl = make_slots<false>(T, t, nullptr);
+    const Slots sl = make_slots<false>(T, t, nullptr, false);
     double pf[NPF];
     prefetch(pf, io->u.v[con_order[0]], T, sl);
     __syncthreads();
@@ -1362,6 +1376,7 @@ void free_plan(dgr_plan_s* plan) {
   (void)hipFree(plan->d_io);
   (void)hipFree(plan->d_nbr);
   (void)hipFree(plan->d_delta);
+  (void)hipFree(plan->d_cdelta);
   (void)hipFree(plan->d_fill_items);
   (void)hipFree(plan->d_fill_groups);
   (void)hipFree(plan->d_fill_copies);
@@ -1439,7 +1454,7 @@ void make_devblock(const dgr_block& bl, DevBlock& db) {
   for (int d = 0; d < 3; ++d) db.ax[d] = host_axis(db.h[d], (int)bl.sz[d], bl.bflag, d);
   db.bflag = bl.bflag;
   db.group = bl.group;
-  db.pad = 0;
+  db.cbase = 0;
 }
 
 // Same-level neighbour table: blocks of equal size and spacing whose
@@ -1701,6 +1716,16 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
   for (uint32_t b = 0; b < nblocks; ++b)
     plan->dof_end = std::max<uint64_t>(plan->dof_end, blocks[b].offset + (uint64_t)blocks[b].sz[0] * blocks[b].sz[1] *
                                                                            blocks[b].sz[2]);
+  // compact region of the RK4 stage buffers: every block's interior points,
+  // block after block
+  for (uint32_t b = 0; b < nblocks; ++b) {
+    if (plan->ncomp > 0xffffffffull) {
+      delete plan;
+      return DGR_E_INVALID;
+    }
+    hb[b].cbase = (uint32_t)plan->ncomp;
+    plan->ncomp += (uint64_t)(blocks[b].sz[0] - 6) * (blocks[b].sz[1] - 6) * (blocks[b].sz[2] - 6);
+  }
   plan->chi_floor = p->chi_floor;
   build_neighbours(blocks, hb, nblocks, plan->nbr);
   for (int i = 0; i < 4; ++i) plan->params.lambda[i] = p->lambda[i];
@@ -1724,6 +1749,7 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
       hipMalloc(&plan->d_io, sizeof(IOPtrs)) != hipSuccess ||
       hipMalloc(&plan->d_nbr, sizeof(int32_t) * plan->nbr.size()) != hipSuccess ||
       hipMalloc(&plan->d_delta, sizeof(long long) * plan->nbr.size()) != hipSuccess ||
+      hipMalloc(&plan->d_cdelta, sizeof(long long) * plan->nbr.size()) != hipSuccess ||
       hipMalloc(&plan->d_flag, 256) != hipSuccess) {
     free_plan(plan);
     return DGR_E_NOMEM;
@@ -1748,18 +1774,30 @@ extern "C" int dgr_plan_create_ex(dgr_plan_t* plan_out, const dgr_block* blocks,
   // gathering RHS: index distance from a ghost point to the same point of
   // the neighbour's interior, per block and ghost direction (0 where no
   // neighbour: the block's own padding); alive until the uploads are done
-  std::vector<long long> hd(plan->nbr.size(), 0);
+  std::vector<long long> hd(plan->nbr.size(), 0), hcd(plan->nbr.size(), 0);
   if (e == hipSuccess) {
-    for (uint32_t b = 0; b < nblocks; ++b)
+    const long long dofe = (long long)plan->dof_end;
+    for (uint32_t b = 0; b < nblocks; ++b) {
+      const long long s0 = hb[b].sz[0], s1 = hb[b].sz[1], s2 = hb[b].sz[2];
+      const long long n0 = s0 - 6, n1 = s1 - 6;
+      // the block's own interior in a stage buffer's compact region (> 0:
+      // dof_end exceeds the block's offset by its size)
+      hcd[(size_t)b * NNBR + 13] = dofe + (long long)hb[b].cbase - (long long)hb[b].offset;
       for (int d = 0; d < NNBR; ++d) {
         const int n = plan->nbr[(size_t)b * NNBR + d];
-        if (n < 0) continue;
+        if (n < 0 || d == 13) continue;
         const long long dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
-        const long long s0 = hb[b].sz[0], s1 = hb[b].sz[1], s2 = hb[b].sz[2];
         hd[(size_t)b * NNBR + d] = (long long)hb[n].offset - (long long)hb[b].offset -
                                    (dx * (s0 - 7) + s0 * (dy * (s1 - 7) + s1 * dz * (s2 - 7)));
+        // same point of the neighbour's interior, compact region; > 0, since
+        // the shift is below the block's interior size
+        hcd[(size_t)b * NNBR + d] = dofe + (long long)hb[n].cbase - (long long)hb[b].offset -
+                                    (dx * (s0 - 7) + n0 * (dy * (s1 - 7) + n1 * dz * (s2 - 7)));
       }
+    }
     e = hipMemcpyAsync(plan->d_delta, hd.data(), sizeof(long long) * hd.size(), hipMemcpyHostToDevice, up);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(plan->d_cdelta, hcd.data(), sizeof(long long) * hcd.size(), hipMemcpyHostToDevice, up);
   }
   if (e == hipSuccess && !hslot.empty())
     e = hipMemcpyAsync(plan->d_tile_slot, hslot.data(), sizeof(uint32_t) * hslot.size(), hipMemcpyHostToDevice, up);
@@ -1841,13 +1879,14 @@ RhsKernel rhs_kernel(bool bnd, int order, bool gather) {
 }
 }  // namespace
 
-int dgr_internal::launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st, bool gather) {
+int dgr_internal::launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st, bool gather, bool stage) {
+  if (stage && !gather) return DGR_E_INVALID;
   const int ni = (int)plan->ntiles_interior;
   const int nbd = (int)plan->ntiles - ni;
   auto grid = [&](int n) { return plan->grid_mult == 0 ? n : std::min(plan->grid * (int)plan->grid_mult, n); };
   const auto kin = rhs_kernel(false, plan->order, gather);
   const auto kbd = rhs_kernel(true, plan->order, gather);
-  const long long* nb = gather ? plan->d_delta : nullptr;
+  const long long* nb = gather ? (stage ? plan->d_cdelta : plan->d_delta) : nullptr;
   const uint32_t* sl = gather ? plan->d_tile_slot : nullptr;
   // both ranges present: the boundary kernel (the longer tiles, launched
   // first) runs on the plan's second stream beside the interior kernel
@@ -1860,10 +1899,10 @@ int dgr_internal::launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st
   }
   if (nbd > 0)
     hipLaunchKernelGGL(kbd, dim3(grid(nbd)), dim3(NT), 0, sb, plan->d_blocks, plan->d_tiles, ni, nbd, d_io,
-                       plan->params, nb, sl);
+                       plan->params, nb, sl, (int)stage);
   if (ni > 0)
     hipLaunchKernelGGL(kin, dim3(grid(ni)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles, 0, ni, d_io,
-                       plan->params, nb, sl);
+                       plan->params, nb, sl, (int)stage);
   DGR_HIP(hipGetLastError());
   if (fork) {
     DGR_HIP(hipEventRecord(plan->ev_join, plan->aux));
@@ -1891,7 +1930,7 @@ extern "C" int dgr_bssn_rhs_per_block(dgr_plan_t plan, double* const* rhs, const
     const int nt = (int)plan->block_tile_count[b];
     const auto k = rhs_kernel(tb >= (int)plan->ntiles_interior, plan->order, false);
     hipLaunchKernelGGL(k, dim3(nt), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, tb, nt, plan->d_io,
-                       plan->params, (const long long*)nullptr, (const uint32_t*)nullptr);
+                       plan->params, (const long long*)nullptr, (const uint32_t*)nullptr, 0);
   }
   DGR_HIP(hipGetLastError());
   return DGR_OK;
@@ -1926,10 +1965,10 @@ extern "C" int dgr_bssn_rhs_blkwise(dgr_plan_t plan, double* const* rhs, const d
   auto grid = [&](int m) { return std::min(plan->grid * 2, m); };
   if (nin > 0)
     hipLaunchKernelGGL(kin, dim3(grid((int)nin)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_sub, 0, (int)nin,
-                       plan->d_io, plan->params, (const long long*)nullptr, (const uint32_t*)nullptr);
+                       plan->d_io, plan->params, (const long long*)nullptr, (const uint32_t*)nullptr, 0);
   if (nbd > 0)
     hipLaunchKernelGGL(kbd, dim3(grid(nbd)), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_sub, (int)nin, nbd,
-                       plan->d_io, plan->params, (const long long*)nullptr, (const uint32_t*)nullptr);
+                       plan->d_io, plan->params, (const long long*)nullptr, (const uint32_t*)nullptr, 0);
   DGR_HIP(hipGetLastError());
   return DGR_OK;
 }
@@ -1958,7 +1997,7 @@ extern "C" int dgr_bssn_rhs_blk(dgr_plan_t plan, double* out, const double* in, 
   DGR_HIP(hipMemcpy(plan->d_io_blk, &io, sizeof(IOPtrs), hipMemcpyHostToDevice));
   const auto k = rhs_kernel(tb >= plan->ntiles_interior, plan->order, false);
   hipLaunchKernelGGL(k, dim3(tc), dim3(NT), 0, st, plan->d_blocks, plan->d_tiles_blk, (int)tb, (int)tc,
-                     plan->d_io_blk, plan->params, (const long long*)nullptr, (const uint32_t*)nullptr);
+                     plan->d_io_blk, plan->params, (const long long*)nullptr, (const uint32_t*)nullptr, 0);
   DGR_HIP(hipGetLastError());
   return DGR_OK;
 }
@@ -2261,10 +2300,10 @@ extern "C" int dgr_bssn_rhs_blocks_noalloc(double* const* rhs, const double* con
   auto grid = [&](int n) { return std::min(std::max(g_na.ncu[dev], 1), n); };
   if (ni > 0)
     hipLaunchKernelGGL(rhs_kernel(false, order, false), dim3(grid((int)ni)), dim3(NT), 0, st, d_blocks, d_tiles, 0,
-                       (int)ni, d_io, P, nullptr, nullptr);
+                       (int)ni, d_io, P, nullptr, nullptr, 0);
   if (nbd > 0)
     hipLaunchKernelGGL(rhs_kernel(true, order, false), dim3(grid(nbd)), dim3(NT), 0, st, d_blocks, d_tiles,
-                       (int)ni, nbd, d_io, P, nullptr, nullptr);
+                       (int)ni, nbd, d_io, P, nullptr, nullptr, 0);
   DGR_HIP(hipGetLastError());
   return DGR_OK;
 }

@@ -29,7 +29,7 @@ struct DevBlock {
   double h[3];
   AxisC ax[3];
   uint32_t group;
-  uint32_t pad;
+  uint32_t cbase;  // first slot of the block's interior points in a compact region
 };
 
 struct DevTile {
@@ -69,12 +69,14 @@ struct FillItem {
   uint8_t n[3];        // box extent per axis (3 or sz-6)
   uint8_t s0[3];       // position of the box's first point in the coarse element, in half node spacings
   uint8_t pad[2];
+  uint64_t csrc;       // src in a stage buffer: its slot in the compact interior region
 };
 
 // Ghost fill across a level jump, coarse side: every ghost point coincides
 // with a node of a finer block and copies it.
 struct FillCopy {
   uint64_t dst, src;
+  uint64_t csrc;  // src in a stage buffer: its slot in the compact interior region
 };
 
 
@@ -113,6 +115,16 @@ struct dgr_plan_s {
   bool io_valid = false;
   int32_t* d_nbr = nullptr;              // NNBR neighbour slots per block
   long long* d_delta = nullptr;          // NNBR gather distances per block (gathering RHS)
+  // Stage buffers of the RK4 evolver: per variable the padded layout
+  // (dof_end points; only its ghosts are read) followed by a compact region
+  // of every block's interior points, block after block (block b's from
+  // DevBlock::cbase, x fastest).  d_cdelta: per block and direction, the
+  // distance from the compact index a point would have in its own block
+  // to its source in such a buffer (the neighbour's compact slot, the
+  // block's own for direction 13), or 0 where the source is the block's
+  // own padding.
+  uint64_t ncomp = 0;
+  long long* d_cdelta = nullptr;
   std::vector<int32_t> nbr;              // host copy of the neighbour table
   uint64_t dof_end = 0;                  // one past the last padded point of any block
   double chi_floor = 0.1;                // CHI_FLOOR of the constraint projection
@@ -161,12 +173,16 @@ namespace dgr_internal {
 // the right-hand side is written in the compact tile-major layout of the
 // stage kernels: variable v of interior point (a, b, c) of block-order tile
 // w at rhs[v][w * 343 + a + 7 (b + 7 c)].
-int launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st, bool gather);
+// With stage, the inputs are stage buffers (interior points read from the
+// compact region; gather only).
+int launch_rhs(dgr_plan_t plan, const IOPtrs* d_io, hipStream_t st, bool gather, bool stage = false);
 
 // Find the level-jump ghost fills of a plan's blocks and upload them
 // (plan->d_fill_*); blocks without a same-level neighbour in a direction are
 // looked up one level coarser and one level finer.
 int build_level_fill(dgr_plan_t plan, const dgr_block* blocks, const std::vector<DevBlock>& hb);
-// Fill the level-jump ghosts of the 24 arrays u (asynchronous on st).
-int launch_level_fill(dgr_plan_t plan, const OutPtrs& u, hipStream_t st);
+// Fill the level-jump ghosts of the 24 arrays u (asynchronous on st); with
+// stage, u are stage buffers whose interior points are read from the
+// compact region.
+int launch_level_fill(dgr_plan_t plan, const OutPtrs& u, hipStream_t st, bool stage = false);
 }  // namespace dgr_internal

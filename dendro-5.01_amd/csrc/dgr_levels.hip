@@ -69,7 +69,7 @@ constexpr int VB = 4;
 // of the element once and interpolates every box of the group from it.
 __global__ void __launch_bounds__(NF) k_fill_interp(const FillItem* __restrict__ items,
                                                      const uint32_t* __restrict__ groups, uint32_t ngroups,
-                                                     OutPtrs u) {
+                                                     OutPtrs u, int stage) {
   __shared__ double W[13][7];
   __shared__ double sC[VB][343], sX[VB][343], sY[VB][343];
   const int t = threadIdx.x;
@@ -77,12 +77,15 @@ __global__ void __launch_bounds__(NF) k_fill_interp(const FillItem* __restrict__
   for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
     const uint32_t i0 = groups[g], i1 = groups[g + 1];
     const FillItem E = items[i0];  // src and ssz are the group's
+    // the element's nodes: padded layout, or a stage buffer's compact region
+    const uint64_t es = stage ? E.csrc : E.src;
+    const uint64_t e0 = stage ? E.ssz[0] - 6 : E.ssz[0], e1 = stage ? E.ssz[1] - 6 : E.ssz[1];
     for (int v0 = 0; v0 < 24; v0 += VB) {
       __syncthreads();  // previous pass (or group) done with sC, sX, sY; W visible
       for (int e = t; e < VB * 343; e += NF) {
         const int w = e / 343, m = e % 343;
-        const gdouble* src = (const gdouble*)u.v[v0 + w] + E.src;
-        sC[w][m] = src[(m % 7) + (uint64_t)E.ssz[0] * ((m / 7) % 7 + (uint64_t)E.ssz[1] * (m / 49))];
+        const gdouble* src = (const gdouble*)u.v[v0 + w] + es;
+        sC[w][m] = src[(m % 7) + e0 * ((m / 7) % 7 + e1 * (m / 49))];
       }
       __syncthreads();
       for (uint32_t it = i0; it < i1; ++it) {
@@ -129,12 +132,14 @@ __global__ void __launch_bounds__(NF) k_fill_interp(const FillItem* __restrict__
   }
 }
 
-__global__ void __launch_bounds__(256) k_fill_copy(const FillCopy* __restrict__ c, uint64_t n, OutPtrs u) {
+__global__ void __launch_bounds__(256) k_fill_copy(const FillCopy* __restrict__ c, uint64_t n, OutPtrs u,
+                                                   int stage) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const FillCopy C = c[i];
+    const uint64_t src = stage ? C.csrc : C.src;  // a fine block's interior node
     double x[24];  // all loads before the stores
 #pragma unroll
-    for (int v = 0; v < 24; ++v) x[v] = ((const gdouble*)u.v[v])[C.src];
+    for (int v = 0; v < 24; ++v) x[v] = ((const gdouble*)u.v[v])[src];
 #pragma unroll
     for (int v = 0; v < 24; ++v) ((gdouble*)u.v[v])[C.dst] = x[v];
   }
@@ -239,6 +244,7 @@ int dgr_internal::build_level_fill(dgr_plan_t plan, const dgr_block* blocks, con
         if (!ok) continue;
         it.dst = B.offset + lo[0] + (uint64_t)B.sz[0] * (lo[1] + (uint64_t)B.sz[1] * lo[2]);
         it.src = C.offset + 3 + (uint64_t)C.sz[0] * (3 + (uint64_t)C.sz[1] * 3);
+        it.csrc = plan->dof_end + C.cbase;  // node (3,3,3): the block's first interior point
         items.push_back(it);
         plan->coarse_dirs[b] |= 1u << d;
         continue;
@@ -270,18 +276,20 @@ int dgr_internal::build_level_fill(dgr_plan_t plan, const dgr_block* blocks, con
             }
             if (F < 0) continue;
             const DevBlock& FB = hb[F];
-            uint64_t src = FB.offset;
-            uint64_t stride = 1;
+            uint64_t src = FB.offset, csrc = plan->dof_end + FB.cbase;
+            uint64_t stride = 1, cstride = 1;
             bool ok = true;
             for (int a = 0; a < 3; ++a) {
               const double xi = (x[a] - (FB.pmin[a] + 3 * FB.h[a])) / FB.h[a];
               const long long xr = llround(xi);
               ok = ok && std::fabs(xi - xr) < 1e-6 && xr >= 0 && xr <= 6;
               src += (uint64_t)(3 + xr) * stride;
+              csrc += (uint64_t)xr * cstride;
               stride *= FB.sz[a];
+              cstride *= FB.sz[a] - 6;
             }
             if (!ok) continue;
-            copies.push_back({B.offset + i + (uint64_t)n * (j + (uint64_t)n * k), src});
+            copies.push_back({B.offset + i + (uint64_t)n * (j + (uint64_t)n * k), src, csrc});
           }
     }
   }
@@ -318,16 +326,16 @@ int dgr_internal::build_level_fill(dgr_plan_t plan, const dgr_block* blocks, con
   return DGR_OK;
 }
 
-int dgr_internal::launch_level_fill(dgr_plan_t plan, const OutPtrs& u, hipStream_t st) {
+int dgr_internal::launch_level_fill(dgr_plan_t plan, const OutPtrs& u, hipStream_t st, bool stage) {
   if (plan->nfill_copies) {
     const uint64_t g = std::min<uint64_t>((plan->nfill_copies + 255) / 256, 65536);
     hipLaunchKernelGGL(k_fill_copy, dim3((unsigned)g), dim3(256), 0, st, plan->d_fill_copies, plan->nfill_copies,
-                       u);
+                       u, (int)stage);
   }
   if (plan->nfill_items) {
     const uint32_t g = std::min<uint32_t>(plan->nfill_groups, 16384);
     hipLaunchKernelGGL(k_fill_interp, dim3(g), dim3(NF), 0, st, plan->d_fill_items, plan->d_fill_groups,
-                       plan->nfill_groups, u);
+                       plan->nfill_groups, u, (int)stage);
   }
   DGR_HIP(hipGetLastError());
   return DGR_OK;

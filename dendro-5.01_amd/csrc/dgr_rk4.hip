@@ -62,8 +62,9 @@ struct Ptrs24 {
 // sum: stage 1 forms it from k_0 and the last stage adds k_2 before k_3,
 // the same operations in the same order.
 struct Work {
-  double* base;   // stage state (24 x dof, padded layout)
+  double* base;   // stage state: per variable dof padded points, then the compact interior region
   uint64_t dof;
+  uint64_t sstride;  // doubles per variable of the stage state (dof + compact region)
   double* kc;     // k of this stage, compact: 24 x nacc
   double* acc;    // running sum, compact: 24 x nacc
   uint64_t nacc;
@@ -79,8 +80,8 @@ struct WorkCursor {
   double* q;        // stage state of variable v at padded point o
   __device__ __forceinline__ WorkCursor(const Work& W, uint64_t o) : q(W.base + o) {}
   __device__ __forceinline__ gdouble& stg() { return ((gdouble*)q)[0]; }
-  __device__ __forceinline__ void next(uint64_t dof) {
-    q += dof;
+  __device__ __forceinline__ void next(uint64_t n) {
+    q += n;
     asm volatile("" : "+v"(q));
   }
 };
@@ -133,12 +134,16 @@ __device__ __forceinline__ int enforce_point(double (&u)[24], double chi_floor) 
   return bad;
 }
 
-// Interior point of a tile handled by thread t (plain row-major order).
-__device__ __forceinline__ bool tile_point(const DevTile& tl, const DevBlock& B, int t, uint64_t& o) {
+// Interior point of a tile handled by thread t (plain row-major order):
+// its padded index o and, if asked, its slot in the compact interior region
+// of a stage buffer (block after block, x fastest).
+__device__ __forceinline__ bool tile_point(const DevTile& tl, const DevBlock& B, int t, uint64_t& o,
+                                           uint64_t* oc = nullptr) {
   const int ti = t % 7, tj = (t / 7) % 7, tk = t / 49;
   if (t >= 343 || ti >= tl.nx || tj >= tl.ny || tk >= tl.nz) return false;
   const uint64_t i = tl.i0 + ti, j = tl.j0 + tj, k = tl.k0 + tk;
   o = B.offset + i + (uint64_t)B.sz[0] * (j + (uint64_t)B.sz[1] * k);
+  if (oc) *oc = B.cbase + (i - 3) + (uint64_t)(B.sz[0] - 6) * ((j - 3) + (uint64_t)(B.sz[1] - 6) * (k - 3));
   return true;
 }
 
@@ -154,7 +159,10 @@ __constant__ double c_rkC[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
 // or the state after the last stage).  Every operand of a point is loaded
 // before the first store.  The stores are non-temporal: no kernel reads
 // them back before the whole stage is written, and they would only evict
-// the inputs still to be read.
+// the inputs still to be read.  The stage state goes to the compact region
+// of the stage buffer (whole lines; the padded layout's 7-point row
+// fragments cost a third of the stage's time), which the next RHS and the
+// level-jump fills read.
 template <int KIND>
 __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ blocks, const DevTile* __restrict__ tiles,
                                               int ntiles, Ptrs24 state, Work W, const double* __restrict__ gdt,
@@ -164,8 +172,8 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
     if (w >= ntiles) return;
     const DevTile tl = tiles[w];
     const DevBlock& B = blocks[tl.block];
-    uint64_t o;
-    if (!tile_point(tl, B, threadIdx.x, o)) return;
+    uint64_t o, oc;
+    if (!tile_point(tl, B, threadIdx.x, o, &oc)) return;
     const double dt = gdt[B.group];
     const uint64_t co = (uint64_t)w * 343 + threadIdx.x;
     double un[24], kk[24];
@@ -244,11 +252,11 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
 #pragma unroll
       for (int v = 0; v < 24; ++v) __builtin_nontemporal_store(un[v], (gdouble*)&state.v[v][o]);
     } else {
-      WorkCursor d(W, o);
+      WorkCursor d(W, W.dof + oc);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
         __builtin_nontemporal_store(un[v], &d.stg());
-        d.next(W.dof);
+        d.next(W.sstride);
       }
     }
   }
@@ -315,7 +323,8 @@ int grid_for(dgr_plan_t plan, int n) {
 struct dgr_rk4_s {
   dgr_plan_t plan = nullptr;
   uint64_t dof = 0;
-  double* buf = nullptr;        // stage state (24 x dof), k, running sum, k_0 (24 x 343 per tile each)
+  double* buf = nullptr;        // stage state (24 x sstride), k, running sum, k_0 (24 x 343 per tile each)
+  uint64_t sstride = 0;         // stage state per variable: dof padded points, then the compact interior region
   Ptrs24 state{}, stg{}, k{};  // the state and the work arrays in buf
   uint64_t nacc = 0;           // compact running-sum slots per variable (343 per tile)
   IOPtrs* d_io = nullptr;       // RHS pointer table of each stage (input -> its k array)
@@ -338,14 +347,14 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
   for (int s = 0; s < 4; ++s) {
     // k_s = RHS(u_s): stage 0 reads the state, later stages the stage
     // buffer; k_s goes to the array of table s (k_1, k_0 / k_3, k_2)
-    int rc = launch_rhs(p, r->d_io + s, st, true);
+    int rc = launch_rhs(p, r->d_io + s, st, true, s > 0);
     if (rc) return rc;
     // stage combine + projection; the next RHS reads ghosts from the
     // neighbours' interiors (gather), so no refill runs between stages
-    double* const c0 = r->buf + 24 * r->dof;
+    double* const c0 = r->buf + 24 * r->sstride;
     double* const k1 = c0, *const acc = c0 + 24 * r->nacc, *const k03 = c0 + 48 * r->nacc, *const k2 = c0 + 72 * r->nacc;
     double* const kc = s == 0 || s == 3 ? k03 : (s == 1 ? k1 : k2);
-    const Work W = {r->buf, r->dof, kc, acc, r->nacc, s == 1 ? k03 : k2};
+    const Work W = {r->buf, r->dof, r->sstride, kc, acc, r->nacc, s == 1 ? k03 : k2};
     const dim3 g(std::max(nt, 1)), b(384);
     if (s == 0)
       hipLaunchKernelGGL(k_stage<ST_FIRST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
@@ -362,7 +371,7 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
     // ghosts across level jumps (adaptive layouts) of the new stage state
     OutPtrs o;
     for (int v = 0; v < 24; ++v) o.v[v] = s < 3 ? r->stg.v[v] : r->state.v[v];
-    rc = launch_level_fill(p, o, st);
+    rc = launch_level_fill(p, o, st, s < 3);
     if (rc) return rc;
   }
   DGR_HIP(hipGetLastError());
@@ -441,10 +450,11 @@ extern "C" int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* 
   return flag ? DGR_E_CONSTRAINT : DGR_OK;
 }
 
-// stage state in the padded layout; k_1, the running sum, k_0 / k_3 and
-// k_2 compact
+// stage state (per variable the padded layout, whose ghosts the level fills
+// and the reset write, then the interior points in the compact region the
+// stage kernels write); k_1, the running sum, k_0 / k_3 and k_2 compact
 extern "C" uint64_t dgr_rk4_work_doubles(dgr_plan_t plan) {
-  return plan ? 24ull * plan->dof_end + 96ull * 343ull * plan->ntiles : 0;
+  return plan ? 24ull * (plan->dof_end + plan->ncomp) + 96ull * 343ull * plan->ntiles : 0;
 }
 
 extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double* const* u, double* work,
@@ -453,6 +463,7 @@ extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double*
   dgr_rk4_s* r = new dgr_rk4_s();
   r->plan = plan;
   r->dof = plan->dof_end;
+  r->sstride = plan->dof_end + plan->ncomp;
   r->use_graph = use_graph != 0;
   r->h_gdt.assign(plan->ngroups, 0.0);
   for (int v = 0; v < 24; ++v) r->state.v[v] = u[v];
@@ -469,13 +480,13 @@ extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double*
   Ptrs24& stg = r->stg;
   Ptrs24& k = r->k;
   for (int v = 0; v < 24; ++v) {
-    stg.v[v] = r->buf + (size_t)v * r->dof;
-    k.v[v] = r->buf + (size_t)24 * r->dof + (size_t)v * r->nacc;
+    stg.v[v] = r->buf + (size_t)v * r->sstride;
+    k.v[v] = r->buf + (size_t)24 * r->sstride + (size_t)v * r->nacc;
   }
   // RHS of stage s: input (the state, then the stage buffer) -> k_1,
   // k_0 / k_3 or k_2 (compact arrays after the stage buffer)
   IOPtrs io[4];
-  double* const kb = r->buf + (size_t)24 * r->dof;
+  double* const kb = r->buf + (size_t)24 * r->sstride;
   for (int v = 0; v < 24; ++v) {
     io[0].u.v[v] = r->state.v[v];
     io[0].r.v[v] = kb + (size_t)(48 + v) * r->nacc;  // k_0

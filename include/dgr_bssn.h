@@ -257,8 +257,9 @@ int dgr_init_puncture(dgr_plan_t plan, double* const* u, const dgr_puncture* hol
 typedef struct dgr_rk4_s* dgr_rk4_t;
 int dgr_rk4_create(dgr_rk4_t* rk, dgr_plan_t plan, double* const* u, int use_graph);
 /* Same, with caller-owned work memory of at least dgr_rk4_work_doubles(plan)
- * doubles = 24 x dof (the stage state) + 4 x 24 x 343 x tiles (k and the
- * running sum in the compact tile-major layout). */
+ * doubles = 24 x (dof + interior points) (the stage state: per variable the
+ * padded layout, then every block's interior points compact) + 4 x 24 x 343
+ * x tiles (k and the running sum in the compact tile-major layout). */
 uint64_t dgr_rk4_work_doubles(dgr_plan_t plan);
 int dgr_rk4_create_with_work(dgr_rk4_t* rk, dgr_plan_t plan, double* const* u, double* work, int use_graph);
 /* Copy the state into the stage buffer, stream-ordered on `stream`.  Ghost
