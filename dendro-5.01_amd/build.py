@@ -28,15 +28,40 @@ def run(cmd, cwd=None):
     subprocess.check_call(cmd, cwd=cwd)
 
 
+SOURCES = ["csrc/dgr_bssn.hip", "csrc/dgr_rk4.hip", "csrc/dgr_levels.hip", "csrc/dgr_init.hip",
+           "csrc/dgr_tp.hip", "host/bssn_rhs_host.cpp"]
+# the dense TwoPunctures Jacobian is factored with rocSOLVER (on rocBLAS)
+LINK = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
+
+
 def build_hip():
+    """Compile every source (in parallel) and link libdgr_bssn.so.  With
+    DGR_BUILD_INCREMENTAL=1 in the environment, objects newer than their
+    source and every header are kept (development only)."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIB, exist_ok=True)
+    obj_dir = os.path.join(LIB, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    incremental = os.environ.get("DGR_BUILD_INCREMENTAL") == "1"
+    headers = [os.path.join(d, f) for d in (os.path.join(PKG, "csrc"), os.path.join(ROOT, "include"))
+               for f in os.listdir(d) if f.endswith(".h")]
+    newest_header = max(os.path.getmtime(h) for h in headers)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+
+    def compile_one(rel):
+        src = os.path.join(PKG, rel)
+        obj = os.path.join(obj_dir, os.path.basename(rel) + ".o")
+        if incremental and os.path.exists(obj) and os.path.getmtime(obj) > max(os.path.getmtime(src), newest_header):
+            return obj
+        run([HIPCC] + compile_flags + ["-c", "-o", obj + ".tmp", src])
+        os.replace(obj + ".tmp", obj)
+        return obj
+
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
     out = os.path.join(LIB, "libdgr_bssn.so")
     tmp = out + ".tmp"
-    run([HIPCC] + HIP_FLAGS + ["-o", tmp, os.path.join(PKG, "csrc", "dgr_bssn.hip"),
-                               os.path.join(PKG, "csrc", "dgr_rk4.hip"),
-                               os.path.join(PKG, "csrc", "dgr_levels.hip"),
-                               os.path.join(PKG, "csrc", "dgr_init.hip"),
-                               os.path.join(PKG, "host", "bssn_rhs_host.cpp")])
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + LINK)
     os.replace(tmp, out)
     return out
 

@@ -128,6 +128,14 @@ def lib():
         L.dgr_enforce_constraints.argtypes = [vp, pp, vp]
         L.dgr_physcon.argtypes = [vp, pp, pp, vp]
         L.dgr_init_puncture.argtypes = [vp, pp, ctypes.POINTER(Puncture), vp]
+        L.dgr_tp_params_default.argtypes = [ctypes.POINTER(TPParams)]
+        L.dgr_tp_params_default.restype = None
+        L.dgr_tp_solve.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(TPParams)]
+        L.dgr_tp_info.argtypes = [vp, ctypes.POINTER(TPInfo)]
+        L.dgr_tp_collocation.argtypes = [vp, vp, ctypes.c_uint64]
+        L.dgr_tp_u.argtypes = [vp, vp, vp, ctypes.c_uint64]
+        L.dgr_tp_evaluate.argtypes = [vp, vp, pp, vp]
+        L.dgr_tp_destroy.argtypes = [vp]
         L.dgr_bssn_rhs_blkwise.argtypes = [vp, pp, pp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, vp]
         L.dgr_bssn_rhs_blk.argtypes = [vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp]
         L.dgr_ets_create.argtypes = [ctypes.POINTER(vp), vp, pp, ctypes.c_int]
@@ -188,6 +196,96 @@ def punctures(holes):
         arr[q].spin_theta = h.get("spin_theta", 0.0)
         arr[q].spin_phi = h.get("spin_phi", 0.0)
     return arr
+
+
+class TPParams(ctypes.Structure):
+    """dgr_tp_params (include/dgr_bssn.h): the reference's TPID parameters."""
+    _v3 = ctypes.c_double * 3
+    _fields_ = [("par_b", ctypes.c_double), ("par_m_plus", ctypes.c_double), ("par_m_minus", ctypes.c_double),
+                ("target_M_plus", ctypes.c_double), ("target_M_minus", ctypes.c_double),
+                ("par_P_plus", _v3), ("par_P_minus", _v3), ("par_S_plus", _v3), ("par_S_minus", _v3),
+                ("center_offset", _v3), ("give_bare_mass", ctypes.c_int), ("npoints_A", ctypes.c_int),
+                ("npoints_B", ctypes.c_int), ("npoints_phi", ctypes.c_int), ("Newton_maxit", ctypes.c_int),
+                ("initial_lapse", ctypes.c_int), ("Newton_tol", ctypes.c_double), ("adm_tol", ctypes.c_double),
+                ("initial_lapse_psi_exponent", ctypes.c_double), ("TP_epsilon", ctypes.c_double),
+                ("TP_Tiny", ctypes.c_double), ("TP_Extend_Radius", ctypes.c_double)]
+
+
+class TPInfo(ctypes.Structure):
+    """dgr_tp_info_t."""
+    _fields_ = [("mp", ctypes.c_double), ("mm", ctypes.c_double), ("mp_adm", ctypes.c_double),
+                ("mm_adm", ctypes.c_double), ("E", ctypes.c_double), ("J", ctypes.c_double * 3),
+                ("newton_steps", ctypes.c_int), ("last_update", ctypes.c_double)]
+
+
+TP_LAPSE = {"antisymmetric": 0, "averaged": 1, "psin": 2, "brownsville": 3, "unit": 4}
+
+
+def tp_params(**kw):
+    """dgr_tp_params: the reference's defaults (dgr_tp_params_default) with
+    the given fields replaced (3-vectors as sequences; initial_lapse also as
+    a name of TP_LAPSE)."""
+    p = TPParams()
+    lib().dgr_tp_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        if k == "initial_lapse" and isinstance(v, str):
+            v = TP_LAPSE[v]
+        cur = getattr(p, k)
+        if isinstance(cur, ctypes.Array):
+            for d in range(3):
+                cur[d] = float(v[d])
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class TwoPunctures:
+    """TwoPunctures initial data (dgr_tp_*): the spectral solve on
+    construction, then evaluate(plan, u) fills every padded point."""
+
+    def __init__(self, params=None, **kw):
+        self.params = params if params is not None else tp_params(**kw)
+        h = ctypes.c_void_p()
+        check(lib().dgr_tp_solve(ctypes.byref(h), ctypes.byref(self.params)), "dgr_tp_solve")
+        self.handle = h
+
+    def info(self):
+        i = TPInfo()
+        check(lib().dgr_tp_info(self.handle, ctypes.byref(i)), "dgr_tp_info")
+        return {"mp": i.mp, "mm": i.mm, "mp_adm": i.mp_adm, "mm_adm": i.mm_adm, "E": i.E, "J": list(i.J),
+                "newton_steps": i.newton_steps, "last_update": i.last_update}
+
+    def collocation(self):
+        import numpy as np
+        p = self.params
+        out = np.empty(p.npoints_A * p.npoints_B * p.npoints_phi)
+        check(lib().dgr_tp_collocation(self.handle, out.ctypes.data_as(ctypes.c_void_p), out.size),
+              "dgr_tp_collocation")
+        return out
+
+    def u_at(self, xyz):
+        """u at absolute coordinates (n, 3), evaluated on the host."""
+        import numpy as np
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.empty(len(xyz))
+        check(lib().dgr_tp_u(self.handle, xyz.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
+                             len(xyz)), "dgr_tp_u")
+        return out
+
+    def evaluate(self, plan, u, stream=None):
+        up, _u = ptr_array(var_ptrs(u))
+        check(lib().dgr_tp_evaluate(self.handle, plan.handle, up, _stream_ptr(stream)), "dgr_tp_evaluate")
+
+    def close(self):
+        if self.handle:
+            lib().dgr_tp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 NUM_CONSTRAINTS = 6

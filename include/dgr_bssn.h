@@ -242,6 +242,65 @@ typedef struct {
  * Asynchronous on stream. */
 int dgr_init_puncture(dgr_plan_t plan, double* const* u, const dgr_puncture* holes, void* stream);
 
+/* TwoPunctures initial data (BSSN_GR/src/TwoPunctures.cpp:24-498, solver in
+ * TPNewton.cpp / TPFuncAndJacobian.cpp / TPCoordTransf.cpp / TPEquations.cpp).
+ * The reference's TwoPunctures(x, y, z, vars, &mp, &mm, &mp_adm, &mm_adm, &E,
+ * &J1, &J2, &J3) solves on its first call and then fills one point per call;
+ * here the solve is dgr_tp_solve and the fill of every padded point of a
+ * plan's blocks is dgr_tp_evaluate.  Field names are the reference's TPID
+ * parameters (BSSN_GR/include/parameters.h:262-310); dgr_tp_params_default
+ * gives its defaults.  Punctures sit at center_offset +- (par_b, 0, 0).
+ * Always evaluated from the spectral coefficients (the reference's
+ * grid_setup_method EVALUATION); no multiply_old_lapse. */
+enum {
+  DGR_TP_LAPSE_ANTISYMMETRIC = 0, /* (1 - m+/2r+ - m-/2r-) / (1 + m+/2r+ + m-/2r-) */
+  DGR_TP_LAPSE_AVERAGED = 1,      /* (1 + antisymmetric) / 2 */
+  DGR_TP_LAPSE_PSIN = 2,          /* psi_BL^initial_lapse_psi_exponent (the reference's default) */
+  DGR_TP_LAPSE_BROWNSVILLE = 3,   /* 2 / (1 + psi_BL^exponent) */
+  DGR_TP_LAPSE_UNIT = 4           /* 1 */
+};
+#define DGR_TP_MAX_UNKNOWNS 40000 /* npoints_A x npoints_B x npoints_phi (a dense N x N Jacobian in HBM) */
+typedef struct {
+  double par_b;                           /* punctures at x = +-par_b */
+  double par_m_plus, par_m_minus;         /* bare masses (initial guesses unless give_bare_mass) */
+  double target_M_plus, target_M_minus;   /* ADM masses sought when give_bare_mass == 0 */
+  double par_P_plus[3], par_P_minus[3];   /* Bowen-York momenta */
+  double par_S_plus[3], par_S_minus[3];   /* Bowen-York spins */
+  double center_offset[3];
+  int give_bare_mass;
+  int npoints_A, npoints_B, npoints_phi;  /* spectral resolution: A, B <= 64 / 256, phi <= 256 */
+  int Newton_maxit;
+  int initial_lapse;                      /* DGR_TP_LAPSE_* */
+  double Newton_tol;                      /* stop when the largest Newton update is below it */
+  double adm_tol;
+  double initial_lapse_psi_exponent;
+  double TP_epsilon, TP_Tiny, TP_Extend_Radius;
+} dgr_tp_params;
+typedef struct {
+  double mp, mm;            /* bare masses used */
+  double mp_adm, mm_adm;    /* puncture ADM masses */
+  double E;                 /* total ADM mass */
+  double J[3];              /* ADM angular momentum */
+  int newton_steps;         /* Newton steps taken (all bare-mass iterations) */
+  double last_update;       /* largest change of the last step */
+} dgr_tp_info_t;
+typedef struct dgr_tp_s* dgr_tp_t;
+void dgr_tp_params_default(dgr_tp_params* p);
+/* The spectral solve (synchronous; uses the current device and about
+ * 8 N^2 bytes of device memory while it runs). */
+int dgr_tp_solve(dgr_tp_t* tp, const dgr_tp_params* p);
+int dgr_tp_info(dgr_tp_t tp, dgr_tp_info_t* info);
+/* The solution V (u = (A - 1) V) at the collocation points, index
+ * i + npoints_A (j + npoints_B k); n >= npoints_A npoints_B npoints_phi. */
+int dgr_tp_collocation(dgr_tp_t tp, double* values, uint64_t n);
+/* u at n points (xyz: 3 n host doubles, absolute coordinates), on the host. */
+int dgr_tp_u(dgr_tp_t tp, const double* xyz, double* out, uint64_t n);
+/* The 24 BSSN variables at every padded point of every block (x = pmin +
+ * i h): chi, gt, At, K from psi^4 delta and the Bowen-York curvature, the
+ * lapse of initial_lapse, beta = Gt = B = 0.  Asynchronous on stream. */
+int dgr_tp_evaluate(dgr_tp_t tp, dgr_plan_t plan, double* const* u, void* stream);
+int dgr_tp_destroy(dgr_tp_t tp);
+
 /* Device-resident RK4 evolution of the 24 state arrays u (caller-owned
  * device memory, ghost layers filled).  One step replaces the RK4 branch of
  * RK_BSSN::performSingleIteration (BSSN_GR/src/rkBSSN.cpp:803-1011):
