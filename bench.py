@@ -384,10 +384,14 @@ def serving(njobs, budget_gb, max_points, rate):
 def extras(plan, blocks, U, stream, steps, warm):
     """Timings of the entry points around the RHS on the same grid: the
     constraint monitor (dgr_physcon + masked norms), puncture initial data
-    (dgr_init_puncture), one RK3 step of the Butcher-tableau stepper and the
-    RHS with the 6th-order stencils (deriv64, default schedule)."""
+    (dgr_init_puncture), one RK3 step of the Butcher-tableau stepper, the
+    RHS with the 6th-order stencils (deriv64, default schedule), and
+    TwoPunctures initial data with the parameters of the reference's
+    BSSN_GR/pars/bbh.par.json (spectral solve, then the grid fill)."""
+    import time
+
     import torch
-    from pydgr import ETS, ETS_RK3, Plan, make_params
+    from pydgr import ETS, ETS_RK3, Plan, TwoPunctures, make_params, tp_params
 
     pts = layout_interior_points(blocks)
     padded = sum(b["sz"][0] * b["sz"][1] * b["sz"][2] for b in blocks)
@@ -421,6 +425,21 @@ def extras(plan, blocks, U, stream, steps, warm):
     W.zero_()
     d64_ms = timed(lambda: p64.rhs(W, U, stream), steps)
     p64.close()
+    # TwoPunctures: BSSN_GR/pars/bbh.par.json (b = 4, bare masses 0.5,
+    # P = (0, +-0.114, 0), centre offset (0, 0, sqrt(2) 1e-3), psi^-2 lapse,
+    # 30 x 30 x 16 spectral points)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tp = TwoPunctures(tp_params(par_b=4.0, par_m_plus=0.5, par_m_minus=0.5, give_bare_mass=1,
+                                par_P_plus=(0.0, 0.114, 0.0), par_P_minus=(0.0, -0.114, 0.0),
+                                center_offset=(0.0, 0.0, 1.4142135623730951e-3), initial_lapse="psin",
+                                initial_lapse_psi_exponent=-2.0, npoints_A=30, npoints_B=30, npoints_phi=16))
+    tp_solve_s = time.perf_counter() - t0
+    tp_info = tp.info()
+    tp_ms = timed(lambda: tp.evaluate(plan, W, stream), max(2, steps // 4))
+    tp.close()
+    # spectral sum: 32 (Chebyshev in A, padded) x 30 x 16 FMAs per point
+    tp_flops = 2.0 * 32 * 30 * 16 * padded
     del C, W
     torch.cuda.empty_cache()
     return {"physcon": {"ms": pc_ms, "points_per_s": pts / (pc_ms * 1e-3),
@@ -431,7 +450,12 @@ def extras(plan, blocks, U, stream, steps, warm):
                               "achieved_gbs": 8 * 24 * padded / (init_ms * 1e-3) / 1e9},
             "ets_rk3_step": {"ms": rk3_ms, "rhs_points_per_s": 3 * pts / (rk3_ms * 1e-3)},
             "rhs_deriv64": {"ms": d64_ms, "points_per_s": pts / (d64_ms * 1e-3),
-                            "hbm_frac": algorithmic_bytes(blocks) / (d64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+                            "hbm_frac": algorithmic_bytes(blocks) / (d64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "two_punctures": {"solve_s": tp_solve_s, "spectral_points": [30, 30, 16],
+                              "newton_steps": tp_info["newton_steps"], "last_update": tp_info["last_update"],
+                              "adm_mass": tp_info["E"], "evaluate_ms": tp_ms,
+                              "padded_points_per_s": padded / (tp_ms * 1e-3),
+                              "evaluate_fp64_tflops": tp_flops / (tp_ms * 1e-3) / 1e12}}
 
 
 def layout_interior_points(blocks):

@@ -235,10 +235,38 @@ void apply_lap(const Grid& g, const std::vector<double>& V, std::vector<double>&
 }
 
 struct DevGrid {
-  const double *DA, *DAA, *DB, *DBB, *Dpp, *T, *cAA, *cA, *cBB, *cB, *cpp, *diag;
+  const double *DA, *DAA, *DB, *DBB, *Dpp, *T, *cAA, *cA, *cBB, *cB, *cpp, *diag, *rs;
 };
 
-// Dense Jacobian, column-major (LAPACK): J[p + q N].
+// Largest magnitude in each row of the Jacobian, from the operator's
+// structure (row p couples its A line, B line and phi line).
+void row_max(const Grid& g, const std::vector<double>& diag, std::vector<double>& out) {
+  const int n1 = g.n1, n2 = g.n2, n3 = g.n3;
+  out.resize(g.N);
+  for (int k = 0; k < n3; ++k)
+    for (int j = 0; j < n2; ++j)
+      for (int i = 0; i < n1; ++i) {
+        const int p = i + n1 * (j + n2 * k);
+        const double T = g.T[p];
+        double m = 0.0, dsum = diag[p];
+        for (int i2 = 0; i2 < n1; ++i2) {
+          const double v = g.cAA[p] * (2.0 * g.DA[(size_t)i * n1 + i2] + T * g.DAA[(size_t)i * n1 + i2]) +
+                           g.cA[p] * ((i == i2 ? 1.0 : 0.0) + T * g.DA[(size_t)i * n1 + i2]);
+          if (i2 == i) dsum += v; else m = std::max(m, std::fabs(v));
+        }
+        for (int j2 = 0; j2 < n2; ++j2) {
+          const double v = T * (g.cBB[p] * g.DBB[(size_t)j * n2 + j2] + g.cB[p] * g.DB[(size_t)j * n2 + j2]);
+          if (j2 == j) dsum += v; else m = std::max(m, std::fabs(v));
+        }
+        for (int k2 = 0; k2 < n3; ++k2) {
+          const double v = T * g.cpp[p] * g.Dpp[(size_t)k * n3 + k2];
+          if (k2 == k) dsum += v; else m = std::max(m, std::fabs(v));
+        }
+        out[p] = std::max(m, std::fabs(dsum));
+      }
+}
+
+// Dense Jacobian, column-major (LAPACK): J[p + q N], row p scaled by rs[p].
 __global__ void __launch_bounds__(256) k_tp_jacobian(DevGrid G, int n1, int n2, int n3, double* __restrict__ J) {
   const long long N = (long long)n1 * n2 * n3, NN = N * N;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < NN; e += (long long)gridDim.x * blockDim.x) {
@@ -253,7 +281,7 @@ __global__ void __launch_bounds__(256) k_tp_jacobian(DevGrid G, int n1, int n2, 
     if (i == i2 && k == k2) v += T * (G.cBB[p] * G.DBB[j * n2 + j2] + G.cB[p] * G.DB[j * n2 + j2]);
     if (i == i2 && j == j2) v += T * G.cpp[p] * G.Dpp[k * n3 + k2];
     if (p == q) v += G.diag[p];
-    J[e] = v;
+    J[e] = v * G.rs[p];
   }
 }
 
@@ -280,14 +308,15 @@ struct Solver {
       return DGR_E_NOMEM;
     // small operators and pointwise coefficients in one buffer
     const size_t n1 = g.n1, n2 = g.n2, n3 = g.n3;
-    const size_t sizes[12] = {n1 * n1, n1 * n1, n2 * n2, n2 * n2, n3 * n3, N, N, N, N, N, N, N};
+    const size_t sizes[13] = {n1 * n1, n1 * n1, n2 * n2, n2 * n2, n3 * n3, N, N, N, N, N, N, N, N};
     size_t tot = 0;
     for (size_t s : sizes) tot += s;
     DGR_HIP(hipMalloc(&dsmall, sizeof(double) * tot));
     const std::vector<double>* src[11] = {&g.DA, &g.DAA, &g.DB, &g.DBB, &g.Dpp, &g.T, &g.cAA, &g.cA, &g.cBB, &g.cB, &g.cpp};
-    const double** dst[12] = {&G.DA, &G.DAA, &G.DB, &G.DBB, &G.Dpp, &G.T, &G.cAA, &G.cA, &G.cBB, &G.cB, &G.cpp, &G.diag};
+    const double** dst[13] = {&G.DA,  &G.DAA, &G.DB, &G.DBB, &G.Dpp, &G.T,   &G.cAA,
+                              &G.cA,  &G.cBB, &G.cB, &G.cpp, &G.diag, &G.rs};
     size_t off = 0;
-    for (int q = 0; q < 12; ++q) {
+    for (int q = 0; q < 13; ++q) {
       *dst[q] = dsmall + off;
       if (q < 11) DGR_HIP(hipMemcpyAsync(dsmall + off, src[q]->data(), sizeof(double) * sizes[q], hipMemcpyHostToDevice, st));
       off += sizes[q];
@@ -295,10 +324,15 @@ struct Solver {
     DGR_HIP(hipStreamSynchronize(st));
     return DGR_OK;
   }
-  // One Newton step on V; returns max |update|.
+  // One Newton step on V; returns max |update|.  Each equation (row of the
+  // Jacobian and entry of the residual) is divided by the row's largest
+  // magnitude: the rows next to the punctures and the axis are up to 1e17
+  // times larger than others, and partial pivoting on the unscaled matrix
+  // (condition ~1e19-1e23 at 16-30 points per direction, ~1e3 scaled)
+  // returns updates dominated by rounding.
   int step(std::vector<double>& V, double* upd) {
     const int N = g.N;
-    std::vector<double> F, diag(N);
+    std::vector<double> F, diag(N), rs;
     apply_lap(g, V, F);
     for (int q = 0; q < N; ++q) {
       const double psi = g.psiBL[q] + g.T[q] * V[q];
@@ -306,7 +340,13 @@ struct Solver {
       F[q] += 0.125 * g.AA[q] / p7;
       diag[q] = -0.875 * g.AA[q] / p8 * g.T[q];
     }
+    row_max(g, diag, rs);
+    for (int q = 0; q < N; ++q) {
+      rs[q] = rs[q] > 0.0 ? 1.0 / rs[q] : 1.0;
+      F[q] *= rs[q];
+    }
     DGR_HIP(hipMemcpyAsync(const_cast<double*>(G.diag), diag.data(), sizeof(double) * N, hipMemcpyHostToDevice, st));
+    DGR_HIP(hipMemcpyAsync(const_cast<double*>(G.rs), rs.data(), sizeof(double) * N, hipMemcpyHostToDevice, st));
     DGR_HIP(hipMemcpyAsync(dF, F.data(), sizeof(double) * N, hipMemcpyHostToDevice, st));
     const long long NN = (long long)N * N;
     const unsigned grid = (unsigned)std::min<long long>((NN + 255) / 256, 1 << 20);
