@@ -154,3 +154,20 @@ def test_hamiltonian_constraint_converges_on_the_grid(cuda):
     print("Hamiltonian constraint rms, h and h/2:", norms, "ratio", norms[0] / norms[1], flush=True)
     assert norms[0] / norms[1] > 8.0  # fourth-order stencils: 16
     tp.close()
+
+
+def test_reference_binary_parameters_converge_at_default_resolution(cuda):
+    # BSSN_GR/pars/bbh.par.json at the reference's 30 x 30 x 16 points, its
+    # default Newton_tol (1e-10) and Newton_maxit (5): without the row
+    # scaling of the Newton equations this case diverged
+    E = []
+    for nphi in (8, 16):
+        tp = TwoPunctures(tp_params(par_b=4.0, par_m_plus=0.5, par_m_minus=0.5, give_bare_mass=1,
+                                    par_P_plus=(0.0, 0.114, 0.0), par_P_minus=(0.0, -0.114, 0.0),
+                                    npoints_A=30, npoints_B=30, npoints_phi=nphi))
+        info = tp.info()
+        print(nphi, info, flush=True)
+        assert info["last_update"] < 1e-10 and info["newton_steps"] <= 5
+        E.append(info["E"])
+        tp.close()
+    assert 1.0 < E[0] < 1.05 and abs(E[0] - E[1]) < 1e-10
