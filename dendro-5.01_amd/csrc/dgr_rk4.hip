@@ -642,12 +642,13 @@ template <bool PROJECT>
 __global__ void __launch_bounds__(384) k_ets_combine(const DevBlock* __restrict__ blocks,
                                                     const DevTile* __restrict__ tiles, int ntiles, Ptrs24 out,
                                                     Ptrs24 base, const double* __restrict__ kb, uint64_t nacc,
-                                                    EtsTerms T, double chi_floor, unsigned* __restrict__ flag) {
+                                                    EtsTerms T, double chi_floor, unsigned* __restrict__ flag,
+                                                    uint64_t cdof) {
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const DevTile tl = tiles[w];
     const DevBlock& B = blocks[tl.block];
-    uint64_t o;
-    if (!tile_point(tl, B, threadIdx.x, o)) continue;
+    uint64_t o, oc;
+    if (!tile_point(tl, B, threadIdx.x, o, &oc)) continue;
     const uint64_t co = (uint64_t)w * 343 + threadIdx.x;
     if constexpr (PROJECT) {
       double u[24];
@@ -660,8 +661,9 @@ __global__ void __launch_bounds__(384) k_ets_combine(const DevBlock* __restrict_
       }
       const int bad = enforce_point(u, chi_floor);
       if (bad) atomicOr(flag, (unsigned)bad);
+      // a stage input: the compact interior region of the stage buffer
 #pragma unroll
-      for (int v = 0; v < 24; ++v) out.v[v][o] = u[v];
+      for (int v = 0; v < 24; ++v) out.v[v][cdof + oc] = u[v];
     } else {
       for (int v = 0; v < 24; ++v) {
 #pragma clang fp contract(off)
@@ -682,7 +684,8 @@ struct dgr_ets_s {
   std::vector<double> a, b, c;  // a[s*ns + p], b[k], c[s]
   double time = 0.0;
   uint64_t nacc = 0;      // compact k slots per variable (343 per tile)
-  double* buf = nullptr;  // tmp (24 x dof) then k_0 .. k_{ns-1} (24 x nacc each, compact)
+  uint64_t sstride = 0;   // tmp per variable: dof padded points, then the compact interior region
+  double* buf = nullptr;  // tmp (24 x sstride) then k_0 .. k_{ns-1} (24 x nacc each, compact)
   Ptrs24 state{}, tmp{};
   IOPtrs* d_io = nullptr;  // RHS tables: stage s reads tmp, writes k_s
   unsigned* d_flag = nullptr;
@@ -715,18 +718,19 @@ extern "C" int dgr_ets_create_tableau(dgr_ets_t* out, dgr_plan_t plan, double* c
   for (int v = 0; v < 24; ++v) e->state.v[v] = u[v];
   const size_t nb = sizeof(double) * e->dof;
   e->nacc = 343ull * plan->ntiles;
-  const size_t nbuf = sizeof(double) * 24 * (e->dof + (size_t)stages * e->nacc);
+  e->sstride = plan->dof_end + plan->ncomp;
+  const size_t nbuf = sizeof(double) * 24 * (e->sstride + (size_t)stages * e->nacc);
   if (hipMalloc(&e->buf, nbuf) != hipSuccess ||
       hipMalloc(&e->d_io, sizeof(IOPtrs) * stages) != hipSuccess || hipMalloc(&e->d_flag, 256) != hipSuccess) {
     ets_destroy(e);
     return DGR_E_NOMEM;
   }
   std::vector<IOPtrs> io(stages);
-  for (int v = 0; v < 24; ++v) e->tmp.v[v] = e->buf + (size_t)v * e->dof;
+  for (int v = 0; v < 24; ++v) e->tmp.v[v] = e->buf + (size_t)v * e->sstride;
   for (uint32_t s = 0; s < stages; ++s)
     for (int v = 0; v < 24; ++v) {
       io[s].u.v[v] = e->tmp.v[v];
-      io[s].r.v[v] = e->buf + (size_t)24 * e->dof + ((size_t)s * 24 + v) * e->nacc;
+      io[s].r.v[v] = e->buf + (size_t)24 * e->sstride + ((size_t)s * 24 + v) * e->nacc;
     }
   hipError_t er = hipMemcpy(e->d_io, io.data(), sizeof(IOPtrs) * stages, hipMemcpyHostToDevice);
   if (er == hipSuccess) er = hipMemset(e->d_flag, 0, 256);
@@ -766,7 +770,7 @@ extern "C" int dgr_ets_evolve(dgr_ets_t e, double dt, void* stream) {
   dgr_plan_t p = e->plan;
   const int nt = (int)p->ntiles;
   const int g = grid_for(p, nt);
-  const double* kb = e->buf + (size_t)24 * e->dof;
+  const double* kb = e->buf + (size_t)24 * e->sstride;
   OutPtrs tmp_o;
   for (int v = 0; v < 24; ++v) tmp_o.v[v] = e->tmp.v[v];
   for (int s = 0; s < e->ns; ++s) {
@@ -777,11 +781,13 @@ extern "C" int dgr_ets_evolve(dgr_ets_t e, double dt, void* stream) {
       T.c[T.n] = aq * dt;
       T.idx[T.n++] = q;
     }
+    // the stage input goes to tmp's compact region (whole lines), which the
+    // level-jump fills and the gathering RHS read
     hipLaunchKernelGGL(k_ets_combine<true>, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->tmp,
-                       e->state, kb, e->nacc, T, p->chi_floor, e->d_flag);
-    int rc = launch_level_fill(p, tmp_o, st);
+                       e->state, kb, e->nacc, T, p->chi_floor, e->d_flag, e->dof);
+    int rc = launch_level_fill(p, tmp_o, st, true);
     if (rc) return rc;
-    rc = launch_rhs(p, e->d_io + s, st, true);
+    rc = launch_rhs(p, e->d_io + s, st, true, true);
     if (rc) return rc;
   }
   EtsTerms T{};
@@ -791,7 +797,7 @@ extern "C" int dgr_ets_evolve(dgr_ets_t e, double dt, void* stream) {
   }
   T.n = e->ns;
   hipLaunchKernelGGL(k_ets_combine<false>, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->state,
-                     e->state, kb, e->nacc, T, p->chi_floor, e->d_flag);
+                     e->state, kb, e->nacc, T, p->chi_floor, e->d_flag, (uint64_t)0);
   hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(p, (int)p->nblocks)), dim3(RK_NT), 0, st, p->d_blocks, p->d_nbr,
                      (int)p->nblocks, e->state);
   DGR_HIP(hipGetLastError());
