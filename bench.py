@@ -381,6 +381,24 @@ def serving(njobs, budget_gb, max_points, rate):
     return stats
 
 
+def serving_in_child(njobs, budget_gb, max_points, rate):
+    """One serving run in a child process (this script with --serving-child):
+    the server's threads drive the HIP runtime from several host threads at
+    once, and a failure there must not take the rest of the measurement with
+    it.  Returns the child's statistics, or an error record."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--serving-child", repr(rate), "--serving-jobs", str(njobs),
+           "--serving-budget-gb", repr(budget_gb), "--serving-max-points", str(max_points or 0)]
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
+    except subprocess.TimeoutExpired:
+        return {"error": "serving run did not finish within 900 s", "rate_jobs_per_s": rate}
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": "serving run exited with code %d" % r.returncode, "rate_jobs_per_s": rate}
+    return json.loads(lines[-1])
+
+
 def extras(plan, blocks, U, stream, steps, warm):
     """Timings of the entry points around the RHS on the same grid: the
     constraint monitor (dgr_physcon + masked norms), puncture initial data
@@ -479,12 +497,18 @@ def main():
     ap.add_argument("--serving-budget-gb", type=float, default=200.0)
     ap.add_argument("--serving-max-points", type=int, default=3_000_000,
                     help="cap on resident interior points (0: memory budget only)")
+    ap.add_argument("--serving-child", type=float, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-reps", type=int, default=5, help="timed CPU-oracle runs (median reported)")
     ap.add_argument("--no-autotune", action="store_true", help="library default schedule instead of the autotuner's")
     ap.add_argument("--calibration-copy-gib", type=float, default=0.0,
                     help="first copy a float64 tensor of this many GiB three times: a plain copy of known size "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     a = ap.parse_args()
+    if a.serving_child is not None:
+        # child of serving_in_child: one serving run, its statistics as JSON
+        print(json.dumps(serving(a.serving_jobs, a.serving_budget_gb, a.serving_max_points or None,
+                                 a.serving_child)), flush=True)
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box here has one GPU); "
               "run with --gpus 1", file=sys.stderr)
@@ -596,10 +620,15 @@ def main():
         log("adaptive: RHS %.4f ms, RK4 %.4f ms" % (line["adaptive"]["rhs_ms"], line["adaptive"]["rk4_ms_per_step"]))
     if a.serving_jobs > 0:
         line["serving"] = {}
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()  # the child takes most of HBM for its pool
         for rate in a.serving_rates:
-            st = serving(a.serving_jobs, a.serving_budget_gb, a.serving_max_points or None, rate)
-            st["throughput_vs_uniform_rk4"] = st["rhs_points_per_s"] / value
+            st = serving_in_child(a.serving_jobs, a.serving_budget_gb, a.serving_max_points or None, rate)
             line["serving"]["%g_per_s" % rate] = st
+            if "error" in st:
+                log("serving %g/s: %s" % (rate, st["error"]))
+                continue
+            st["throughput_vs_uniform_rk4"] = st["rhs_points_per_s"] / value
             log("serving %g/s: p50 %.3f s p99 %.3f s, %.3g RHS pts/s (%.2f of uniform RK4), main-loop host time "
                 "%.1f%% of makespan" % (rate, st["p50_latency_s"], st["p99_latency_s"], st["rhs_points_per_s"],
                                         st["throughput_vs_uniform_rk4"], 100 * st["main_loop_host_share"]))

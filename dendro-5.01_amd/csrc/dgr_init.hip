@@ -130,7 +130,56 @@ __global__ void __launch_bounds__(256) k_init_puncture(const DevBlock* __restric
   }
 }
 
+// Perturbed flat space: the flat values plus, on every variable v,
+// amp sum_b exp(-|x - c_vb|^2 / width^2) over the bumps in order, with the
+// operations in the order of the host generator (pydgr.data.flat_bumps).
+constexpr int kMaxBumps = 16;
+__global__ void __launch_bounds__(256) k_init_bumps(const DevBlock* __restrict__ blocks, uint32_t nblocks,
+                                                    Ptrs24Out u, const double* __restrict__ centers, int nbumps,
+                                                    double amp, double iw2) {
+  __shared__ double sc[24 * kMaxBumps * 3];
+  for (int e = threadIdx.x; e < 24 * nbumps * 3; e += blockDim.x) sc[e] = centers[e];
+  __syncthreads();
+  for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const DevBlock& B = blocks[b];
+    const int s0 = B.sz[0], s1 = B.sz[1], n = s0 * s1 * (int)B.sz[2];
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+#pragma clang fp contract(off)
+      const int i = e % s0, j = (e / s0) % s1, k = e / (s0 * s1);
+      const double x = B.pmin[0] + i * B.h[0], y = B.pmin[1] + j * B.h[1], z = B.pmin[2] + k * B.h[2];
+      const uint64_t o = B.offset + (uint64_t)e;
+      for (int v = 0; v < 24; ++v) {
+        double acc = 0.0;
+        for (int q = 0; q < nbumps; ++q) {
+          const double* c = sc + 3 * (v * nbumps + q);
+          const double dx = x - c[0], dy = y - c[1], dz = z - c[2];
+          acc = acc + exp(-(dx * dx + dy * dy + dz * dz) * iw2);
+        }
+        const bool one = v == dgr::V_ALPHA || v == dgr::V_CHI || v == dgr::V_SG0 || v == dgr::V_SG0 + 3 ||
+                         v == dgr::V_SG0 + 5;
+        u.v[v][o] = (one ? 1.0 : 0.0) + amp * acc;
+      }
+    }
+  }
+}
+
 }  // namespace
+
+extern "C" int dgr_init_bumps(dgr_plan_t plan, double* const* u, const double* centers, int nbumps, double amp,
+                              double width, void* stream) {
+  if (!plan || !u || !centers || nbumps < 0 || nbumps > kMaxBumps || !(width > 0.0)) return DGR_E_INVALID;
+  Ptrs24Out pu;
+  for (int v = 0; v < 24; ++v) {
+    if (!u[v]) return DGR_E_INVALID;
+    pu.v[v] = u[v];
+  }
+  if (plan->nblocks == 0) return DGR_OK;
+  const uint32_t g = std::min<uint32_t>(plan->nblocks, 16384);
+  hipLaunchKernelGGL(k_init_bumps, dim3(g), dim3(256), 0, (hipStream_t)stream, plan->d_blocks, plan->nblocks, pu,
+                     centers, nbumps, amp, 1.0 / (width * width));
+  DGR_HIP(hipGetLastError());
+  return DGR_OK;
+}
 
 extern "C" int dgr_init_puncture(dgr_plan_t plan, double* const* u, const dgr_puncture* holes, void* stream) {
   if (!plan || !u || !holes) return DGR_E_INVALID;

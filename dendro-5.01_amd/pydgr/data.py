@@ -69,11 +69,15 @@ def _flat(dof, device, out=None):
     return U
 
 
-def flat_bumps(blocks, dof, seed=0, amp=1e-3, width=0.1, nbumps=8, box=None, device="cpu", out=None):
+def flat_bumps(blocks, dof, seed=0, amp=1e-3, width=0.1, nbumps=8, box=None, device="cpu", out=None, plan=None):
     """Flat space (alpha=chi=1, gt=delta, rest 0) plus, on every variable,
     the sum of `nbumps` Gaussians amp*exp(-|x-c|^2/width^2) with seeded
     centres uniform in the bounding box of the blocks.  Written into `out`
-    (24, dof) when given; temporaries hold a few variables at a time."""
+    (24, dof) when given; temporaries hold a few variables at a time.  With
+    a plan of these blocks (on a GPU), one fused kernel (dgr_init_bumps)
+    writes every padded point, in the same operation order."""
+    if plan is not None:
+        return _flat_bumps_plan(plan, blocks, dof, seed, amp, width, nbumps, box, device, out)
     x, y, z, idx = coords(blocks, device)
     if box is None:
         lo = [min(b["pmin"][d] for b in blocks) for d in range(3)]
@@ -97,6 +101,25 @@ def flat_bumps(blocks, dof, seed=0, amp=1e-3, width=0.1, nbumps=8, box=None, dev
             acc += torch.exp(-((x[None, :] - c[:, 0:1]) ** 2 + (y[None, :] - c[:, 1:2]) ** 2 +
                                (z[None, :] - c[:, 2:3]) ** 2) * iw2)
         U[v:v + G, idx] += amp * acc
+    return U
+
+
+def _flat_bumps_plan(plan, blocks, dof, seed, amp, width, nbumps, box, device, out):
+    if box is None:
+        lo = [min(b["pmin"][d] for b in blocks) for d in range(3)]
+        hi = [max(b["pmax"][d] for b in blocks) for d in range(3)]
+    else:
+        lo, hi = box
+    rng = np.random.default_rng(seed)
+    cen = torch.tensor(rng.uniform(lo, hi, size=(NUM_VARS, nbumps, 3)), dtype=torch.float64)
+    cen = cen.pin_memory().to(device, non_blocking=True)
+    U = out if out is not None else torch.empty((NUM_VARS, dof), dtype=torch.float64, device=device)
+    if tuple(U.shape) != (NUM_VARS, dof) or U.dtype != torch.float64:
+        raise ValueError("out must be float64 of shape (24, %d)" % dof)
+    stream = torch.cuda.current_stream(U.device)
+    plan.init_bumps(U, cen, amp, width, stream=stream)
+    # the centres stay alive until the kernel has read them
+    cen.record_stream(stream)
     return U
 
 

@@ -128,6 +128,7 @@ def lib():
         L.dgr_enforce_constraints.argtypes = [vp, pp, vp]
         L.dgr_physcon.argtypes = [vp, pp, pp, vp]
         L.dgr_init_puncture.argtypes = [vp, pp, ctypes.POINTER(Puncture), vp]
+        L.dgr_init_bumps.argtypes = [vp, pp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp]
         L.dgr_tp_params_default.argtypes = [ctypes.POINTER(TPParams)]
         L.dgr_tp_params_default.restype = None
         L.dgr_tp_solve.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(TPParams)]
@@ -332,6 +333,11 @@ def block_neighbours(blocks):
     return out
 
 
+def torch_float64():
+    import torch
+    return torch.float64
+
+
 def _stream_ptr(stream):
     if stream is None:
         return None
@@ -408,6 +414,15 @@ class Plan:
         up, _u = ptr_array(var_ptrs(u))
         arr = punctures(holes)
         check(lib().dgr_init_puncture(self.handle, up, arr, _stream_ptr(stream)), "dgr_init_puncture")
+
+    def init_bumps(self, u, centers, amp, width, stream=None):
+        """Perturbed flat space at every padded point (dgr_init_bumps):
+        centers is a device float64 tensor (24, nbumps, 3)."""
+        assert centers.is_cuda and centers.dtype == torch_float64() and centers.dim() == 3 and centers.shape[0] == 24
+        c = centers.contiguous()
+        up, _u = ptr_array(var_ptrs(u))
+        check(lib().dgr_init_bumps(self.handle, up, ctypes.c_void_p(c.data_ptr()), int(c.shape[1]), float(amp),
+                                   float(width), _stream_ptr(stream)), "dgr_init_bumps")
 
     def physcon(self, con, u, stream=None):
         """Constraint monitor (physical_constraints): con is a (6, dof)

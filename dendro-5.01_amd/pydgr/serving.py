@@ -89,14 +89,17 @@ class Job:
             self.blocks = blocks
         return self.blocks
 
-    def initial_state(self, device, out=None):
-        """The job's 24 initial arrays (written into `out` when given)."""
+    def initial_state(self, device, out=None, plan=None):
+        """The job's 24 initial arrays (written into `out` when given; flat
+        jobs with the fused kernel when a plan of the job's blocks is
+        given)."""
         blocks = self.build()
         if self.kind == "bl":
             q = self.mass_ratio
             m1, m2 = q / (1.0 + q), 1.0 / (1.0 + q)
             return data.brill_lindquist(blocks, self.dof, m1=m1, m2=m2, sep=8.0, device=device, out=out)
-        return data.flat_bumps(blocks, self.dof, seed=self.seed, amp=1e-3, width=0.1, device=device, out=out)
+        return data.flat_bumps(blocks, self.dof, seed=self.seed, amp=1e-3, width=0.1, device=device, out=out,
+                               plan=plan)
 
 
 class PoolAllocator:

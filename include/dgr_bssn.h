@@ -241,6 +241,13 @@ typedef struct {
  * gt = delta, all else zero.  holes[2]; masses must be positive.
  * Asynchronous on stream. */
 int dgr_init_puncture(dgr_plan_t plan, double* const* u, const dgr_puncture* holes, void* stream);
+/* No counterpart in the reference: synthetic perturbed flat space for
+ * benchmarks and served jobs.  At every padded point, alpha = chi = 1,
+ * gt = delta, all else 0, plus on variable v the sum over the bumps b of
+ * amp exp(-|x - c_vb|^2 / width^2); centers: device array of 24 x nbumps x 3
+ * doubles (c_vb at 3 (v nbumps + b)), nbumps <= 16.  Asynchronous on stream. */
+int dgr_init_bumps(dgr_plan_t plan, double* const* u, const double* centers, int nbumps, double amp, double width,
+                   void* stream);
 
 /* TwoPunctures initial data (BSSN_GR/src/TwoPunctures.cpp:24-498, solver in
  * TPNewton.cpp / TPFuncAndJacobian.cpp / TPCoordTransf.cpp / TPEquations.cpp).

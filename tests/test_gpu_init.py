@@ -1,7 +1,9 @@
 """Device initial data (dgr_init_puncture) through the C ABI against the
 oracle's restatement of punctureData at every padded point: <= 1e-12 of
 each variable's max |value| (libm pow / log / sqrt may differ in the last
-bits between host and device), floors and zero variables exact."""
+bits between host and device), floors and zero variables exact; and the
+synthetic perturbed flat space of dgr_init_bumps bitwise equal to the torch
+generator it replaces on the device."""
 import numpy as np
 import pytest
 import torch
@@ -58,3 +60,17 @@ def test_evolution_starts_from_device_data(cuda):
     got = R.cpu().numpy()
     rel = np.abs(got[:, ii] - ref[:, ii]).max(1) / np.maximum(np.abs(ref[:, ii]).max(1), 1e-300)
     assert rel.max() <= 1e-10, rel
+
+
+def test_init_bumps_equals_the_torch_generator_bitwise(cuda):
+    # dgr_init_bumps (one fused kernel) and pydgr.data.flat_bumps (torch on the
+    # same device) perform the same operations in the same order
+    for blocks, dof in (layout.block_grid(70), layout.mixed_lattices()):
+        plan = Plan(blocks, make_params())
+        ref = data.flat_bumps(blocks, dof, seed=4, device=cuda)
+        got = data.flat_bumps(blocks, dof, seed=4, device=cuda, out=torch.full((24, dof), float("nan"),
+                                                                                dtype=torch.float64, device=cuda),
+                              plan=plan)
+        torch.cuda.synchronize()
+        assert torch.equal(got, ref)
+        plan.close()
