@@ -480,6 +480,55 @@ __host__ __device__ inline double extend(double M, double r, double R) {
   return M * (3.0 / 8.0 * r * r * r * r / (R * R * R * R * R) - 5.0 / 4.0 * r * r / (R * R * R) + 15.0 / 8.0 / R);
 }
 
+// The 24 BSSN variables at a point (coordinates relative to the centre)
+// from u there (TwoPunctures.cpp:285-498): psi with the regularised
+// distances, the lapse of initial_lapse, gd = psi^4 delta, K_ij =
+// A_ij / psi^2, then chi = det(gd)^(-1/3), gt = chi gd, K = tr K and
+// At = chi (K_ij - gd_ij K / 3); Gt = beta = B = 0.
+__host__ __device__ inline void tp_point(const dgr_tp_params& P, double mp, double mm, double xx, double yy,
+                                         double zz, double U, double out[24]) {
+  double rp = sqrt((xx - P.par_b) * (xx - P.par_b) + yy * yy + zz * zz);
+  double rm = sqrt((xx + P.par_b) * (xx + P.par_b) + yy * yy + zz * zz);
+  const double e4 = P.TP_epsilon * P.TP_epsilon * P.TP_epsilon * P.TP_epsilon;
+  rp = pow(rp * rp * rp * rp + e4, 0.25);
+  rm = pow(rm * rm * rm * rm + e4, 0.25);
+  if (rp < P.TP_Tiny) rp = P.TP_Tiny;
+  if (rm < P.TP_Tiny) rm = P.TP_Tiny;
+  const double R = P.TP_Extend_Radius;
+  const double ip = rp < R ? extend(1.0, rp, R) : 1.0 / rp;  // 1/r, continued inside R
+  const double im = rm < R ? extend(1.0, rm, R) : 1.0 / rm;
+  const double psi1 = 1.0 + 0.5 * mp * ip + 0.5 * mm * im + U;
+  double alpha = 1.0;
+  if (P.initial_lapse == DGR_TP_LAPSE_ANTISYMMETRIC || P.initial_lapse == DGR_TP_LAPSE_AVERAGED) {
+    alpha = (1.0 - 0.5 * mp * ip - 0.5 * mm * im) / (1.0 + 0.5 * mp * ip + 0.5 * mm * im);
+    if (P.initial_lapse == DGR_TP_LAPSE_AVERAGED) alpha = 0.5 * (1.0 + alpha);
+  } else if (P.initial_lapse == DGR_TP_LAPSE_PSIN || P.initial_lapse == DGR_TP_LAPSE_BROWNSVILLE) {
+    const double ps = 1.0 + 0.5 * mp * ip + 0.5 * mm * im;  // psi without u
+    alpha = pow(ps, P.initial_lapse_psi_exponent);
+    if (P.initial_lapse == DGR_TP_LAPSE_BROWNSVILLE) alpha = 2.0 / (1.0 + alpha);
+  }
+  double Aij[3][3];
+  by_aij<double>(P, xx, yy, zz, Aij, true);
+  const double p2 = psi1 * psi1, p4 = p2 * p2;
+  const double det = p4 * p4 * p4, chi = pow(1.0 / det, 1.0 / 3.0);
+  const double gu = 1.0 / p4;
+  const double trK = gu * (Aij[0][0] + Aij[1][1] + Aij[2][2]) / p2;
+  out[dgr::V_ALPHA] = alpha;
+  out[dgr::V_CHI] = chi;
+  out[dgr::V_K] = trK;
+  for (int c = 0; c < 3; ++c) {
+    out[dgr::V_GT0 + c] = 0.0;
+    out[dgr::V_BETA0 + c] = 0.0;
+    out[dgr::V_B0 + c] = 0.0;
+  }
+  for (int q = 0; q < 6; ++q) {
+    const int r = dgr::spi(q), c = dgr::spj(q);
+    const double gd = r == c ? p4 : 0.0;
+    out[dgr::V_SG0 + q] = chi * gd;
+    out[dgr::V_SA0 + q] = chi * (Aij[r][c] / p2 - gd * trK / 3.0);
+  }
+}
+
 constexpr int kLdsCoef = 16384;  // coefficients staged in LDS (128 KB) when they fit
 
 template <int NA, bool LDS>
@@ -531,52 +580,11 @@ __global__ void __launch_bounds__(512) k_tp_eval(const DevBlock* __restrict__ bl
         V += ff * sf;
       }
       const double U = (A - 1.0) * V;
-      // psi and the lapse (TwoPunctures.cpp:285-420), distances regularised
-      double rp = sqrt((xx - P.par_b) * (xx - P.par_b) + yy * yy + zz * zz);
-      double rm = sqrt((xx + P.par_b) * (xx + P.par_b) + yy * yy + zz * zz);
-      const double e4 = P.TP_epsilon * P.TP_epsilon * P.TP_epsilon * P.TP_epsilon;
-      rp = pow(rp * rp * rp * rp + e4, 0.25);
-      rm = pow(rm * rm * rm * rm + e4, 0.25);
-      if (rp < P.TP_Tiny) rp = P.TP_Tiny;
-      if (rm < P.TP_Tiny) rm = P.TP_Tiny;
-      const double R = P.TP_Extend_Radius;
-      const double ip = rp < R ? extend(1.0, rp, R) : 1.0 / rp;  // 1/r, continued inside R
-      const double im = rm < R ? extend(1.0, rm, R) : 1.0 / rm;
-      const double psi1 = 1.0 + 0.5 * mp * ip + 0.5 * mm * im + U;
-      double alpha = 1.0;
-      if (P.initial_lapse == DGR_TP_LAPSE_ANTISYMMETRIC || P.initial_lapse == DGR_TP_LAPSE_AVERAGED) {
-        alpha = (1.0 - 0.5 * mp * ip - 0.5 * mm * im) / (1.0 + 0.5 * mp * ip + 0.5 * mm * im);
-        if (P.initial_lapse == DGR_TP_LAPSE_AVERAGED) alpha = 0.5 * (1.0 + alpha);
-      } else if (P.initial_lapse == DGR_TP_LAPSE_PSIN || P.initial_lapse == DGR_TP_LAPSE_BROWNSVILLE) {
-        const double ps = 1.0 + 0.5 * mp * ip + 0.5 * mm * im;  // psi without u
-        alpha = pow(ps, P.initial_lapse_psi_exponent);
-        if (P.initial_lapse == DGR_TP_LAPSE_BROWNSVILLE) alpha = 2.0 / (1.0 + alpha);
-      }
-      // ADM data gd = psi^4 delta, K_ij = A_ij / psi^2, then BSSN:
-      // chi = det(gd)^(-1/3), gt = chi gd, K = tr K, At = chi (K_ij - gd_ij K / 3)
-      double Aij[3][3];
-      by_aij<double>(P, xx, yy, zz, Aij, true);
-      const double p2 = psi1 * psi1, p4 = p2 * p2;
-      const double det = p4 * p4 * p4, chi = pow(1.0 / det, 1.0 / 3.0);
-      const double gu = 1.0 / p4;
-      const double trK = gu * (Aij[0][0] + Aij[1][1] + Aij[2][2]) / p2;
+      double out[24];
+      tp_point(P, mp, mm, xx, yy, zz, U, out);
       const uint64_t o = Bk.offset + (uint64_t)e;
-      u.v[dgr::V_ALPHA][o] = alpha;
-      u.v[dgr::V_CHI][o] = chi;
-      u.v[dgr::V_K][o] = trK;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        u.v[dgr::V_GT0 + c][o] = 0.0;
-        u.v[dgr::V_BETA0 + c][o] = 0.0;
-        u.v[dgr::V_B0 + c][o] = 0.0;
-      }
-#pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        const int r = dgr::spi(q), c = dgr::spj(q);
-        const double gd = r == c ? p4 : 0.0;
-        u.v[dgr::V_SG0 + q][o] = chi * gd;
-        u.v[dgr::V_SA0 + q][o] = chi * (Aij[r][c] / p2 - gd * trK / 3.0);
-      }
+      for (int v = 0; v < 24; ++v) u.v[v][o] = out[v];
     }
   }
 }
@@ -736,6 +744,13 @@ extern "C" int dgr_tp_u(dgr_tp_t tp, const double* xyz, double* out, uint64_t n)
   for (uint64_t q = 0; q < n; ++q)
     out[q] = u_host(*tp, xyz[3 * q] - tp->p.center_offset[0], xyz[3 * q + 1] - tp->p.center_offset[1],
                     xyz[3 * q + 2] - tp->p.center_offset[2]);
+  return DGR_OK;
+}
+
+extern "C" int dgr_tp_point(dgr_tp_t tp, double x, double y, double z, double* vars) {
+  if (!tp || !vars) return DGR_E_INVALID;
+  const double xx = x - tp->p.center_offset[0], yy = y - tp->p.center_offset[1], zz = z - tp->p.center_offset[2];
+  tp_point(tp->p, tp->mp, tp->mm, xx, yy, zz, u_host(*tp, xx, yy, zz), vars);
   return DGR_OK;
 }
 

@@ -20,6 +20,15 @@ BSSNGlobals& bssn_params() {
   return g;
 }
 
+dgr_tp_params& tp_params() {
+  static dgr_tp_params p = [] {
+    dgr_tp_params d;
+    dgr_tp_params_default(&d);
+    return d;
+  }();
+  return p;
+}
+
 }  // namespace dgr
 
 namespace {
@@ -224,4 +233,28 @@ void physical_constraints(double** uZipConVars, const double** uZipVars, const u
   }
   dgr_plan_destroy(plan);
   (void)hipFree(buf);
+}
+
+void TwoPunctures(const double xx1, const double yy1, const double zz1, double* vars, double* mp, double* mm,
+                  double* mp_adm, double* mm_adm, double* E, double* J1, double* J2, double* J3) {
+  // the solve on the first call, kept for the process (TwoPunctures.cpp:48-200)
+  static std::mutex m;
+  static dgr_tp_t tp = nullptr;
+  static dgr_tp_info_t info;
+  {
+    std::lock_guard<std::mutex> lock(m);
+    if (!tp) {
+      check(dgr_tp_solve(&tp, &dgr::tp_params()), "TwoPunctures (dgr_tp_solve)");
+      check(dgr_tp_info(tp, &info), "dgr_tp_info");
+    }
+  }
+  check(dgr_tp_point(tp, xx1, yy1, zz1, vars), "TwoPunctures (dgr_tp_point)");
+  *mp = info.mp;
+  *mm = info.mm;
+  *mp_adm = info.mp_adm;
+  *mm_adm = info.mm_adm;
+  *E = info.E;
+  *J1 = info.J[0];
+  *J2 = info.J[1];
+  *J3 = info.J[2];
 }

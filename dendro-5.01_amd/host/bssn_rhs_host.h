@@ -22,6 +22,8 @@
 #ifndef DGR_BSSN_RHS_HOST_H
 #define DGR_BSSN_RHS_HOST_H
 
+#include "../../include/dgr_bssn.h"
+
 namespace dgr {
 
 struct BSSNGlobals {
@@ -36,6 +38,11 @@ struct BSSNGlobals {
 
 // Process-wide parameter set used by bssnRHS / bssnrhs.
 BSSNGlobals& bssn_params();
+
+// Process-wide TwoPunctures parameters (the reference's TPID:: globals,
+// BSSN_GR/include/parameters.h:262-310), starting from its defaults; set
+// them before the first TwoPunctures call, which solves.
+dgr_tp_params& tp_params();
 
 // What bssnRHS reads from an ot::Block (BSSN_GR/src/rhs.cpp:45-62): the
 // offset, the allocation sizes, the boundary flag and the padded extent
@@ -60,5 +67,15 @@ void bssnrhs(double** uzipVarsRHS, const double** uZipVars, const unsigned int& 
 // BSSN_GR/include/grDef.h:34) of the same unzipped layout as u.
 void physical_constraints(double** uZipConVars, const double** uZipVars, const unsigned int& offset,
                           const double* pmin, const double* pmax, const unsigned int* sz, const unsigned int& bflag);
+
+// TwoPunctures (BSSN_GR/include/TwoPunctures.h, BSSN_GR/src/TwoPunctures.cpp:
+// 24-498): the first call solves with dgr::tp_params(), every call writes the
+// 24 variables of the point (x, y, z) into vars (VAR order) and the bare
+// masses, puncture ADM masses, total ADM mass and angular momentum.  The
+// coordinates are physical: the reference's GRIDX_TO_X is the caller's.  One
+// point per call on the host, as the reference; dgr_tp_evaluate fills whole
+// grids on the device.
+void TwoPunctures(const double xx1, const double yy1, const double zz1, double* vars, double* mp, double* mm,
+                  double* mp_adm, double* mm_adm, double* E, double* J1, double* J2, double* J3);
 
 #endif

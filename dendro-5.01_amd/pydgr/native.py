@@ -136,6 +136,7 @@ def lib():
         L.dgr_tp_collocation.argtypes = [vp, vp, ctypes.c_uint64]
         L.dgr_tp_u.argtypes = [vp, vp, vp, ctypes.c_uint64]
         L.dgr_tp_evaluate.argtypes = [vp, vp, pp, vp]
+        L.dgr_tp_point.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp]
         L.dgr_tp_destroy.argtypes = [vp]
         L.dgr_bssn_rhs_blkwise.argtypes = [vp, pp, pp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, vp]
         L.dgr_bssn_rhs_blk.argtypes = [vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp]
@@ -271,6 +272,15 @@ class TwoPunctures:
         out = np.empty(len(xyz))
         check(lib().dgr_tp_u(self.handle, xyz.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p),
                              len(xyz)), "dgr_tp_u")
+        return out
+
+    def point(self, x, y, z):
+        """The 24 variables at one point on the host (dgr_tp_point, the
+        reference's per-point TwoPunctures call)."""
+        import numpy as np
+        out = np.empty(24)
+        check(lib().dgr_tp_point(self.handle, float(x), float(y), float(z), out.ctypes.data_as(ctypes.c_void_p)),
+              "dgr_tp_point")
         return out
 
     def evaluate(self, plan, u, stream=None):

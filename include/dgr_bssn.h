@@ -302,6 +302,10 @@ int dgr_tp_info(dgr_tp_t tp, dgr_tp_info_t* info);
 int dgr_tp_collocation(dgr_tp_t tp, double* values, uint64_t n);
 /* u at n points (xyz: 3 n host doubles, absolute coordinates), on the host. */
 int dgr_tp_u(dgr_tp_t tp, const double* xyz, double* out, uint64_t n);
+/* The reference's per-point TwoPunctures call on the host: the 24 BSSN
+ * variables at one point (absolute coordinates) into vars[24], the same
+ * formulas as dgr_tp_evaluate (host libm instead of the device's). */
+int dgr_tp_point(dgr_tp_t tp, double x, double y, double z, double* vars);
 /* The 24 BSSN variables at every padded point of every block (x = pmin +
  * i h): chi, gt, At, K from psi^4 delta and the Bowen-York curvature, the
  * lapse of initial_lapse, beta = Gt = B = 0.  Asynchronous on stream. */

@@ -142,5 +142,26 @@ int main() {
   }
   (void)hipFree(du);
   (void)hipFree(dr);
+  {
+    // TwoPunctures driven like the reference (TPID parameters, then one
+    // call per point): the same bits as a solve plus dgr_tp_point
+    dgr_tp_params& tpp = dgr::tp_params();
+    tpp.par_b = 1.5; tpp.give_bare_mass = 1; tpp.par_m_plus = 0.5; tpp.par_m_minus = 0.4;
+    tpp.par_P_plus[1] = 0.12; tpp.par_P_minus[1] = -0.12; tpp.par_S_plus[2] = 0.05;
+    tpp.npoints_A = 12; tpp.npoints_B = 12; tpp.npoints_phi = 6;
+    dgr_tp_t tp = nullptr;
+    int tp_bad = dgr_tp_solve(&tp, &tpp) != DGR_OK;
+    const double pts[3][3] = {{0.3, 0.7, -0.2}, {2.5, -1.0, 0.4}, {-1.2, 0.1, 0.05}};
+    for (int q = 0; q < 3 && !tp_bad; ++q) {
+      double a[24], b[24], mp, mm, mpa, mma, E, J1, J2, J3;
+      TwoPunctures(pts[q][0], pts[q][1], pts[q][2], a, &mp, &mm, &mpa, &mma, &E, &J1, &J2, &J3);
+      tp_bad |= dgr_tp_point(tp, pts[q][0], pts[q][1], pts[q][2], b) != DGR_OK;
+      for (int v = 0; v < 24; ++v) tp_bad |= !(a[v] == b[v]);
+      tp_bad |= !(mp == 0.5 && mm == 0.4 && E > 0.9 && J3 > 0.0);
+    }
+    if (tp) dgr_tp_destroy(tp);
+    printf("TwoPunctures per point: %s\n", tp_bad ? "MISMATCH" : "same bits as dgr_tp_point");
+    fail |= tp_bad;
+  }
   return fail;
 }

@@ -171,3 +171,21 @@ def test_reference_binary_parameters_converge_at_default_resolution(cuda):
         E.append(info["E"])
         tp.close()
     assert 1.0 < E[0] < 1.05 and abs(E[0] - E[1]) < 1e-10
+
+
+def test_host_point_evaluation_matches_the_grid_fill(cuda):
+    # dgr_tp_point (the reference's per-point call, on the host) against the
+    # device fill at sample padded points: host libm vs device math
+    p = tp_params(npoints_A=12, npoints_B=12, npoints_phi=6, center_offset=(0.05, 0.0, -0.03), **BASE)
+    tp = TwoPunctures(p)
+    blocks, dof = layout.uniform_grid(2, dx=0.29)
+    plan, U = evaluate(cuda, tp, blocks, dof)
+    got = U.cpu().numpy()
+    X = grid_points(blocks, dof)
+    rng = np.random.default_rng(5)
+    for o in rng.choice(dof, size=40, replace=False):
+        host = tp.point(*X[:, o])
+        scale = np.abs(got[:, o]).max()
+        assert np.abs(host - got[:, o]).max() <= 1e-13 * scale, o
+    tp.close()
+    plan.close()
