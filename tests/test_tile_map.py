@@ -80,8 +80,15 @@ def test_face_map_groups_closure_points(sides):
     nfaces = sum(1 for x in sides if x)
     assert len(pos) == T ** 3 - (T - 3) ** nfaces * T ** (3 - nfaces)
     waves = -(-len(pos) // 64)
-    # closure points first: at most one extra wave where residues force interleaving,
-    # none for a single face (the three slabs fill waves 0-2)
-    assert max(pos) < 64 * (waves + (0 if nfaces == 1 else 1))
+    if nfaces == 1:
+        # wave w holds the 49 points at distance w (one closure row for the
+        # wave's closure lanes) and 15 points at distance >= 3
+        for w in range(3):
+            d = [face_distance(pts[t], sides) for t in range(64 * w, 64 * w + 64)]
+            assert d.count(w) == 49 and sum(1 for x in d if x >= 3) == 15
+        assert max(pos) < 64 * 3
+        return
+    # closure points first: at most one extra wave where residues force interleaving
+    assert max(pos) < 64 * (waves + 1)
     # the nearest points (one-sided rows) fill wave 0
     assert all(face_distance(pts[t], sides) <= 1 for t in range(64))
