@@ -15,9 +15,17 @@
  * that neighbour's interior (gather mode of the RHS kernel), which is what a
  * refill before it would have copied there.
  *
+ * During one evolution call the state lives in a state buffer of the work
+ * memory laid out like the stage buffer (padded arrays for the ghosts, then
+ * the interior points block after block): packed from the caller's arrays
+ * when the call starts, unpacked into them when it ends.  Every read and
+ * write of the state inside a step then covers whole 64-byte lines instead
+ * of the 7-point row fragments of the padded layout.
+ *
  * Kernels:
  *   k_stage          fused stage combine + running final sum + constraint
  *                    projection (BSSN_GR/src/bssn_constraints.cpp:12-172)
+ *   k_pack           caller's state <-> the state buffer's interior region
  *   k_enforce        the projection alone
  *   k_ghost_refill   padding <- neighbour interiors (uniform level only;
  *                    coarse/fine interpolation is not implemented); run on
@@ -53,8 +61,9 @@ struct Ptrs24 {
   double* v[24];
 };
 
-// RK4 work memory: one allocation holding the stage state (stg) as 24
-// padded arrays of dof doubles each, then in a compact tile-major layout
+// RK4 work memory: one allocation holding the state buffer (h0) and the
+// stage state (stg), each 24 hybrid arrays (dof padded points, then the
+// compact interior region), then in a compact tile-major layout
 // (343 slots per tile in block order, the layout the RHS kernels write in
 // their gather mode) k_1, the running final sum, k_0 (later k_3) and k_2;
 // all addressed arithmetically so that only the caller's state needs a
@@ -151,7 +160,10 @@ __device__ __forceinline__ bool tile_point(const DevTile& tl, const DevBlock& B,
 __constant__ double c_rkU[3] = {1.0 / 2.0, 1.0 / 2.0, 1.0};
 __constant__ double c_rkC[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
 
-// Stage s of the RK4 step, one workgroup per tile.  gdt[g] is the time step
+// Stage s of the RK4 step, one workgroup per tile.  `state` is the
+// evolver's state buffer: during a run the state lives in its compact
+// interior region (whole lines), unpacked into the caller's arrays when the
+// run ends.  gdt[g] is the time step
 // of block group g; the coefficients are formed as the reference forms them
 // (RK4_U[s+1] * dt, RK4_C[s] * dt) and then multiply k.  u_prev is always
 // the state at the start of the step; KIND fixes whether the running sum
@@ -184,7 +196,7 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
       AccCursor c(W.kc, co);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
-        un[v] = state.v[v][o];  // u_prev: the state at the start of the step
+        un[v] = state.v[v][W.dof + oc];  // u_prev: the state at the start of the step
         kk[v] = c.acc();
         c.next(W.nacc);
       }
@@ -200,7 +212,7 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
       AccCursor a(W.kx, co);
 #pragma unroll
       for (int v = 0; v < 24; ++v) {
-        un[v] = state.v[v][o];  // u_prev: the state at the start of the step
+        un[v] = state.v[v][W.dof + oc];  // u_prev: the state at the start of the step
         kk[v] = c.acc();
         acc[v] = a.acc();       // k_0
         c.next(W.nacc);
@@ -250,7 +262,7 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
     if (bad) atomicOr(flag, (unsigned)bad);
     if (KIND == ST_LAST) {
 #pragma unroll
-      for (int v = 0; v < 24; ++v) __builtin_nontemporal_store(un[v], (gdouble*)&state.v[v][o]);
+      for (int v = 0; v < 24; ++v) __builtin_nontemporal_store(un[v], (gdouble*)&state.v[v][W.dof + oc]);
     } else {
       WorkCursor d(W, W.dof + oc);
 #pragma unroll
@@ -259,6 +271,27 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
         d.next(W.sstride);
       }
     }
+  }
+}
+
+// The caller's state into the state buffer's compact region (PACK), or
+// back into the caller's interior points.  One workgroup per tile.
+template <bool PACK>
+__global__ void __launch_bounds__(384) k_pack(const DevBlock* __restrict__ blocks, const DevTile* __restrict__ tiles,
+                                             int ntiles, Ptrs24 u, Ptrs24 buf, uint64_t cdof) {
+  const int w = blockIdx.x;
+  if (w >= ntiles) return;
+  const DevTile tl = tiles[w];
+  const DevBlock& B = blocks[tl.block];
+  uint64_t o, oc;
+  if (!tile_point(tl, B, threadIdx.x, o, &oc)) return;
+  double t[24];  // all loads before the stores
+#pragma unroll
+  for (int v = 0; v < 24; ++v) t[v] = PACK ? u.v[v][o] : buf.v[v][cdof + oc];
+#pragma unroll
+  for (int v = 0; v < 24; ++v) {
+    if (PACK) buf.v[v][cdof + oc] = t[v];
+    else __builtin_nontemporal_store(t[v], (gdouble*)&u.v[v][o]);
   }
 }
 
@@ -323,9 +356,9 @@ int grid_for(dgr_plan_t plan, int n) {
 struct dgr_rk4_s {
   dgr_plan_t plan = nullptr;
   uint64_t dof = 0;
-  double* buf = nullptr;        // stage state (24 x sstride), k, running sum, k_0 (24 x 343 per tile each)
+  double* buf = nullptr;        // state buffer, stage state (24 x sstride each), k, running sum, k_0 (24 x 343 per tile each)
   uint64_t sstride = 0;         // stage state per variable: dof padded points, then the compact interior region
-  Ptrs24 state{}, stg{}, k{};  // the state and the work arrays in buf
+  Ptrs24 state{}, h0{}, stg{}, k{};  // the caller's state; the state buffer, stage buffer and k arrays in buf
   uint64_t nacc = 0;           // compact running-sum slots per variable (343 per tile)
   IOPtrs* d_io = nullptr;       // RHS pointer table of each stage (input -> its k array)
   double* d_gdt = nullptr;      // time step per block group
@@ -345,33 +378,34 @@ int enqueue_step(dgr_rk4_s* r, hipStream_t st) {
   dgr_plan_t p = r->plan;
   const int nt = (int)p->ntiles;
   for (int s = 0; s < 4; ++s) {
-    // k_s = RHS(u_s): stage 0 reads the state, later stages the stage
-    // buffer; k_s goes to the array of table s (k_1, k_0 / k_3, k_2)
-    int rc = launch_rhs(p, r->d_io + s, st, true, s > 0);
+    // k_s = RHS(u_s): stage 0 reads the state buffer, later stages the
+    // stage buffer (both hybrid: whole-line compact interiors); k_s goes to
+    // the array of table s (k_1, k_0 / k_3, k_2)
+    int rc = launch_rhs(p, r->d_io + s, st, true, true);
     if (rc) return rc;
     // stage combine + projection; the next RHS reads ghosts from the
     // neighbours' interiors (gather), so no refill runs between stages
-    double* const c0 = r->buf + 24 * r->sstride;
+    double* const c0 = r->buf + 48 * r->sstride;
     double* const k1 = c0, *const acc = c0 + 24 * r->nacc, *const k03 = c0 + 48 * r->nacc, *const k2 = c0 + 72 * r->nacc;
     double* const kc = s == 0 || s == 3 ? k03 : (s == 1 ? k1 : k2);
-    const Work W = {r->buf, r->dof, r->sstride, kc, acc, r->nacc, s == 1 ? k03 : k2};
+    const Work W = {r->stg.v[0], r->dof, r->sstride, kc, acc, r->nacc, s == 1 ? k03 : k2};
     const dim3 g(std::max(nt, 1)), b(384);
     if (s == 0)
-      hipLaunchKernelGGL(k_stage<ST_FIRST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
+      hipLaunchKernelGGL(k_stage<ST_FIRST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->h0, W, r->d_gdt,
                          p->chi_floor, s, r->d_flag);
     else if (s == 1)
-      hipLaunchKernelGGL(k_stage<ST_SECOND>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
+      hipLaunchKernelGGL(k_stage<ST_SECOND>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->h0, W, r->d_gdt,
                          p->chi_floor, s, r->d_flag);
     else if (s == 2)
-      hipLaunchKernelGGL(k_stage<ST_THIRD>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
+      hipLaunchKernelGGL(k_stage<ST_THIRD>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->h0, W, r->d_gdt,
                          p->chi_floor, s, r->d_flag);
     else
-      hipLaunchKernelGGL(k_stage<ST_LAST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->state, W, r->d_gdt,
+      hipLaunchKernelGGL(k_stage<ST_LAST>, g, b, 0, st, p->d_blocks, p->d_tiles_blk, nt, r->h0, W, r->d_gdt,
                          p->chi_floor, s, r->d_flag);
     // ghosts across level jumps (adaptive layouts) of the new stage state
     OutPtrs o;
-    for (int v = 0; v < 24; ++v) o.v[v] = s < 3 ? r->stg.v[v] : r->state.v[v];
-    rc = launch_level_fill(p, o, st, s < 3);
+    for (int v = 0; v < 24; ++v) o.v[v] = s < 3 ? r->stg.v[v] : r->h0.v[v];
+    rc = launch_level_fill(p, o, st, true);
     if (rc) return rc;
   }
   DGR_HIP(hipGetLastError());
@@ -450,11 +484,12 @@ extern "C" int dgr_enforce_constraints(dgr_plan_t plan, double* const* u, void* 
   return flag ? DGR_E_CONSTRAINT : DGR_OK;
 }
 
-// stage state (per variable the padded layout, whose ghosts the level fills
-// and the reset write, then the interior points in the compact region the
-// stage kernels write); k_1, the running sum, k_0 / k_3 and k_2 compact
+// state buffer and stage buffer (each per variable the padded layout, whose
+// ghosts the level fills, the pack and the reset write, then the interior
+// points in the compact region the stage kernels write); k_1, the running
+// sum, k_0 / k_3 and k_2 compact
 extern "C" uint64_t dgr_rk4_work_doubles(dgr_plan_t plan) {
-  return plan ? 24ull * (plan->dof_end + plan->ncomp) + 96ull * 343ull * plan->ntiles : 0;
+  return plan ? 48ull * (plan->dof_end + plan->ncomp) + 96ull * 343ull * plan->ntiles : 0;
 }
 
 extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double* const* u, double* work,
@@ -480,15 +515,16 @@ extern "C" int dgr_rk4_create_with_work(dgr_rk4_t* out, dgr_plan_t plan, double*
   Ptrs24& stg = r->stg;
   Ptrs24& k = r->k;
   for (int v = 0; v < 24; ++v) {
-    stg.v[v] = r->buf + (size_t)v * r->sstride;
-    k.v[v] = r->buf + (size_t)24 * r->sstride + (size_t)v * r->nacc;
+    r->h0.v[v] = r->buf + (size_t)v * r->sstride;
+    stg.v[v] = r->buf + (size_t)(24 + v) * r->sstride;
+    k.v[v] = r->buf + (size_t)48 * r->sstride + (size_t)v * r->nacc;
   }
-  // RHS of stage s: input (the state, then the stage buffer) -> k_1,
+  // RHS of stage s: input (the state buffer, then the stage buffer) -> k_1,
   // k_0 / k_3 or k_2 (compact arrays after the stage buffer)
   IOPtrs io[4];
-  double* const kb = r->buf + (size_t)24 * r->sstride;
+  double* const kb = r->buf + (size_t)48 * r->sstride;
   for (int v = 0; v < 24; ++v) {
-    io[0].u.v[v] = r->state.v[v];
+    io[0].u.v[v] = r->h0.v[v];
     io[0].r.v[v] = kb + (size_t)(48 + v) * r->nacc;  // k_0
     io[1].u.v[v] = stg.v[v];
     io[1].r.v[v] = k.v[v];                           // k_1
@@ -571,6 +607,20 @@ extern "C" int dgr_rk4_run(dgr_rk4_t r, int nsteps, void* stream) {
       if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
     }
   }
+  const int nt = (int)r->plan->ntiles;
+  if (rc == DGR_OK && nsteps > 0) {
+    // the caller's state into the state buffer: the padded arrays (the
+    // ghosts no fill reaches keep their values) and the compact interior
+    const size_t nb = sizeof(double) * r->dof;
+    for (int v = 0; v < 24 && e == hipSuccess; ++v)
+      e = hipMemcpyAsync(r->h0.v[v], r->state.v[v], nb, hipMemcpyDeviceToDevice, r->st);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_pack<true>, dim3(std::max(nt, 1)), dim3(384), 0, r->st, r->plan->d_blocks,
+                         r->plan->d_tiles_blk, nt, r->state, r->h0, (uint64_t)r->dof);
+      e = hipGetLastError();
+    }
+    if (e != hipSuccess) rc = hip_fail(e, "dgr_rk4_run: pack");
+  }
   for (int n = 0; rc == DGR_OK && n < nsteps; ++n) {
     if (r->use_graph) {
       e = hipGraphLaunch(r->exec, r->st);
@@ -580,12 +630,23 @@ extern "C" int dgr_rk4_run(dgr_rk4_t r, int nsteps, void* stream) {
     }
   }
   if (rc == DGR_OK && nsteps > 0) {
-    // the state's ghost layers, left stale by the gathering stages, are
-    // refilled once at the end of the run
-    hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(r->plan, (int)r->plan->nblocks)), dim3(RK_NT), 0, r->st,
-                       r->plan->d_blocks, r->plan->d_nbr, (int)r->plan->nblocks, r->state);
+    // the new state back into the caller's interior points; its ghost
+    // layers, last written before the run, are refilled once: same-level
+    // ghosts from the neighbours' interiors, then the level jumps
+    hipLaunchKernelGGL(k_pack<false>, dim3(std::max(nt, 1)), dim3(384), 0, r->st, r->plan->d_blocks,
+                       r->plan->d_tiles_blk, nt, r->state, r->h0, (uint64_t)r->dof);
     e = hipGetLastError();
-    if (e != hipSuccess) rc = hip_fail(e, "k_ghost_refill");
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(r->plan, (int)r->plan->nblocks)), dim3(RK_NT), 0, r->st,
+                         r->plan->d_blocks, r->plan->d_nbr, (int)r->plan->nblocks, r->state);
+      e = hipGetLastError();
+    }
+    if (e != hipSuccess) rc = hip_fail(e, "dgr_rk4_run: unpack");
+    if (rc == DGR_OK) {
+      OutPtrs o;
+      for (int v = 0; v < 24; ++v) o.v[v] = r->state.v[v];
+      rc = launch_level_fill(r->plan, o, r->st, false);
+    }
   }
   if (rc == DGR_OK) {
     e = hipEventRecord(ev, r->st);

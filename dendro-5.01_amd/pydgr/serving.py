@@ -135,11 +135,11 @@ class PoolAllocator:
 
 def slot_doubles(job):
     """Pool doubles of a slot for `job`: state (24 x dof) + RK4 work
-    (dgr_rk4_work_doubles = 24 x (dof + interior points) + 96 x 343 per
+    (dgr_rk4_work_doubles = 48 x (dof + interior points) + 96 x 343 per
     tile; a 13^3 block is one tile of 343 interior points; Slot checks it
     against the library's figure)."""
     job.build()
-    return 24 * job.dof + 24 * (job.dof + 343 * job.nblocks) + 96 * 343 * job.nblocks
+    return 24 * job.dof + 48 * (job.dof + 343 * job.nblocks) + 96 * 343 * job.nblocks
 
 
 class Slot:

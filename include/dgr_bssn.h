@@ -321,15 +321,17 @@ int dgr_tp_destroy(dgr_tp_t tp);
  * step is captured into a hipGraph on first use (use_graph != 0) and
  * replayed.  The RHS of stages 1-4 reads ghost points that have a same-level
  * neighbour from that neighbour's interior, which gives the same values as a
- * refill after each stage; the state's own ghost layers are refilled once at
- * the end of every dgr_rk4_run / _step / _steps call.  Work memory:
- * dgr_rk4_work_doubles(plan) doubles. */
+ * refill after each stage.  Within one dgr_rk4_run / _step / _steps call
+ * the state lives in the work memory; the call ends by writing it back into
+ * u and refilling u's ghost layers (same-level, then level jumps).  Work
+ * memory: dgr_rk4_work_doubles(plan) doubles. */
 typedef struct dgr_rk4_s* dgr_rk4_t;
 int dgr_rk4_create(dgr_rk4_t* rk, dgr_plan_t plan, double* const* u, int use_graph);
 /* Same, with caller-owned work memory of at least dgr_rk4_work_doubles(plan)
- * doubles = 24 x (dof + interior points) (the stage state: per variable the
- * padded layout, then every block's interior points compact) + 4 x 24 x 343
- * x tiles (k and the running sum in the compact tile-major layout). */
+ * doubles = 48 x (dof + interior points) (the state during a call and the
+ * stage state: per variable the padded layout, then every block's interior
+ * points compact) + 4 x 24 x 343 x tiles (k and the running sum in the
+ * compact tile-major layout). */
 uint64_t dgr_rk4_work_doubles(dgr_plan_t plan);
 int dgr_rk4_create_with_work(dgr_rk4_t* rk, dgr_plan_t plan, double* const* u, double* work, int use_graph);
 /* Copy the state into the stage buffer, stream-ordered on `stream`.  Ghost

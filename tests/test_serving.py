@@ -55,12 +55,12 @@ def test_job_results_identical_alone_and_in_mix(cuda):
                 Job(job_id=2, arrival=0.0, nb=2, steps=4, kind="bl", mass_ratio=1.3),
                 Job(job_id=3, arrival=0.0, nb=2, steps=1, kind="flat", seed=5)]
     # budget for the 27-block job and one 8-block job at a time
-    budget = 40 * 10 ** 6
+    budget = 60 * 10 ** 6
     mix = mk()
     srv = Server(budget, device=cuda)
     stats = srv.run(mix, realtime=False)
     srv.close()
-    # the three geometries (47.7 MB) never fit together: a job waited for
+    # the three geometries (71.4 MB) never fit together: a job waited for
     # memory or a free slot of another geometry was evicted
     assert stats["jobs"] == 4 and stats["admission_waits"] + stats["slots_evicted"] >= 1, stats
     for j in mk():
@@ -78,8 +78,8 @@ def test_same_geometry_reuses_its_slot_and_partial_grids(cuda):
     corners keep the job's own initial values, so a job gives the same bits
     in a reused slot as in a fresh one."""
     jobs = [Job(job_id=i, arrival=0.0, nblocks=10, steps=2, kind="flat", seed=i) for i in range(3)]
-    # a budget for one slot of this geometry (11 MB): the jobs take turns in it
-    srv = Server(15 * 10 ** 6, device=cuda)
+    # a budget for one slot of this geometry (16.6 MB): the jobs take turns in it
+    srv = Server(20 * 10 ** 6, device=cuda)
     stats = srv.run(jobs, realtime=False)
     srv.close()
     assert stats["jobs"] == 3 and stats["slots_built"] == 1 and stats["admission_waits"] >= 1, stats
