@@ -25,11 +25,12 @@
  * Kernels:
  *   k_stage          fused stage combine + running final sum + constraint
  *                    projection (BSSN_GR/src/bssn_constraints.cpp:12-172)
- *   k_pack           caller's state <-> the state buffer's interior region
+ *   k_hybrid         caller's state <-> the state buffer (pack at the start
+ *                    of an evolution call; at its end unpack with the
+ *                    same-level ghost refill fused in)
  *   k_enforce        the projection alone
- *   k_ghost_refill   padding <- neighbour interiors (uniform level only;
- *                    coarse/fine interpolation is not implemented); run on
- *                    the state once at the end of each evolution call
+ *   k_ghost_refill   padding <- neighbour interiors (same-level ghosts;
+ *                    dgr_ghost_refill)
  * One step (four RHS + stage launches) is captured once into a hipGraph and
  * replayed.
  */
@@ -274,24 +275,57 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
   }
 }
 
-// The caller's state into the state buffer's compact region (PACK), or
-// back into the caller's interior points.  One workgroup per tile.
+// Between the caller's padded arrays u and the state buffer h, one block
+// per workgroup iteration.  cdelta is the plan's stage-buffer direction
+// table: per block and ghost direction the index distance to the same point
+// of a neighbour's (or for direction 13 the block's own) interior in the
+// compact region, 0 where no same-level neighbour exists.
+// PACK: interior points to the compact region, and the ghosts without a
+// same-level neighbour that the stages read from the padding (level jumps,
+// concave corners) to the padded part.  Ghosts beyond a flagged face are
+// staged as zeros by the RHS whatever the padding holds, and are skipped.
+// UNPACK: the interior points, and every ghost that has a same-level
+// neighbour from that neighbour's interior (the same-level refill); the
+// other ghosts are left to the level fills or keep their values.
 template <bool PACK>
-__global__ void __launch_bounds__(384) k_pack(const DevBlock* __restrict__ blocks, const DevTile* __restrict__ tiles,
-                                             int ntiles, Ptrs24 u, Ptrs24 buf, uint64_t cdof) {
-  const int w = blockIdx.x;
-  if (w >= ntiles) return;
-  const DevTile tl = tiles[w];
-  const DevBlock& B = blocks[tl.block];
-  uint64_t o, oc;
-  if (!tile_point(tl, B, threadIdx.x, o, &oc)) return;
-  double t[24];  // all loads before the stores
+__global__ void __launch_bounds__(RK_NT) k_hybrid(const DevBlock* __restrict__ blocks,
+                                                  const long long* __restrict__ cdelta, int nblocks, Ptrs24 u,
+                                                  Ptrs24 h) {
+  for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const DevBlock& B = blocks[b];
+    const int s0 = B.sz[0], s1 = B.sz[1], s2 = B.sz[2];
+    const int n = s0 * s1 * s2;
+    for (int e = threadIdx.x; e < n; e += RK_NT) {
+      const int i = e % s0, j = (e / s0) % s1, k = e / (s0 * s1);
+      const int dx = i < 3 ? 0 : (i >= s0 - 3 ? 2 : 1);
+      const int dy = j < 3 ? 0 : (j >= s1 - 3 ? 2 : 1);
+      const int dz = k < 3 ? 0 : (k >= s2 - 3 ? 2 : 1);
+      const int dir = dx + 3 * dy + 9 * dz;
+      const long long d = cdelta[(size_t)b * NNBR + dir];
+      const uint64_t o = B.offset + e;
+      // the point's slot in the compact region (through a neighbour's
+      // interior for a ghost), valid where d != 0
+      const uint64_t c = (uint64_t)((long long)B.offset + (i - 3) +
+                                    (long long)(s0 - 6) * ((j - 3) + (long long)(s1 - 6) * (k - 3)) + d);
+      double t[24];
+      if (PACK) {
+        const unsigned f = B.bflag;
+        const bool beyond = ((f & 1u) && i < 3) || ((f & 2u) && i >= s0 - 3) || ((f & 4u) && j < 3) ||
+                            ((f & 8u) && j >= s1 - 3) || ((f & 16u) && k < 3) || ((f & 32u) && k >= s2 - 3);
+        if (dir != 13 && (d != 0 || beyond)) continue;
+        const uint64_t q = dir == 13 ? c : o;
 #pragma unroll
-  for (int v = 0; v < 24; ++v) t[v] = PACK ? u.v[v][o] : buf.v[v][cdof + oc];
+        for (int v = 0; v < 24; ++v) t[v] = u.v[v][o];
 #pragma unroll
-  for (int v = 0; v < 24; ++v) {
-    if (PACK) buf.v[v][cdof + oc] = t[v];
-    else __builtin_nontemporal_store(t[v], (gdouble*)&u.v[v][o]);
+        for (int v = 0; v < 24; ++v) h.v[v][q] = t[v];
+      } else {
+        if (d == 0) continue;
+#pragma unroll
+        for (int v = 0; v < 24; ++v) t[v] = h.v[v][c];
+#pragma unroll
+        for (int v = 0; v < 24; ++v) u.v[v][o] = t[v];
+      }
+    }
   }
 }
 
@@ -607,18 +641,12 @@ extern "C" int dgr_rk4_run(dgr_rk4_t r, int nsteps, void* stream) {
       if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
     }
   }
-  const int nt = (int)r->plan->ntiles;
+  const int nb = (int)r->plan->nblocks;
   if (rc == DGR_OK && nsteps > 0) {
-    // the caller's state into the state buffer: the padded arrays (the
-    // ghosts no fill reaches keep their values) and the compact interior
-    const size_t nb = sizeof(double) * r->dof;
-    for (int v = 0; v < 24 && e == hipSuccess; ++v)
-      e = hipMemcpyAsync(r->h0.v[v], r->state.v[v], nb, hipMemcpyDeviceToDevice, r->st);
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_pack<true>, dim3(std::max(nt, 1)), dim3(384), 0, r->st, r->plan->d_blocks,
-                         r->plan->d_tiles_blk, nt, r->state, r->h0, (uint64_t)r->dof);
-      e = hipGetLastError();
-    }
+    // the caller's state into the state buffer
+    hipLaunchKernelGGL(k_hybrid<true>, dim3(grid_for(r->plan, nb)), dim3(RK_NT), 0, r->st, r->plan->d_blocks,
+                       r->plan->d_cdelta, nb, r->state, r->h0);
+    e = hipGetLastError();
     if (e != hipSuccess) rc = hip_fail(e, "dgr_rk4_run: pack");
   }
   for (int n = 0; rc == DGR_OK && n < nsteps; ++n) {
@@ -630,17 +658,11 @@ extern "C" int dgr_rk4_run(dgr_rk4_t r, int nsteps, void* stream) {
     }
   }
   if (rc == DGR_OK && nsteps > 0) {
-    // the new state back into the caller's interior points; its ghost
-    // layers, last written before the run, are refilled once: same-level
-    // ghosts from the neighbours' interiors, then the level jumps
-    hipLaunchKernelGGL(k_pack<false>, dim3(std::max(nt, 1)), dim3(384), 0, r->st, r->plan->d_blocks,
-                       r->plan->d_tiles_blk, nt, r->state, r->h0, (uint64_t)r->dof);
+    // the new state back into the caller's arrays, interiors and same-level
+    // ghosts in one pass, then the ghosts at level jumps
+    hipLaunchKernelGGL(k_hybrid<false>, dim3(grid_for(r->plan, nb)), dim3(RK_NT), 0, r->st, r->plan->d_blocks,
+                       r->plan->d_cdelta, nb, r->state, r->h0);
     e = hipGetLastError();
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(r->plan, (int)r->plan->nblocks)), dim3(RK_NT), 0, r->st,
-                         r->plan->d_blocks, r->plan->d_nbr, (int)r->plan->nblocks, r->state);
-      e = hipGetLastError();
-    }
     if (e != hipSuccess) rc = hip_fail(e, "dgr_rk4_run: unpack");
     if (rc == DGR_OK) {
       OutPtrs o;
