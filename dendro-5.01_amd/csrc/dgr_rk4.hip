@@ -287,11 +287,15 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
 // UNPACK: the interior points, and every ghost that has a same-level
 // neighbour from that neighbour's interior (the same-level refill); the
 // other ghosts are left to the level fills or keep their values.
+// XCD-aware: workgroup w runs on XCD w % 8, so the grid (a multiple of 8)
+// is renumbered to give each XCD a contiguous run of blocks, whose
+// neighbours' interiors the unpack reads from the same L2.
 template <bool PACK>
 __global__ void __launch_bounds__(RK_NT) k_hybrid(const DevBlock* __restrict__ blocks,
                                                   const long long* __restrict__ cdelta, int nblocks, Ptrs24 u,
                                                   Ptrs24 h) {
-  for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+  const int first = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+  for (int b = first; b < nblocks; b += gridDim.x) {
     const DevBlock& B = blocks[b];
     const int s0 = B.sz[0], s1 = B.sz[1], s2 = B.sz[2];
     const int n = s0 * s1 * s2;
@@ -384,6 +388,8 @@ int grid_for(dgr_plan_t plan, int n) {
   const int g = plan->grid * 64;
   return n < g ? (n > 0 ? n : 1) : g;
 }
+// The same, rounded up to a multiple of the 8 XCDs (k_hybrid).
+int grid_xcd(dgr_plan_t plan, int n) { return (grid_for(plan, n) + 7) / 8 * 8; }
 
 }  // namespace
 
@@ -644,7 +650,7 @@ extern "C" int dgr_rk4_run(dgr_rk4_t r, int nsteps, void* stream) {
   const int nb = (int)r->plan->nblocks;
   if (rc == DGR_OK && nsteps > 0) {
     // the caller's state into the state buffer
-    hipLaunchKernelGGL(k_hybrid<true>, dim3(grid_for(r->plan, nb)), dim3(RK_NT), 0, r->st, r->plan->d_blocks,
+    hipLaunchKernelGGL(k_hybrid<true>, dim3(grid_xcd(r->plan, nb)), dim3(RK_NT), 0, r->st, r->plan->d_blocks,
                        r->plan->d_cdelta, nb, r->state, r->h0);
     e = hipGetLastError();
     if (e != hipSuccess) rc = hip_fail(e, "dgr_rk4_run: pack");
@@ -660,7 +666,7 @@ extern "C" int dgr_rk4_run(dgr_rk4_t r, int nsteps, void* stream) {
   if (rc == DGR_OK && nsteps > 0) {
     // the new state back into the caller's arrays, interiors and same-level
     // ghosts in one pass, then the ghosts at level jumps
-    hipLaunchKernelGGL(k_hybrid<false>, dim3(grid_for(r->plan, nb)), dim3(RK_NT), 0, r->st, r->plan->d_blocks,
+    hipLaunchKernelGGL(k_hybrid<false>, dim3(grid_xcd(r->plan, nb)), dim3(RK_NT), 0, r->st, r->plan->d_blocks,
                        r->plan->d_cdelta, nb, r->state, r->h0);
     e = hipGetLastError();
     if (e != hipSuccess) rc = hip_fail(e, "dgr_rk4_run: unpack");
