@@ -15,8 +15,8 @@
  *                spacing, so each axis needs the 7 Lagrange weights of the
  *                coarse nodes at one of the 13 positions 0, 1/2, ..., 6.  The
  *                box is interpolated separably, x then y then z, in LDS
- *                (k_fill_interp, one workgroup per group of up to 8 boxes
- *                interpolated from one coarse element);
+ *                (k_fill_interp, one wave per variable of a group of up to
+ *                8 boxes interpolated from one coarse element);
  *   coarse side  every ghost point coincides with a node of a finer block
  *                (coarse spacing = 2 fine spacings) and copies it
  *                (k_fill_copy).
@@ -61,73 +61,100 @@ __device__ __forceinline__ void lagrange_table(double (*W)[7], int t) {
   }
 }
 
-// Four variables per pass: four times the work between barriers.
-constexpr int VB = 4;
+// Waves per interpolation workgroup: each wave interpolates one variable of
+// one group on its own, so its sweeps synchronise only within the wave.
+constexpr int FW = NF / 64;
 
-// One workgroup per group of boxes interpolated from one coarse element
-// (items[groups[g]] .. items[groups[g+1]-1]): each pass stages VB variables
-// of the element once and interpolates every box of the group from it.
+// Lane-level order of a wave's LDS accesses: a wave's LDS operations run in
+// issue order, so only the compiler has to be kept from moving them across
+// this point (and the values other lanes wrote made visible to this one).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One task per wave: variable v of the group of boxes interpolated from one
+// coarse element (items[groups[g]] .. items[groups[g+1]-1]), task g * 24 + v,
+// so the waves of a workgroup take neighbouring variables of one element.
+// Lane (b, c) < 49 of the wave holds the element's x line (b, c) of v in
+// registers for the whole group; every box of the group is interpolated
+// from them, x then y then z, through two LDS buffers of the wave.
 __global__ void __launch_bounds__(NF) k_fill_interp(const FillItem* __restrict__ items,
                                                      const uint32_t* __restrict__ groups, uint32_t ngroups,
                                                      OutPtrs u, int stage) {
   __shared__ double W[13][7];
-  __shared__ double sC[VB][343], sX[VB][343], sY[VB][343];
-  const int t = threadIdx.x;
-  lagrange_table(W, t);
-  for (uint32_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+  __shared__ double sX[FW][343], sY[FW][343];
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+  lagrange_table(W, (int)threadIdx.x);
+  __syncthreads();
+  double* const X = sX[wv];
+  double* const Y = sY[wv];
+  const uint64_t ntask = (uint64_t)ngroups * 24;
+  for (uint64_t task = (uint64_t)blockIdx.x * FW + wv; task < ntask; task += (uint64_t)gridDim.x * FW) {
+    const uint32_t g = (uint32_t)(task / 24);
+    const int v = (int)(task % 24);
     const uint32_t i0 = groups[g], i1 = groups[g + 1];
     const FillItem E = items[i0];  // src and ssz are the group's
     // the element's nodes: padded layout, or a stage buffer's compact region
     const uint64_t es = stage ? E.csrc : E.src;
     const uint64_t e0 = stage ? E.ssz[0] - 6 : E.ssz[0], e1 = stage ? E.ssz[1] - 6 : E.ssz[1];
-    for (int v0 = 0; v0 < 24; v0 += VB) {
-      __syncthreads();  // previous pass (or group) done with sC, sX, sY; W visible
-      for (int e = t; e < VB * 343; e += NF) {
-        const int w = e / 343, m = e % 343;
-        const gdouble* src = (const gdouble*)u.v[v0 + w] + es;
-        sC[w][m] = src[(m % 7) + e0 * ((m / 7) % 7 + e1 * (m / 49))];
-      }
-      __syncthreads();
-      for (uint32_t it = i0; it < i1; ++it) {
-        const FillItem I = items[it];
-        const int nx = I.n[0], ny = I.n[1], nz = I.n[2];
-        const int nX = nx * 49, nY = nx * ny * 7, nZ = nx * ny * nz;
-        // along x: X[p][b][c] = sum_a W[s0x + p][a] C[a][b][c]
-        // (the previous box's y sweep, the last reader of sX, ended at a barrier)
-        for (int e = t; e < VB * nX; e += NF) {
-          const int w = e / nX, f = e % nX;
-          const int p = f % nx, b = (f / nx) % 7, c = f / (nx * 7);
+    const gdouble* src = (const gdouble*)u.v[v] + es;
+    double line[7];  // C[a][b][c], a = 0..6, of lane (b, c)
+    if (lane < 49) {
+      const int b = lane % 7, c = lane / 7;
+#pragma unroll
+      for (int a = 0; a < 7; ++a) line[a] = src[a + e0 * (b + e1 * c)];
+    }
+    for (uint32_t it = i0; it < i1; ++it) {
+      const FillItem I = items[it];
+      const int nx = I.n[0], ny = I.n[1], nz = I.n[2];
+      // One lane per line along the sweep's axis: the lane holds the
+      // line's 7 values and forms every output of the line, with weights
+      // that are the same for the whole wave (LDS broadcasts).
+      // along x: X[p][b][c] = sum_a W[s0x + p][a] C[a][b][c]; lane = (b, c)
+      if (lane < 49) {
+        const int b = lane % 7, c = lane / 7;
+        for (int p = 0; p < nx; ++p) {
           const double* wt = W[I.s0[0] + p];
-          const double* cc = sC[w] + 7 * b + 49 * c;
           double r = 0.0;
 #pragma unroll
-          for (int a = 0; a < 7; ++a) r += wt[a] * cc[a];
-          sX[w][f] = r;
+          for (int a = 0; a < 7; ++a) r += wt[a] * line[a];
+          X[p + nx * (b + 7 * c)] = r;
         }
-        __syncthreads();
-        // along y: Y[p][q][c] = sum_b W[s0y + q][b] X[p][b][c]
-        for (int e = t; e < VB * nY; e += NF) {
-          const int w = e / nY, f = e % nY;
-          const int p = f % nx, q = (f / nx) % ny, c = f / (nx * ny);
+      }
+      wave_sync();
+      // along y: Y[p][q][c] = sum_b W[s0y + q][b] X[p][b][c]; lane = (p, c)
+      if (lane < nx * 7) {
+        const int p = lane % nx, c = lane / nx;
+        double col[7];
+#pragma unroll
+        for (int b = 0; b < 7; ++b) col[b] = X[p + nx * (b + 7 * c)];
+        for (int q = 0; q < ny; ++q) {
           const double* wt = W[I.s0[1] + q];
           double r = 0.0;
 #pragma unroll
-          for (int b = 0; b < 7; ++b) r += wt[b] * sX[w][p + nx * (b + 7 * c)];
-          sY[w][f] = r;
+          for (int b = 0; b < 7; ++b) r += wt[b] * col[b];
+          Y[p + nx * (q + ny * c)] = r;
         }
-        __syncthreads();
-        // along z, into the fine block's padding
-        for (int e = t; e < VB * nZ; e += NF) {
-          const int w = e / nZ, f = e % nZ;
-          const int p = f % nx, q = (f / nx) % ny, r3 = f / (nx * ny);
+      }
+      wave_sync();
+      // along z, into the fine block's padding; lane = (p, q)
+      gdouble* dst = (gdouble*)u.v[v] + I.dst;
+      if (lane < nx * ny) {
+        const int p = lane % nx, q = lane / nx;
+        double col[7];
+#pragma unroll
+        for (int c = 0; c < 7; ++c) col[c] = Y[p + nx * (q + ny * c)];
+        for (int r3 = 0; r3 < nz; ++r3) {
           const double* wt = W[I.s0[2] + r3];
           double r = 0.0;
 #pragma unroll
-          for (int c = 0; c < 7; ++c) r += wt[c] * sY[w][p + nx * (q + ny * c)];
-          gdouble* dst = (gdouble*)u.v[v0 + w] + I.dst;
+          for (int c = 0; c < 7; ++c) r += wt[c] * col[c];
           dst[p + (uint64_t)I.dsz[0] * (q + (uint64_t)I.dsz[1] * r3)] = r;
         }
       }
+      wave_sync();  // the next box (or task) rewrites X and Y
     }
   }
 }
@@ -333,8 +360,9 @@ int dgr_internal::launch_level_fill(dgr_plan_t plan, const OutPtrs& u, hipStream
                        u, (int)stage);
   }
   if (plan->nfill_items) {
-    const uint32_t g = std::min<uint32_t>(plan->nfill_groups, 16384);
-    hipLaunchKernelGGL(k_fill_interp, dim3(g), dim3(NF), 0, st, plan->d_fill_items, plan->d_fill_groups,
+    // one wave per group and variable
+    const uint64_t g = std::min<uint64_t>(((uint64_t)plan->nfill_groups * 24 + FW - 1) / FW, 65536);
+    hipLaunchKernelGGL(k_fill_interp, dim3((unsigned)g), dim3(NF), 0, st, plan->d_fill_items, plan->d_fill_groups,
                        plan->nfill_groups, u, (int)stage);
   }
   DGR_HIP(hipGetLastError());
