@@ -592,8 +592,9 @@ def main():
         "data": "synthetic (seeded gauge wave + 1e-6 white noise on padded blocks)",
         "config": {"workload": "uniform grid: %d blocks of 13^3 (one order-6 element each), gauge wave, one full RK4 "
                                "step per step (hipGraph): 4 RHS (each gathering same-level ghosts from neighbour "
-                               "interiors) + 4 stage combines with the constraint projection; one same-level ghost "
-                               "refill of the state per run() call" % len(blocks),
+                               "interiors) + 4 stage combines with the constraint projection; per run() call of "
+                               "the timed steps, one pack of the state into the evolver's hybrid state buffer and "
+                               "one unpack with the same-level ghost refill" % len(blocks),
                    "blocks": len(blocks), "block_size": 13, "interior_points": pts, "ko_sigma": 0.4,
                    "cfl": 0.25, "dx": dx, "deriv": "deriv42"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
