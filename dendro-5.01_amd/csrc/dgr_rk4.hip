@@ -723,10 +723,12 @@ struct EtsTerms {
 
 // out = base + sum_q c[q] k_{idx[q]} on the interior points of every tile;
 // k_p variable v in the compact layout of the RHS in gather mode, at
-// kb + (p*24 + v) * nacc + (tile * 343 + local index).  PROJECT: the stage
-// input, projected (enforce_bssn_constraints) before it is stored; a point
-// the projection rejects (det gt < 0) keeps the combined values, as the
-// projection leaves such a node untouched.
+// kb + (p*24 + v) * nacc + (tile * 343 + local index); base is read at the
+// padded points, out written to the compact interior region of a stage
+// buffer (cdof doubles past each variable's padded points).  PROJECT: the
+// stage input, projected (enforce_bssn_constraints) before it is stored; a
+// point the projection rejects (det gt < 0) keeps the combined values, as
+// the projection leaves such a node untouched.
 template <bool PROJECT>
 __global__ void __launch_bounds__(384) k_ets_combine(const DevBlock* __restrict__ blocks,
                                                     const DevTile* __restrict__ tiles, int ntiles, Ptrs24 out,
@@ -758,7 +760,7 @@ __global__ void __launch_bounds__(384) k_ets_combine(const DevBlock* __restrict_
 #pragma clang fp contract(off)
         double x = base.v[v][o];
         for (int q = 0; q < T.n; ++q) x = x + T.c[q] * kb[((uint64_t)T.idx[q] * 24 + v) * nacc + co];
-        out.v[v][o] = x;
+        out.v[v][cdof + oc] = x;
       }
     }
   }
@@ -885,10 +887,13 @@ extern "C" int dgr_ets_evolve(dgr_ets_t e, double dt, void* stream) {
     T.idx[k] = k;
   }
   T.n = e->ns;
-  hipLaunchKernelGGL(k_ets_combine<false>, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->state,
-                     e->state, kb, e->nacc, T, p->chi_floor, e->d_flag, (uint64_t)0);
-  hipLaunchKernelGGL(k_ghost_refill, dim3(grid_for(p, (int)p->nblocks)), dim3(RK_NT), 0, st, p->d_blocks, p->d_nbr,
-                     (int)p->nblocks, e->state);
+  // the new state into tmp's compact region (the last RHS has read it),
+  // then into u's interiors and same-level ghosts in one pass
+  hipLaunchKernelGGL(k_ets_combine<false>, dim3(g), dim3(384), 0, st, p->d_blocks, p->d_tiles_blk, nt, e->tmp,
+                     e->state, kb, e->nacc, T, p->chi_floor, e->d_flag, e->dof);
+  const int nb = (int)p->nblocks;
+  hipLaunchKernelGGL(k_hybrid<false>, dim3(grid_xcd(p, nb)), dim3(RK_NT), 0, st, p->d_blocks, p->d_cdelta, nb,
+                     e->state, e->tmp);
   DGR_HIP(hipGetLastError());
   OutPtrs so;
   for (int v = 0; v < 24; ++v) so.v[v] = e->state.v[v];
