@@ -30,6 +30,10 @@ def run(cmd, cwd=None):
 
 SOURCES = ["csrc/dgr_bssn.hip", "csrc/dgr_rk4.hip", "csrc/dgr_levels.hip", "csrc/dgr_init.hip",
            "csrc/dgr_tp.hip", "host/bssn_rhs_host.cpp"]
+# Per-source flags.  The RHS kernels are scheduled with the AMDGPU-specific
+# register-pressure trackers: the same instructions, fewer spill reloads in
+# the staged equations (the 4096-block RHS 1.19 -> 1.13 ms).
+SOURCE_FLAGS = {"csrc/dgr_bssn.hip": ["-mllvm", "-amdgpu-use-amdgpu-trackers"]}
 # the dense TwoPunctures Jacobian is factored with rocSOLVER (on rocBLAS)
 LINK = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 
@@ -53,7 +57,7 @@ def build_hip():
         obj = os.path.join(obj_dir, os.path.basename(rel) + ".o")
         if incremental and os.path.exists(obj) and os.path.getmtime(obj) > max(os.path.getmtime(src), newest_header):
             return obj
-        run([HIPCC] + compile_flags + ["-c", "-o", obj + ".tmp", src])
+        run([HIPCC] + compile_flags + SOURCE_FLAGS.get(rel, []) + ["-c", "-o", obj + ".tmp", src])
         os.replace(obj + ".tmp", obj)
         return obj
 
