@@ -287,6 +287,9 @@ __global__ void __launch_bounds__(384) k_stage(const DevBlock* __restrict__ bloc
 // UNPACK: the interior points, and every ghost that has a same-level
 // neighbour from that neighbour's interior (the same-level refill); the
 // other ghosts are left to the level fills or keep their values.
+// A block none of whose ghosts needs this pass (PACK: every ghost direction
+// has a same-level neighbour or lies beyond a flagged face; UNPACK: no
+// direction has one) is walked over its interior points only.
 // XCD-aware: workgroup w runs on XCD w % 8, so the grid (a multiple of 8)
 // is renumbered to give each XCD a contiguous run of blocks, whose
 // neighbours' interiors the unpack reads from the same L2.
@@ -294,13 +297,31 @@ template <bool PACK>
 __global__ void __launch_bounds__(RK_NT) k_hybrid(const DevBlock* __restrict__ blocks,
                                                   const long long* __restrict__ cdelta, int nblocks, Ptrs24 u,
                                                   Ptrs24 h) {
+  __shared__ int s_ghosts;
   const int first = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
   for (int b = first; b < nblocks; b += gridDim.x) {
     const DevBlock& B = blocks[b];
     const int s0 = B.sz[0], s1 = B.sz[1], s2 = B.sz[2];
-    const int n = s0 * s1 * s2;
-    for (int e = threadIdx.x; e < n; e += RK_NT) {
-      const int i = e % s0, j = (e / s0) % s1, k = e / (s0 * s1);
+    __syncthreads();  // the previous block's s_ghosts has been read
+    if (threadIdx.x == 0) s_ghosts = 0;
+    __syncthreads();
+    if (threadIdx.x < NNBR && threadIdx.x != 13) {
+      const int dir = (int)threadIdx.x, dx = dir % 3, dy = (dir / 3) % 3, dz = dir / 9;
+      const unsigned f = B.bflag;
+      const bool beyond = (dx == 0 && (f & 1u)) || (dx == 2 && (f & 2u)) || (dy == 0 && (f & 4u)) ||
+                          (dy == 2 && (f & 8u)) || (dz == 0 && (f & 16u)) || (dz == 2 && (f & 32u));
+      const long long d = cdelta[(size_t)b * NNBR + dir];
+      if (PACK ? (d == 0 && !beyond) : (d != 0)) s_ghosts = 1;
+    }
+    __syncthreads();
+    const bool all = s_ghosts != 0;
+    const int m0 = s0 - 6, m1 = s1 - 6;
+    const int n = all ? s0 * s1 * s2 : m0 * m1 * (s2 - 6);
+    for (int q = threadIdx.x; q < n; q += RK_NT) {
+      const int i = all ? q % s0 : 3 + q % m0;
+      const int j = all ? (q / s0) % s1 : 3 + (q / m0) % m1;
+      const int k = all ? q / (s0 * s1) : 3 + q / (m0 * m1);
+      const int e = i + s0 * (j + s1 * k);
       const int dx = i < 3 ? 0 : (i >= s0 - 3 ? 2 : 1);
       const int dy = j < 3 ? 0 : (j >= s1 - 3 ? 2 : 1);
       const int dz = k < 3 ? 0 : (k >= s2 - 3 ? 2 : 1);
@@ -317,11 +338,11 @@ __global__ void __launch_bounds__(RK_NT) k_hybrid(const DevBlock* __restrict__ b
         const bool beyond = ((f & 1u) && i < 3) || ((f & 2u) && i >= s0 - 3) || ((f & 4u) && j < 3) ||
                             ((f & 8u) && j >= s1 - 3) || ((f & 16u) && k < 3) || ((f & 32u) && k >= s2 - 3);
         if (dir != 13 && (d != 0 || beyond)) continue;
-        const uint64_t q = dir == 13 ? c : o;
+        const uint64_t to = dir == 13 ? c : o;
 #pragma unroll
         for (int v = 0; v < 24; ++v) t[v] = u.v[v][o];
 #pragma unroll
-        for (int v = 0; v < 24; ++v) h.v[v][q] = t[v];
+        for (int v = 0; v < 24; ++v) h.v[v][to] = t[v];
       } else {
         if (d == 0) continue;
 #pragma unroll
