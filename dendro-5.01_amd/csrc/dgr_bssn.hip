@@ -88,7 +88,8 @@ struct Planes {
 // ---------------------------------------------------------------------------
 __constant__ dgr_forms::Form c_forms42[dgr_forms::F_NFORMS] = DGR_FORMS_TABLE;
 __constant__ dgr_forms::Form c_forms64[dgr_forms::G_NFORMS] = DGR_FORMS64_TABLE;
-constexpr int NFORMS_MAX = dgr_forms::F_NFORMS > dgr_forms::G_NFORMS ? dgr_forms::F_NFORMS : dgr_forms::G_NFORMS;
+constexpr int NFORMS_MAX =
+    (int)dgr_forms::F_NFORMS > (int)dgr_forms::G_NFORMS ? (int)dgr_forms::F_NFORMS : (int)dgr_forms::G_NFORMS;
 
 // The variable arrays are global memory; saying so turns generic (flat)
 // accesses into global ones, which do not hold up waits on LDS reads.

@@ -30,6 +30,7 @@
 #ifndef DGR_STENCIL_FORMS_H
 #define DGR_STENCIL_FORMS_H
 
+#undef DGR_HD  // (re)defined for what follows; bssn_staged.h defines its own
 #ifdef __HIPCC__
 #define DGR_HD __host__ __device__
 #else
