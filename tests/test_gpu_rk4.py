@@ -101,3 +101,42 @@ def test_graph_replays_match_eager_steps(cuda):
         torch.cuda.synchronize()
         assert torch.equal(Ue, Ug), n
     assert rke.status_ok() and rkg.status_ok()
+
+
+@pytest.mark.parametrize("kind", ["uniform", "adaptive"])
+def test_one_call_equals_split_calls(cuda, kind):
+    """Within one call the state lives in the evolver's work memory and is
+    written back (interiors, then the ghost refills) when the call ends: n
+    steps in one call and the same steps over several calls give bitwise
+    the same arrays, padding included; a call of 0 steps changes nothing."""
+    if kind == "uniform":
+        dx = 1.0 / 32
+        blocks, dof = layout.uniform_grid(3, dx=dx)
+        U = data.flat_bumps(blocks, dof, seed=7, amp=1e-2, width=0.25).to(cuda)
+        prm = make_params()
+        dt = 0.25 * dx
+    else:
+        blocks, dof, _ = layout.adaptive_octree(half_width=4.0, base_level=1, levels=4, centers=((0.7, 0.1, -0.2),),
+                                                factor=1.5)
+        U = data.flat_bumps(blocks, dof, seed=8, amp=1e-3, width=0.5, box=((-4.0,) * 3, (4.0,) * 3)).to(cuda)
+        prm = make_params(ko_sigma=0.4, chi_floor=1e-4)
+        dt = 0.25 * min((b["pmax"][0] - b["pmin"][0]) / 12 for b in blocks)
+    plan = Plan(blocks, prm)
+    plan.ghost_refill(U)  # consistent padding to start from
+    U0 = U.clone()
+    Ua, Ub = U.clone(), U.clone()
+    rka = RK4(plan, Ua, use_graph=True)
+    rkb = RK4(plan, Ub, use_graph=True)
+    rka.step(dt, 0)
+    torch.cuda.synchronize()
+    assert torch.equal(Ua, U0)
+    rka.step(dt, 5)
+    for n in (2, 1, 2):
+        rkb.step(dt, n)
+    torch.cuda.synchronize()
+    assert rka.status_ok() and rkb.status_ok()
+    assert torch.isfinite(Ua).all()
+    assert torch.equal(Ua, Ub)
+    assert not torch.equal(Ua, U0)
+    rka.close()
+    rkb.close()
